@@ -1,0 +1,200 @@
+/*
+ * mbs_hip.h -- C ABI of the MI355X (gfx950) physical-optics diffraction engine.
+ *
+ * The engine evaluates, for every prepared beam of every orientation and every
+ * scattering direction (theta, phi) of a grid, the polygon-aperture Fourier
+ * integral, sums the 2x2 Jones matrices of all beams of one orientation
+ * coherently, converts the sum to a 4x4 Mueller matrix and accumulates
+ * weight * Mueller over orientations.  This is the work the reference does in
+ * HandlerPO::HandleBeamsToLocal + HandlerPO::AddToMuellerLocal on the CPU
+ * (src/handler/HandlerPO.cpp:2561-2796, :2888-2907) and in
+ * HandlerPO::HandleOrientationsToLocalGpu on CUDA
+ * (src/cuda/GpuDiffraction.cu:4159-4778).
+ *
+ * Only plain C types cross this boundary: pointers, sizes and POD structs.  All
+ * entry points return an mbs_status; mbs_last_error() returns the message of
+ * the last failure on that context (or of the last context-less failure when
+ * ctx is NULL).  A context is bound to one HIP device and is not safe to use
+ * from two host threads at once; create one context per host thread.
+ */
+#ifndef MBS_HIP_H
+#define MBS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBS_ABI_VERSION 1
+
+typedef enum mbs_status {
+    MBS_OK = 0,
+    MBS_ERR_INVALID = 1,     /* bad argument or inconsistent sizes          */
+    MBS_ERR_BAD_BEAM = 2,    /* beam outside what the kernels accept        */
+    MBS_ERR_DEVICE = 3,      /* HIP runtime error                           */
+    MBS_ERR_NO_GRID = 4,     /* mbs_set_grid has not been called            */
+    MBS_ERR_NO_BEAMS = 5,    /* mbs_upload_beams has not been called        */
+    MBS_ERR_CAPACITY = 6     /* request exceeds a documented maximum        */
+} mbs_status;
+
+/* Columns of one row of mbs_beam_batch.beam_scalars: the scalar fields of the
+ * reference's PreparedBeam (src/handler/HandlerPO.h:14-46), in this order. */
+enum {
+    MBS_B_HOR = 0,    /* aperture horizontal axis (3)        horAx..horAz   */
+    MBS_B_VER = 3,    /* aperture vertical axis (3)          verAx..verAz   */
+    MBS_B_DIR = 6,    /* beam propagation direction (3)      bdx..bdz       */
+    MBS_B_CEN = 9,    /* beam centre (3)                     cenx..cenz     */
+    MBS_B_AREA = 12,  /* beam area                           beam_area      */
+    MBS_B_NT = 13,    /* normal x beamBasis (3)              pNTx..pNTz     */
+    MBS_B_NP = 16,    /* normal x polarisation basis (3)     pNPx..pNPz     */
+    MBS_B_NXDT = 19,  /* normal x (dir x beamBasis) (3)      pnxDTx..       */
+    MBS_B_NXDP = 22,  /* normal x (dir x polarisation) (3)   pnxDPx..       */
+    MBS_B_JP = 25,    /* J_phased, 8 reals: 00r 00i 01r 01i 10r 10i 11r 11i */
+    MBS_B_NORMAL = 33,/* aperture normal (3), informational                 */
+    MBS_BEAM_SCALARS = 36
+};
+
+/* Aperture-edge flags, one byte per vertex/edge (edge e joins vertex e and
+ * vertex (e+1) mod nv), the edge_valid_x / edge_valid_y of BeamEdgeData
+ * (src/handler/Handler.h:372-388). */
+#define MBS_EDGE_VALID_X 1u
+#define MBS_EDGE_VALID_Y 2u
+
+/* Vertex limits of the kernels, the same as the reference GPU path
+ * (GpuDiffraction.cu:4301-4303): 3 <= nv <= MBS_MAX_VERTICES. */
+#define MBS_MIN_VERTICES 3
+#define MBS_MAX_VERTICES 31
+
+/* Prepared beams of a batch of orientations, as flat host arrays.  Replaces
+ * std::vector<PreparedOrientation> (src/handler/HandlerPO.h:49-55) plus the
+ * packing of pack_prepared_gpu_beam (src/cuda/GpuDiffraction.cu:273-358). */
+typedef struct mbs_beam_batch {
+    int32_t n_orient;
+    int32_t n_beams;
+    const int64_t *orient_offsets; /* [n_orient+1] beam ranges per orientation */
+    const double *weights;         /* [n_orient] orientation weight (sinZenith) */
+    const double *beam_scalars;    /* [n_beams * MBS_BEAM_SCALARS]             */
+    const int32_t *beam_nv;        /* [n_beams] aperture vertex count          */
+    const int32_t *beam_external;  /* [n_beams] 1 = external (shadow) beam     */
+    const int64_t *vert_offsets;   /* [n_beams+1] vertex ranges per beam       */
+    const double *vx;              /* [n_vert] aperture-plane x (edgeData.x)   */
+    const double *vy;              /* [n_vert] aperture-plane y (edgeData.y)   */
+    const double *slope_yx;        /* [n_vert] dy/dx of edge e (if valid_x)    */
+    const double *slope_xy;        /* [n_vert] dx/dy of edge e (if valid_y)    */
+    const uint8_t *edge_flags;     /* [n_vert] MBS_EDGE_VALID_X | _Y           */
+} mbs_beam_batch;
+
+/* Scattering grid: n_az phi bins (phi_p = p*az_step) by n_zen+1 theta rows,
+ * the ScatteringRange of src/handler/Handler.h:15-189.  Cell (p, t) is stored
+ * at index p*(n_zen+1)+t in every grid-shaped array. */
+typedef struct mbs_grid {
+    int32_t n_az;
+    int32_t n_zen;
+    const double *theta; /* [n_zen+1] radians                                */
+    double az_step;      /* radians                                           */
+    const double *vf;    /* [n_az*(n_zen+1)*3] polarisation basis vf per cell */
+} mbs_grid;
+
+/* Physical constants of the handler (src/handler/Handler.cpp:102-126). */
+typedef struct mbs_consts {
+    double wave_index;        /* k = 2*pi/lambda                              */
+    double eps1;              /* edge-coefficient threshold (1e9*DBL_EPSILON) */
+    double compl_wave[2];     /* -i*lambda/(4*pi^2)                           */
+    double inv_compl_wave[2]; /* i/lambda                                     */
+    int32_t legacy_sign;      /* m_legacySign                                 */
+    int32_t reserved;
+} mbs_consts;
+
+/* Flags of mbs_diffract. */
+#define MBS_PREC_FP64 0u      /* FP64 throughout                                */
+#define MBS_PREC_MIXED 1u     /* FP64 phase/moments, FP32 elsewhere             */
+#define MBS_NOSHADOW 4u       /* also accumulate the no-shadow Mueller grid     */
+#define MBS_PER_ORIENT 8u     /* also keep one Mueller grid per orientation     */
+
+typedef struct mbs_ctx mbs_ctx;
+
+/* Library version and build information. */
+int mbs_abi_version(void);
+const char *mbs_build_info(void);
+
+/* ---- GPU runtime queries: replace src/cuda/GpuSupport.h:30-34 ---------- */
+typedef struct mbs_device_info {
+    int32_t device_id;
+    int32_t visible_devices;
+    int64_t total_mem;
+    int64_t free_mem;
+    int32_t compute_units;
+    int32_t clock_khz;
+    int32_t runtime_version;
+    int32_t driver_version;
+    char name[128];
+    char arch[64];
+} mbs_device_info;
+
+/* CheckGpuRuntime (GpuSupport.h:30): fills *info for the current device. */
+int mbs_check_runtime(mbs_device_info *info);
+/* QueryActiveGpuMemory (GpuSupport.h:31). */
+int mbs_query_memory(int64_t *free_bytes, int64_t *total_bytes);
+/* VisibleGpuDeviceCount (GpuSupport.h:33); 0 when no device is usable. */
+int mbs_visible_device_count(void);
+
+/* ---- contexts ------------------------------------------------------------ */
+/* Creates a context on HIP device `device` with its own non-blocking stream. */
+int mbs_ctx_create(int device, mbs_ctx **out);
+int mbs_ctx_destroy(mbs_ctx *ctx);
+const char *mbs_last_error(const mbs_ctx *ctx);
+
+/* Uploads the direction grid (tables are derived on the device).  The grid
+ * stays resident until replaced. */
+int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *grid);
+
+/* Uploads one batch of orientations' prepared beams and repacks them on the
+ * device.  Rejects beams with nv outside [MBS_MIN_VERTICES, MBS_MAX_VERTICES]
+ * with MBS_ERR_BAD_BEAM, as the reference GPU path does. */
+int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *batch,
+                     const mbs_consts *consts);
+
+/* Evaluates the uploaded batch on the current grid and leaves the result on
+ * the device: the weighted Mueller sum over the batch's orientations (and
+ * the no-shadow sum / per-orientation grids when flagged).  Asynchronous on
+ * the context's stream. */
+int mbs_diffract(mbs_ctx *ctx, uint32_t flags);
+
+/* Copies the device result of the last mbs_diffract to the host and ADDS it
+ * to m_full / m_noshadow ([n_az*(n_zen+1)*16], row-major 4x4 per cell), the
+ * accumulate semantics of HandleOrientationsToLocalGpu.  m_noshadow may be
+ * NULL.  Synchronises the context's stream. */
+int mbs_accumulate_mueller(mbs_ctx *ctx, double *m_full, double *m_noshadow);
+
+/* Copies the per-orientation grids of the last mbs_diffract (flag
+ * MBS_PER_ORIENT) to out ([n_orient*n_az*(n_zen+1)*16]), each multiplied by
+ * its orientation weight. */
+int mbs_fetch_per_orient(mbs_ctx *ctx, double *out);
+
+/* One-call convenience: upload, diffract, accumulate. */
+int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid,
+                       const mbs_beam_batch *batch, const mbs_consts *consts,
+                       uint32_t flags, double *m_full, double *m_noshadow);
+
+/* Device pointer of the result grid of the last mbs_diffract (FP64,
+ * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
+int mbs_device_result(mbs_ctx *ctx, const double **m_full,
+                      const double **m_noshadow);
+
+/* The HIP stream the context launches on (a hipStream_t, as void*), for
+ * timing with events recorded on the same stream. */
+void *mbs_ctx_stream(mbs_ctx *ctx);
+
+/* Statistics of the uploaded batch: total beams, beam.direction evaluation
+ * units (sum of beams x directions), histogram of vertex counts
+ * (hist[nv], 32 entries), histogram of valid edges per beam. */
+int mbs_batch_stats(mbs_ctx *ctx, int64_t *n_beams, double *units,
+                    int64_t *nv_hist32, double *mean_edges);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBS_HIP_H */

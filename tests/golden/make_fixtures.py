@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of the PO diffraction hot path.
+
+Runs only in the container that holds the reference sources (it needs the
+binaries that ``make -f oracle/ref.mk`` builds into ``oracle/_ref``).  For each
+case it
+
+1. runs the unmodified reference CLI (``oracle/_ref/mbs_po``) and keeps its
+   Mueller output file;
+2. runs the fixture writer (``oracle/_ref/ref_fixed_harness`` for a fixed
+   orientation, ``oracle/_ref/mbs_po_dump`` for an orientation average), which
+   serialises the reference's own ``PreparedOrientation`` batches together with
+   the Mueller grid the reference's CPU routines compute from them;
+3. checks that (2) reproduces (1), so the dumped beams are the reference's
+   beams;
+4. stores inputs and expected outputs as a compressed ``.npz`` beside this
+   script.
+
+Fixture layout (all little-endian, FP64 unless noted)::
+
+    theta[nZen+1] (rad), az_step, vf[nAz, nZen+1, 3]
+    wave_index, eps1, compl_wave[2], inv_compl_wave[2], legacy_sign (int)
+    orient_offsets[nOrient+1] (int64), weights[nOrient]
+    beam_scalars[nBeams, 36], beam_nv, beam_external, beam_valid (int32)
+    vert_offsets[nBeams+1] (int64), vx, vy, slope_yx, slope_xy, edge_flags (u8)
+    mueller_full[nAz, nZen+1, 16], mueller_noshadow[nAz, nZen+1, 16]
+
+``beam_scalars`` columns: hor(3) ver(3) dir(3) cen(3) area NT(3) NP(3) nxDT(3)
+nxDP(3) J_phased(8: 00r 00i 01r 01i 10r 10i 11r 11i) normal(3) -- the fields of
+the reference's PreparedBeam (src/handler/HandlerPO.h:14-46).
+"""
+import os
+import shutil
+import struct
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF_BIN = os.path.join(REPO, "oracle", "_ref")
+WORK = os.path.join(REPO, "_work", "fixtures")
+
+
+class _Reader:
+    def __init__(self, data):
+        self.d = data
+        self.o = 0
+
+    def take(self, n):
+        b = self.d[self.o:self.o + n]
+        if len(b) != n:
+            raise EOFError
+        self.o += n
+        return b
+
+    def i32(self):
+        return struct.unpack("<i", self.take(4))[0]
+
+    def f64(self, n=None):
+        if n is None:
+            return struct.unpack("<d", self.take(8))[0]
+        return np.frombuffer(self.take(8 * n), dtype="<f8").copy()
+
+    def tag(self):
+        return self.take(4).decode()
+
+    def done(self):
+        return self.o >= len(self.d)
+
+
+def read_dump(path):
+    """Parses the stream written by ref_dump_shim.cpp into numpy arrays."""
+    r = _Reader(open(path, "rb").read())
+    assert r.take(8) == b"MBSFIX01"
+    n_az = r.i32()
+    n_zen = r.i32()
+    out = {"theta": r.f64(n_zen + 1), "az_step": r.f64()}
+    out["vf"] = r.f64(n_az * (n_zen + 1) * 3).reshape(n_az, n_zen + 1, 3)
+    out["wave_index"] = r.f64()
+    out["eps1"] = r.f64()
+    out["compl_wave"] = r.f64(2)
+    out["inv_compl_wave"] = r.f64(2)
+    out["legacy_sign"] = r.i32()
+    r.i32()  # no-shadow flag of the handler (informational)
+    weights, orient_beams = [], []
+    scalars, nv, ext, valid = [], [], [], []
+    vx, vy, syx, sxy, flags = [], [], [], [], []
+    m_full = np.zeros((n_az, n_zen + 1, 16))
+    m_ns = np.zeros((n_az, n_zen + 1, 16))
+    while not r.done():
+        tag = r.tag()
+        assert tag == "BTCH", tag
+        n_or = r.i32()
+        for _ in range(n_or):
+            assert r.tag() == "ORNT"
+            weights.append(r.f64())
+            nb = r.i32()
+            orient_beams.append(nb)
+            for _ in range(nb):
+                n = r.i32()
+                nv.append(n)
+                ext.append(r.i32())
+                valid.append(r.i32())
+                scalars.append(r.f64(36))
+                for _ in range(n):
+                    x, y, a, b = r.f64(4)
+                    vx.append(x)
+                    vy.append(y)
+                    syx.append(a)
+                    sxy.append(b)
+                    flags.append(r.take(1)[0])
+        assert r.tag() == "MFUL"
+        m_full += r.f64(n_az * (n_zen + 1) * 16).reshape(n_az, n_zen + 1, 16)
+        assert r.tag() == "MNSH"
+        m_ns += r.f64(n_az * (n_zen + 1) * 16).reshape(n_az, n_zen + 1, 16)
+    out["orient_offsets"] = np.concatenate([[0], np.cumsum(orient_beams)]).astype(np.int64)
+    out["weights"] = np.array(weights)
+    out["beam_scalars"] = np.array(scalars).reshape(-1, 36)
+    out["beam_nv"] = np.array(nv, dtype=np.int32)
+    out["beam_external"] = np.array(ext, dtype=np.int32)
+    out["beam_valid"] = np.array(valid, dtype=np.int32)
+    out["vert_offsets"] = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+    out["vx"] = np.array(vx)
+    out["vy"] = np.array(vy)
+    out["slope_yx"] = np.array(syx)
+    out["slope_xy"] = np.array(sxy)
+    out["edge_flags"] = np.array(flags, dtype=np.uint8)
+    out["mueller_full"] = m_full
+    out["mueller_noshadow"] = m_ns
+    return out
+
+
+def read_fixed_dat(path, n_az, n_zen):
+    """Reads the per-direction .dat of the fixed path (HandlerPO.cpp:117-153):
+    header 'nZen nAz', then rows 'theta phi energy M11..M44' ordered t-major."""
+    rows = np.loadtxt(path, skiprows=1)
+    assert rows.shape == ((n_zen + 1) * n_az, 19), rows.shape
+    m = rows[:, 3:].reshape(n_zen + 1, n_az, 16)
+    return np.transpose(m, (1, 0, 2)).copy(), rows[:, 0].reshape(n_zen + 1, n_az)[:, 0]
+
+
+def read_total_dat(path):
+    """Reads the azimuth-averaged .dat (HandlerPOTotal.cpp:23-122):
+    header line, then rows 'ScAngle 2pi*dcos M11..M44'."""
+    return np.loadtxt(path, skiprows=1)
+
+
+def run(cmd, cwd, env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    p = subprocess.run(cmd, cwd=cwd, env=e, stdin=subprocess.DEVNULL,
+                       stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    if p.returncode != 0:
+        sys.stderr.write(p.stdout[-4000:])
+        raise SystemExit("command failed: %s" % " ".join(cmd))
+    return p.stdout
+
+
+def rel_err(a, b):
+    """max |a-b| / max(|M11(b)|) per cell, over all 16 elements."""
+    scale = np.abs(b[..., :1])
+    scale = np.where(scale > 0, scale, 1.0)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def fresh_dir(name):
+    # The reference appends to its .dat files, so every case starts empty.
+    wd = os.path.join(WORK, name)
+    if os.path.isdir(wd):
+        shutil.rmtree(wd)
+    os.makedirs(wd)
+    return wd
+
+
+def fixed_case(name, L, D, n_re, n_im, wave, refl, beta, gamma, th0, th1, nphi, nth,
+               keep_rows=None):
+    wd = fresh_dir(name)
+    args = ["--method", "po", "--particle", "1", str(L), str(D),
+            "--refractive-index", str(n_re), str(n_im), "--wavelength-um", str(wave),
+            "--max-reflections", str(refl), "--fixed-orientation", str(beta), str(gamma),
+            "--scattering-grid", str(th0), str(th1), str(nphi), str(nth),
+            "--backend", "cpu", "--cutoff-profile", "off", "--threads", "8",
+            "--close", "--output", "ref"]
+    run([os.path.join(REF_BIN, "mbs_po")] + args, wd)
+    dump = os.path.join(wd, "dump.bin")
+    run([os.path.join(REF_BIN, "ref_fixed_harness"), str(L), str(D), str(n_re), str(n_im),
+         str(wave), str(refl), str(beta), str(gamma), str(th0), str(th1), str(nphi),
+         str(nth)], wd, {"MBS_FIXTURE_DUMP": dump})
+    fx = read_dump(dump)
+    dat_path = os.path.join(wd, "ref", "ref.dat")
+    m_dat, theta_deg = read_fixed_dat(dat_path, nphi, nth)
+    err = rel_err(fx["mueller_full"], m_dat)
+    print("%s: %d beams, harness(HandleBeamsToLocal) vs reference .dat(HandleBeams): "
+          "max|dM|/M11 = %.3e" % (name, len(fx["beam_nv"]), err))
+    assert err < 1e-11, "harness does not reproduce the reference fixed path"
+    fx["mueller_fixed_dat"] = m_dat
+    fx["case"] = np.array(name)
+    if keep_rows is not None:
+        fx["kept_theta_rows"] = np.array(keep_rows, dtype=np.int64)
+        for k in ("mueller_full", "mueller_noshadow", "mueller_fixed_dat"):
+            fx[k] = fx[k][:, keep_rows, :]
+    return fx
+
+
+def sobol_case(name, L, D, n_re, n_im, wave, refl, n_orient, seed, nphi, nth,
+               th0=0, th1=180, tgrid=None, chunk=None):
+    wd = fresh_dir(name)
+    if tgrid is not None:
+        with open(os.path.join(wd, "theta.grid"), "w") as f:
+            f.write("\n".join("%.10g" % t for t in tgrid) + "\n")
+        grid = ["--theta-grid-file", "theta.grid", "--phi-points", str(nphi)]
+    else:
+        grid = ["--scattering-grid", str(th0), str(th1), str(nphi), str(nth)]
+    args = ["--method", "po", "--particle", "1", str(L), str(D),
+            "--refractive-index", str(n_re), str(n_im), "--wavelength-um", str(wave),
+            "--max-reflections", str(refl), "--sobol-seed", str(n_orient), str(seed)] + grid + [
+            "--cutoff-profile", "off", "--threads", "8", "--no-shadow-output", "--close"]
+    if chunk:
+        args += ["--orientation-chunk", str(chunk)]
+    run([os.path.join(REF_BIN, "mbs_po")] + args + ["--backend", "cpu", "--output", "cpu"], wd)
+    dump = os.path.join(wd, "dump.bin")
+    run([os.path.join(REF_BIN, "mbs_po_dump")] + args + ["--backend", "gpu", "--output", "dmp"],
+        wd, {"MBS_FIXTURE_DUMP": dump})
+    fx = read_dump(dump)
+    cpu = read_total_dat(os.path.join(wd, "cpu", "cpu.dat"))
+    dmp = read_total_dat(os.path.join(wd, "dmp", "dmp.dat"))
+    cpu_ns = read_total_dat(os.path.join(wd, "cpu", "cpu_noshadow.dat"))
+    scale = np.abs(cpu[:, 2:3])
+    err = float(np.max(np.abs(dmp[:, 2:] - cpu[:, 2:]) / scale))
+    print("%s: %d orientations, %d beams (min %d max %d per orientation); "
+          "dump-path vs CPU-backend azimuth-averaged .dat: max|dM|/M11 = %.3e"
+          % (name, len(fx["weights"]), len(fx["beam_nv"]),
+             np.diff(fx["orient_offsets"]).min(), np.diff(fx["orient_offsets"]).max(), err))
+    assert err < 1e-11, "dumped batches do not reproduce the reference CPU backend"
+    fx["ref_total_dat"] = cpu
+    fx["ref_total_dat_noshadow"] = cpu_ns
+    fx["case"] = np.array(name)
+    return fx
+
+
+def save(name, fx):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **fx)
+    print("wrote %s (%.1f KiB)" % (path, os.path.getsize(path) / 1024.0))
+
+
+def main():
+    os.makedirs(WORK, exist_ok=True)
+    # Configuration 1: hexagonal column L=20 D=10 um, m=1.3116, lambda=0.532,
+    # 8 reflections, beta=30 gamma=10, theta 0..180 in 720 steps x 48 phi.
+    # Inputs are kept whole; the expected grid every 8th theta row (+ last).
+    rows = list(range(0, 721, 8)) + [720]
+    save("config1_fixed", fixed_case("config1_fixed", 20, 10, 1.3116, 0.0, 0.532, 8,
+                                     30, 10, 0, 180, 48, 720, keep_rows=rows))
+    # Absorbing fixed case (complex refractive index; exercises the absorption
+    # factors folded into J_phased), small grid.
+    save("fixed_absorbing", fixed_case("fixed_absorbing", 12, 8, 1.3116, 0.01, 0.532, 6,
+                                       47, 23, 0, 180, 16, 90))
+    # Orientation average: reference Owen-Sobol (seed 1) orientations of the
+    # configuration-2 particle shape scaled down (L=20 D=10 um) with absorption,
+    # the non-uniform theta grid of configuration 2 on 24 phi bins.
+    tg = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
+    save("sobol_average", sobol_case("sobol_average", 20, 10, 1.3116, 1e-4, 0.532, 6,
+                                     8, 1, 24, 540, tgrid=tg))
+
+
+if __name__ == "__main__":
+    main()
