@@ -1,0 +1,278 @@
+#!/usr/bin/env python3
+"""Throughput of the PO diffraction hot path on one MI355X.
+
+Metric: beam.direction evaluations per second -- one unit is one (beam,
+theta, phi) evaluation of the aperture integral, Jones rotation and coherent
+sum, counted as sum over orientations of beams x directions.
+
+Workload (the orientation-average configuration): hexagonal ice column
+D = 50 um, L = 100 um, lambda = 0.532 um, m = 1.3116 + 1e-4 i; grid of 541
+theta rows (0-10 deg at 0.05 deg, 10.5-180 deg at 0.5 deg) x 96 phi; FP32
+storage with FP64 phase arguments and polygon moments (the mixed kernel), the
+FP64 kernel reported beside it.  One step = one pass of the hot path over one
+resident batch of ORIENT_PER_STEP orientations, each with a beam count drawn
+uniformly from 2000-15000.  Beams are synthetic: resampled from the prepared
+beams the reference's tracer produced for a D = 10 um column of the same
+shape, rigidly rotated and scaled by 5 (mbs-fast_amd/synthetic.py).
+
+Timing: inputs are uploaded once before the timed region; each step is one
+mbs_diffract (main kernel + deterministic reduction).  The roofline figure
+uses the main kernel's duration measured with HIP events recorded on the
+engine's own stream (mbs_get_timing).
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one
+GPU; the orientation sharding across GPUs is described in DESIGN.md.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ORIENT_PER_STEP = 16
+BEAMS_MIN, BEAMS_MAX = 2000, 15000
+FP64_PEAK = 78.6e12   # MI355X FP64 vector (= FP64 MFMA) FLOP/s
+
+
+def unit_flops(batch):
+    """Algorithmic FLOPs of one (beam, direction) unit per beam:
+    169 + 4 nv + 7 nE (FMA = 2, add/mul/rcp = 1), sincos and reciprocals
+    excluded; nE = mean of the two valid-edge counts (the lane uses one set,
+    chosen by |A| vs |B|)."""
+    nv = batch.beam_nv.astype(np.float64)
+    f = batch.edge_flags
+    ex = np.add.reduceat((f & 1).astype(np.float64), batch.vert_offsets[:-1])
+    ey = np.add.reduceat(((f >> 1) & 1).astype(np.float64), batch.vert_offsets[:-1])
+    ne = 0.5 * (ex + ey)
+    return 169.0 + 4.0 * nv + 7.0 * ne, ne
+
+
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _ref_cmd(binary, n_orient, threads, out, tgrid):
+    return [os.path.join(REF_DIR, binary), "--method", "po", "--particle", "1", "100", "50",
+            "--refractive-index", "1.3116", "1e-4", "--wavelength-um", "0.532",
+            "--max-reflections", "10", "--sobol-seed", str(n_orient), "1",
+            "--theta-grid-file", tgrid, "--phi-points", "96", "--cutoff-profile", "off",
+            "--threads", str(threads), "--close", "--output", out]
+
+
+def cpu_baseline_reference(budget_s=12.0):
+    """Times the reference's own CPU double path (oracle/_ref/mbs_po, built
+    from the reference sources with AVX2/FMA, OpenMP on all host threads) on
+    the same particle and grid: Sobol orientations (seed 1) of the D=50 um
+    column, phase-2 (diffraction) seconds as the reference prints them
+    (src/scattering/TracerPOTotal.cpp:6023-6025).  Units come from the
+    reference's own prepared beams of the same command, dumped by
+    oracle/_ref/mbs_po_dump."""
+    import re
+    import shutil
+    import subprocess
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fixtures
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1
+    work = tempfile.mkdtemp(prefix="mbs_cpu_")
+    try:
+        th = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
+        tgrid = os.path.join(work, "theta.grid")
+        with open(tgrid, "w") as f:
+            f.write("\n".join("%.10g" % t for t in th) + "\n")
+        n = max(threads, 4)
+        while True:
+            d = os.path.join(work, "n%d" % n)
+            os.makedirs(d)
+            p = subprocess.run(_ref_cmd("mbs_po", n, threads, "cpu", tgrid) + ["--backend", "cpu"],
+                               cwd=d, stdin=subprocess.DEVNULL, capture_output=True, text=True,
+                               timeout=600)
+            m = re.search(r"Phase 2 \(diffraction, OpenMP\): ([0-9.]+) s", p.stdout)
+            if p.returncode != 0 or not m:
+                raise RuntimeError("reference CPU run failed (rc %d): %s %s"
+                                   % (p.returncode, p.stdout[-300:], p.stderr[-300:]))
+            t2 = float(m.group(1))
+            if t2 >= budget_s / 3 or n >= 256:
+                break
+            n = min(256, int(n * max(2.0, budget_s / 2 / max(t2, 0.05))))
+        dump = os.path.join(d, "dump.bin")
+        env = dict(os.environ, MBS_FIXTURE_DUMP=dump, MBS_FIXTURE_NO_EXPECTED="1")
+        subprocess.run(_ref_cmd("mbs_po_dump", n, threads, "dmp", tgrid) + ["--backend", "gpu"],
+                       cwd=d, stdin=subprocess.DEVNULL, capture_output=True, env=env,
+                       timeout=900, check=True)
+        fx = make_fixtures.read_dump(dump)
+        beams = int(fx["orient_offsets"][-1])
+        units = beams * 541 * 96
+        return {"value": units / t2, "unit": "beam.dir evals/s", "cores": threads,
+                "kind": "reference",
+                "sample": "reference CLI, %d Sobol orientations of the D=50 um column "
+                          "(%d beams, %.1f per orientation) x 541x96 directions = %.3g units; "
+                          "phase-2 (diffraction) %.2f s" % (n, beams, beams / n, units, t2)}
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def cpu_baseline(pkg, batch, grid, consts, budget_s=12.0):
+    """Times the FP64 CPU oracle (OpenMP, all host threads) on a bounded
+    sample: the first beams of the batch's first orientation on the full grid,
+    growing the sample until one run takes ~1/3 of the budget."""
+    from oracle import oracle
+    n = 250
+    last = None
+    spent = 0.0
+    while True:
+        sub = pkg.synthetic.take_beams(batch, 0, n)
+        t0 = time.perf_counter()
+        oracle.diffract(grid, sub, consts)
+        dt = time.perf_counter() - t0
+        spent += dt
+        last = (n, dt)
+        if dt > budget_s / 3 or spent > budget_s or n >= batch.n_beams:
+            break
+        n = min(batch.n_beams, int(n * max(2.0, (budget_s / 3) / max(dt, 1e-3))))
+    n, dt = last
+    units = n * grid.cells
+    return {"value": units / dt, "unit": "beam.dir evals/s", "cores": oracle.threads(),
+            "kind": "port",
+            "sample": "%d beams of one orientation x %d directions (%.3g units), FP64, "
+                      "%.2f s" % (n, grid.cells, units, dt)}
+
+
+class Hip:
+    """Minimal HIP runtime access for device synchronisation."""
+
+    def __init__(self):
+        import ctypes
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.L = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        else:
+            raise RuntimeError("libamdhip64.so not found")
+
+    def sync(self):
+        rc = self.L.hipDeviceSynchronize()
+        if rc != 0:
+            raise RuntimeError("hipDeviceSynchronize: %d" % rc)
+
+
+def run(args):
+    import mbs_loader
+    pkg = mbs_loader.load()
+    abi, syn, eng = pkg.abi, pkg.synthetic, pkg.engine
+    fx = dict(np.load(os.path.join(ROOT, "tests", "golden", "sobol_average.npz")))
+    src = abi.BeamBatch.from_fixture(fx)
+    consts = abi.consts_from_fixture(fx)
+    rng = np.random.default_rng(args.seed)
+    counts = rng.integers(BEAMS_MIN, BEAMS_MAX + 1, ORIENT_PER_STEP)
+    batch = syn.resample(src, counts, scale=5.0, seed=args.seed)
+    grid = syn.config2_grid()
+    flops_b, ne = unit_flops(batch)
+    flops_launch = float(flops_b.sum()) * grid.cells
+    units = float(batch.n_beams) * grid.cells
+    hip = Hip()
+
+    ctx = eng.Context(0)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_timing(True)
+    results = {}
+    for name, flags in (("mixed", abi.PREC_MIXED), ("fp64", abi.PREC_FP64)):
+        for _ in range(args.warmup):
+            ctx.diffract(flags)
+        hip.sync()
+        kernel_ms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.diffract(flags)
+            kernel_ms.append(ctx.timing()["kernel_ms"])  # waits for this step
+        hip.sync()
+        wall = time.perf_counter() - t0
+        kms = float(np.mean(kernel_ms))
+        tm = ctx.timing()
+        results[name] = {
+            "value": units * args.steps / wall,
+            "ms_per_step": 1e3 * wall / args.steps,
+            "kernel_ms": kms,
+            "roofline": {"bound": "mfma", "achieved": flops_launch / (kms * 1e-3) / 1e12,
+                         "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                         "frac": flops_launch / (kms * 1e-3) / FP64_PEAK, "traffic": None},
+            "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
+        }
+    ctx.close()
+    hist = np.bincount(batch.beam_nv, minlength=32)
+    out = {
+        "metric": "beam.dir evals/s",
+        "value": results["mixed"]["value"],
+        "unit": "beam.dir evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": results["mixed"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64-phase",
+        "data": "synthetic: beams resampled from the reference tracer's prepared beams "
+                "(D=10 um column) rotated and scaled to D=50 um",
+        "config": {
+            "workload": "orientation average, hexagonal column D=50 um L=100 um, "
+                        "lambda=0.532 um, m=1.3116+1e-4i, 541 theta x 96 phi",
+            "orientations_per_step": ORIENT_PER_STEP,
+            "beams_per_step": int(batch.n_beams),
+            "beams_per_orientation": "U(%d,%d)" % (BEAMS_MIN, BEAMS_MAX),
+            "directions": grid.cells,
+            "units_per_step": units,
+            "vertex_histogram": {str(k): int(v) for k, v in enumerate(hist) if v},
+            "mean_valid_edges": float(ne.mean()),
+            "flops_per_unit_mean": float(flops_b.mean()),
+        },
+        "kernel_ms": results["mixed"]["kernel_ms"],
+        "roofline": results["mixed"]["roofline"],
+        "fp64": {k: results["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms",
+                                                 "roofline")},
+    }
+    if not args.no_cpu:
+        port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)
+        ref = None
+        if os.path.exists(os.path.join(REF_DIR, "mbs_po")) and \
+                os.path.exists(os.path.join(REF_DIR, "mbs_po_dump")):
+            try:
+                ref = cpu_baseline_reference(args.cpu_budget)
+            except Exception as e:  # the oracle port stays the baseline
+                port["reference_error"] = str(e)
+        out["cpu_baseline"] = ref if ref is not None else port
+        if ref is not None:
+            out["cpu_oracle_port"] = port
+        out["gpu_over_cpu_fp64"] = results["fp64"]["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU time for the baseline sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (orientations "
+              "would shard across GPUs with one sum-reduce of the Mueller grid; see "
+              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        sys.exit("--steps must be >= 1 and --warmup >= 0")
+    run(args)
+
+
+if __name__ == "__main__":
+    main()

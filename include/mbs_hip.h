@@ -109,7 +109,12 @@ typedef struct mbs_consts {
 
 /* Flags of mbs_diffract. */
 #define MBS_PREC_FP64 0u      /* FP64 throughout                                */
-#define MBS_PREC_MIXED 1u     /* FP64 phase/moments, FP32 elsewhere             */
+#define MBS_PREC_MIXED 1u     /* FP64 phase arguments, their reduction, edge
+                                 coefficients and polygon moments; FP32
+                                 phasors, edge sums, rotation and Jones sums.
+                                 Edges with |C| <= max(eps1, 3e-5) take the
+                                 exact half-angle form in FP64, as the
+                                 reference FP32 build does                    */
 #define MBS_NOSHADOW 4u       /* also accumulate the no-shadow Mueller grid     */
 #define MBS_PER_ORIENT 8u     /* also keep one Mueller grid per orientation     */
 
@@ -182,6 +187,34 @@ int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
 int mbs_device_result(mbs_ctx *ctx, const double **m_full,
                       const double **m_noshadow);
+
+/* Stage timing, the counterpart of the reference's MBS_GPU_TIMING breakdown
+ * (src/cuda/GpuDiffraction.cu:550-554, :4767-4775).  When enabled, HIP
+ * events on the context's stream bracket the upload+pack of
+ * mbs_upload_beams and the main kernel and the reduction of mbs_diffract;
+ * mbs_get_timing waits for them and reports milliseconds of the last call of
+ * each. */
+typedef struct mbs_timing {
+    float upload_ms;   /* H2D copies + device packing of the last upload   */
+    float kernel_ms;   /* main diffraction kernel of the last mbs_diffract */
+    float reduce_ms;   /* orientation/segment reduction of the same call   */
+    int32_t n_items;   /* work items of the last mbs_diffract              */
+    int32_t n_blocks;  /* workgroups launched by it                        */
+} mbs_timing;
+int mbs_set_timing(mbs_ctx *ctx, int enable);
+int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out);
+
+/* Tuning: approximate number of workgroups one mbs_diffract launch aims
+ * for (default 4096).  Work items are sized from it. */
+int mbs_set_target_blocks(mbs_ctx *ctx, int blocks);
+
+/* Work decomposition of mbs_diffract: 0 = automatic (beam segments of equal
+ * size unless their partial Jones grids exceed 8 GiB), 1 = runs of whole
+ * orientations, 2 = beam segments.  Results agree to rounding. */
+#define MBS_WORK_AUTO 0
+#define MBS_WORK_ORIENTATIONS 1
+#define MBS_WORK_SEGMENTS 2
+int mbs_set_work_mode(mbs_ctx *ctx, int mode);
 
 /* The HIP stream the context launches on (a hipStream_t, as void*), for
  * timing with events recorded on the same stream. */

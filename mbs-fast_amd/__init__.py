@@ -9,11 +9,11 @@ module name ``mbs_fast_amd``.
 """
 from . import abi  # noqa: F401
 
-__all__ = ["abi", "engine", "handler"]
+__all__ = ["abi", "engine", "handler", "synthetic"]
 
 
 def __getattr__(name):
-    if name in ("engine", "handler", "build"):
+    if name in ("engine", "handler", "build", "synthetic"):
         import importlib
         return importlib.import_module(__name__ + "." + name)
     raise AttributeError(name)

@@ -65,6 +65,11 @@ class mbs_device_info(C.Structure):
                 ("name", C.c_char * 128), ("arch", C.c_char * 64)]
 
 
+class mbs_timing(C.Structure):
+    _fields_ = [("upload_ms", C.c_float), ("kernel_ms", C.c_float),
+                ("reduce_ms", C.c_float), ("n_items", C.c_int32), ("n_blocks", C.c_int32)]
+
+
 def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 

@@ -1,0 +1,749 @@
+// mbs_engine.hip -- C ABI of the gfx950 PO diffraction engine (include/mbs_hip.h).
+//
+// Host side: device buffers, beam upload and packing, work decomposition and
+// kernel launches.  Replaces HandlerPO::HandleOrientationsToLocalGpu and its
+// helpers in src/cuda/GpuDiffraction.cu:4159-4778 (count/pack, buffer
+// management, kernel selection, D2H, host add) with a design of its own:
+// packing runs on the device, the orientation reduction is deterministic
+// (per-item partials summed in fixed order), and work items are sized so a
+// launch always has thousands of workgroups whatever the orientation count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mbs_hip.h"
+#include "po_kernels.hip"
+
+using namespace mbs;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= cap)
+            return hipSuccess;
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(n, 1);
+        hipError_t e = hipMalloc((void **)&p, want * sizeof(T));
+        if (e == hipSuccess)
+            cap = want;
+        return e;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct mbs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // grid
+    bool grid_ok = false;
+    int n_az = 0, n_zen1 = 0, cells = 0, tab_mode = -1;
+    double az_step = 0;
+    DevBuf<double> theta, vf, tab;
+    // beams
+    bool beams_ok = false;
+    int n_orient = 0, n_beams = 0;
+    int64_t n_vert = 0;
+    std::vector<int64_t> h_ooff;
+    std::vector<double> h_weights;
+    int64_t nv_hist[32] = {0};
+    double mean_edges = 0;
+    mbs_consts consts{};
+    DevBuf<double> bs_in, vx_in, vy_in, syx_in, sxy_in, weights;
+    DevBuf<int32_t> nv, ext, vflag;
+    DevBuf<int64_t> voff, ooff;
+    DevBuf<uint8_t> ef_in;
+    DevBuf<double> hdr, vtx;
+    DevBuf<float> hdr32;
+    // work and results
+    DevBuf<WorkItem> items;
+    DevBuf<double> part, part_ns, res, res_ns, per_orient;
+    std::vector<WorkItem> h_items;
+    bool have_result = false, have_ns = false, have_per = false;
+    double *pinned = nullptr;
+    size_t pinned_cap = 0;
+    int target_blocks = 4096;
+    int work_mode = 0;  // 0 auto, 1 whole orientations, 2 beam segments
+    // stage timing
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool ev_up = false, ev_diff = false;
+    int last_items = 0, last_blocks = 0;
+};
+
+static int fail(mbs_ctx *ctx, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx)
+        ctx->err = buf;
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                        \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return fail((ctx), MBS_ERR_DEVICE, "%s failed: %s", #call,            \
+                        hipGetErrorString(e_));                                   \
+    } while (0)
+
+extern "C" {
+
+int mbs_abi_version(void) { return MBS_ABI_VERSION; }
+
+const char *mbs_build_info(void)
+{
+    return "mbs_hip gfx950; FP64 + mixed FP32/FP64-phase kernels; "
+           "deterministic orientation reduction";
+}
+
+int mbs_visible_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+int mbs_check_runtime(mbs_device_info *info)
+{
+    if (!info)
+        return fail(nullptr, MBS_ERR_INVALID, "info is NULL");
+    std::memset(info, 0, sizeof *info);
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(nullptr, MBS_ERR_DEVICE, "no HIP device visible (%s)",
+                    hipGetErrorString(e));
+    int dev = 0;
+    HIP_TRY(nullptr, hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(nullptr, hipGetDeviceProperties(&prop, dev));
+    size_t fr = 0, tot = 0;
+    HIP_TRY(nullptr, hipMemGetInfo(&fr, &tot));
+    info->device_id = dev;
+    info->visible_devices = n;
+    info->total_mem = (int64_t)tot;
+    info->free_mem = (int64_t)fr;
+    info->compute_units = prop.multiProcessorCount;
+    info->clock_khz = prop.clockRate;
+    int rv = 0, dv = 0;
+    (void)hipRuntimeGetVersion(&rv);
+    (void)hipDriverGetVersion(&dv);
+    info->runtime_version = rv;
+    info->driver_version = dv;
+    std::snprintf(info->name, sizeof info->name, "%s", prop.name);
+    std::snprintf(info->arch, sizeof info->arch, "%s", prop.gcnArchName);
+    return MBS_OK;
+}
+
+int mbs_query_memory(int64_t *free_bytes, int64_t *total_bytes)
+{
+    size_t fr = 0, tot = 0;
+    HIP_TRY(nullptr, hipMemGetInfo(&fr, &tot));
+    if (free_bytes)
+        *free_bytes = (int64_t)fr;
+    if (total_bytes)
+        *total_bytes = (int64_t)tot;
+    return MBS_OK;
+}
+
+int mbs_ctx_create(int device, mbs_ctx **out)
+{
+    if (!out)
+        return fail(nullptr, MBS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || device < 0 || device >= n)
+        return fail(nullptr, MBS_ERR_DEVICE, "HIP device %d not available (%d visible)",
+                    device, n);
+    HIP_TRY(nullptr, hipSetDevice(device));
+    mbs_ctx *ctx = new mbs_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, MBS_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return MBS_OK;
+}
+
+int mbs_ctx_destroy(mbs_ctx *ctx)
+{
+    if (!ctx)
+        return MBS_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->theta.release();
+    ctx->vf.release();
+    ctx->tab.release();
+    ctx->bs_in.release();
+    ctx->vx_in.release();
+    ctx->vy_in.release();
+    ctx->syx_in.release();
+    ctx->sxy_in.release();
+    ctx->weights.release();
+    ctx->nv.release();
+    ctx->ext.release();
+    ctx->vflag.release();
+    ctx->voff.release();
+    ctx->ooff.release();
+    ctx->ef_in.release();
+    ctx->hdr.release();
+    ctx->vtx.release();
+    ctx->hdr32.release();
+    ctx->items.release();
+    ctx->part.release();
+    ctx->part_ns.release();
+    ctx->res.release();
+    ctx->res_ns.release();
+    ctx->per_orient.release();
+    if (ctx->pinned)
+        (void)hipHostFree(ctx->pinned);
+    for (hipEvent_t &e : ctx->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MBS_OK;
+}
+
+const char *mbs_last_error(const mbs_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_err.c_str();
+}
+
+void *mbs_ctx_stream(mbs_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *g)
+{
+    if (!ctx || !g || !g->theta || !g->vf)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_set_grid: NULL argument");
+    if (g->n_az <= 0 || g->n_zen < 0)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_set_grid: n_az=%d n_zen=%d", g->n_az, g->n_zen);
+    const int64_t cells = (int64_t)g->n_az * (g->n_zen + 1);
+    if (cells > (1 << 28))
+        return fail(ctx, MBS_ERR_CAPACITY, "grid of %lld cells exceeds 2^28", (long long)cells);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    ctx->grid_ok = false;
+    ctx->have_result = false;
+    ctx->n_az = g->n_az;
+    ctx->n_zen1 = g->n_zen + 1;
+    ctx->cells = (int)cells;
+    ctx->az_step = g->az_step;
+    HIP_TRY(ctx, ctx->theta.ensure(ctx->n_zen1));
+    HIP_TRY(ctx, ctx->vf.ensure((size_t)cells * 3));
+    HIP_TRY(ctx, ctx->tab.ensure((size_t)cells * DT_COUNT));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->theta.p, g->theta, sizeof(double) * ctx->n_zen1,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->vf.p, g->vf, sizeof(double) * cells * 3,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tab_mode = -1;
+    ctx->grid_ok = true;
+    return MBS_OK;
+}
+
+}  // extern "C"
+
+static int ensure_tables(mbs_ctx *ctx)
+{
+    if (ctx->tab_mode == 0)
+        return MBS_OK;
+    const int threads = 256;
+    const int blocks = (ctx->cells + threads - 1) / threads;
+    grid_tables_kernel<<<blocks, threads, 0, ctx->stream>>>(
+        ctx->theta.p, ctx->vf.p, ctx->n_az, ctx->n_zen1, ctx->az_step, ctx->tab.p);
+    HIP_TRY(ctx, hipGetLastError());
+    ctx->tab_mode = 0;
+    return MBS_OK;
+}
+
+extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts *c)
+{
+    if (!ctx || !bt || !c)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: NULL argument");
+    ctx->beams_ok = false;
+    ctx->have_result = false;
+    if (bt->n_orient < 0 || bt->n_beams < 0 || (bt->n_orient > 0 && !bt->orient_offsets)
+        || (bt->n_orient > 0 && !bt->weights))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: bad orientation arrays");
+    if (bt->n_beams > 0 && (!bt->beam_scalars || !bt->beam_nv || !bt->beam_external
+                            || !bt->vert_offsets || !bt->vx || !bt->vy || !bt->slope_yx
+                            || !bt->slope_xy || !bt->edge_flags))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: NULL beam array");
+    if (!(c->wave_index > 0) || !(c->eps1 >= 0)
+        || !(c->compl_wave[0] * c->compl_wave[0] + c->compl_wave[1] * c->compl_wave[1] > 0))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: bad constants");
+    const int no = bt->n_orient, nb = bt->n_beams;
+    if (no > 0 && (bt->orient_offsets[0] != 0 || bt->orient_offsets[no] != nb))
+        return fail(ctx, MBS_ERR_INVALID, "orient_offsets must run from 0 to n_beams");
+    for (int o = 0; o < no; ++o)
+        if (bt->orient_offsets[o + 1] < bt->orient_offsets[o])
+            return fail(ctx, MBS_ERR_INVALID, "orient_offsets decrease at %d", o);
+    if (nb > 0 && bt->vert_offsets[0] != 0)
+        return fail(ctx, MBS_ERR_INVALID, "vert_offsets[0] must be 0");
+    std::memset(ctx->nv_hist, 0, sizeof ctx->nv_hist);
+    int64_t edges = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int n = bt->beam_nv[b];
+        if (n < MBS_MIN_VERTICES || n > MBS_MAX_VERTICES)
+            return fail(ctx, MBS_ERR_BAD_BEAM,
+                        "beam %d has %d aperture vertices (engine accepts %d..%d)", b, n,
+                        MBS_MIN_VERTICES, MBS_MAX_VERTICES);
+        if (bt->vert_offsets[b + 1] - bt->vert_offsets[b] != n)
+            return fail(ctx, MBS_ERR_INVALID, "vert_offsets of beam %d disagree with nv", b);
+        ctx->nv_hist[n]++;
+        for (int64_t v = bt->vert_offsets[b]; v < bt->vert_offsets[b + 1]; ++v)
+            edges += ((bt->edge_flags[v] & 1) ? 1 : 0) + ((bt->edge_flags[v] & 2) ? 1 : 0);
+    }
+    const int64_t nvert = nb > 0 ? bt->vert_offsets[nb] : 0;
+    ctx->mean_edges = nb > 0 ? 0.5 * (double)edges / nb : 0.0;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, ctx->bs_in.ensure((size_t)nb * MBS_BEAM_SCALARS));
+    HIP_TRY(ctx, ctx->nv.ensure(nb));
+    HIP_TRY(ctx, ctx->ext.ensure(nb));
+    HIP_TRY(ctx, ctx->voff.ensure(nb + 1));
+    HIP_TRY(ctx, ctx->ooff.ensure(no + 1));
+    HIP_TRY(ctx, ctx->weights.ensure(no));
+    HIP_TRY(ctx, ctx->vx_in.ensure(nvert));
+    HIP_TRY(ctx, ctx->vy_in.ensure(nvert));
+    HIP_TRY(ctx, ctx->syx_in.ensure(nvert));
+    HIP_TRY(ctx, ctx->sxy_in.ensure(nvert));
+    HIP_TRY(ctx, ctx->ef_in.ensure(nvert));
+    HIP_TRY(ctx, ctx->hdr.ensure((size_t)nb * HDR_STRIDE));
+    HIP_TRY(ctx, ctx->hdr32.ensure((size_t)nb * HDR32_STRIDE));
+    HIP_TRY(ctx, ctx->vtx.ensure((size_t)nvert * VTX_STRIDE));
+    HIP_TRY(ctx, ctx->vflag.ensure(nvert));
+    hipStream_t s = ctx->stream;
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+    if (nb > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->bs_in.p, bt->beam_scalars,
+                                    sizeof(double) * nb * MBS_BEAM_SCALARS,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->nv.p, bt->beam_nv, sizeof(int32_t) * nb,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->ext.p, bt->beam_external, sizeof(int32_t) * nb,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->voff.p, bt->vert_offsets, sizeof(int64_t) * (nb + 1),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->vx_in.p, bt->vx, sizeof(double) * nvert,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->vy_in.p, bt->vy, sizeof(double) * nvert,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->syx_in.p, bt->slope_yx, sizeof(double) * nvert,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->sxy_in.p, bt->slope_xy, sizeof(double) * nvert,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->ef_in.p, bt->edge_flags, nvert,
+                                    hipMemcpyHostToDevice, s));
+    }
+    if (no > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->ooff.p, bt->orient_offsets, sizeof(int64_t) * (no + 1),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->weights.p, bt->weights, sizeof(double) * no,
+                                    hipMemcpyHostToDevice, s));
+    }
+    // small-phase area limit: (legacy ? invComplWave : -invComplWave) * area
+    const double sg = c->legacy_sign ? 1.0 : -1.0;
+    if (nb > 0) {
+        pack_beams_kernel<<<(nb + 127) / 128, 128, 0, s>>>(
+            nb, ctx->bs_in.p, ctx->nv.p, ctx->ext.p, ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
+            ctx->syx_in.p, ctx->sxy_in.p, ctx->ef_in.p, c->wave_index,
+            sg * c->inv_compl_wave[0], sg * c->inv_compl_wave[1], c->compl_wave[0],
+            c->compl_wave[1], ctx->hdr.p, ctx->hdr32.p,
+            ctx->vtx.p, ctx->vflag.p);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+        ctx->ev_up = true;
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    ctx->h_ooff.assign(bt->orient_offsets, bt->orient_offsets + no + 1);
+    if (no == 0)
+        ctx->h_ooff.assign(1, 0);
+    ctx->h_weights.assign(bt->weights, bt->weights + no);
+    ctx->n_orient = no;
+    ctx->n_beams = nb;
+    ctx->n_vert = nvert;
+    ctx->consts = *c;
+    ctx->beams_ok = true;
+    return MBS_OK;
+}
+
+// Splits the batch into work items.
+//  * Segment mode (default): every orientation is cut into beam segments of
+//    about equal size, so items carry equal work whatever the spread of beam
+//    counts between orientations (50 to 40k beams in an aggregate average);
+//    each item writes a partial Jones grid and a second kernel adds the
+//    segments of each orientation in order and converts to Mueller.
+//  * Orientation mode: runs of whole orientations with about equal beam
+//    totals; each item keeps the weighted Mueller sum of its run.  Used when
+//    the partial Jones grids of segment mode would exceed the memory budget.
+static const size_t kPartialBudget = (size_t)8 << 30;  // bytes
+
+static bool plan_items(mbs_ctx *ctx, int n_tiles, std::vector<WorkItem> &items)
+{
+    items.clear();
+    const int no = ctx->n_orient;
+    const int64_t nb = ctx->n_beams;
+    const int target = ctx->target_blocks;
+    const size_t cells = (size_t)ctx->cells;
+    const int64_t want_items = std::max<int64_t>(1, (target + n_tiles - 1) / n_tiles);
+    // segment size in beams: enough items to fill the device, but at least
+    // kMinSeg beams each so the per-item writes stay negligible
+    const int64_t kMinSeg = 32;
+    int64_t seg = std::max<int64_t>(kMinSeg, (nb + want_items - 1) / std::max<int64_t>(want_items, 1));
+    int64_t n_seg = 0;
+    for (int o = 0; o < no; ++o) {
+        const int64_t n = ctx->h_ooff[o + 1] - ctx->h_ooff[o];
+        n_seg += std::max<int64_t>(1, (n + seg - 1) / seg);
+    }
+    const bool segments_fit = (size_t)n_seg * cells * 8 * sizeof(double) <= kPartialBudget;
+    const bool jones = ctx->work_mode == 2 || (ctx->work_mode == 0 && segments_fit && nb > 0);
+    if (jones) {
+        for (int o = 0; o < no; ++o) {
+            const int64_t b0 = ctx->h_ooff[o], b1 = ctx->h_ooff[o + 1];
+            const int64_t n = b1 - b0;
+            const int64_t k = std::max<int64_t>(1, (n + seg - 1) / seg);
+            for (int64_t i = 0; i < k; ++i)
+                items.push_back(WorkItem{o, o + 1, b0 + n * i / k, b0 + n * (i + 1) / k});
+        }
+        return true;
+    }
+    int chunks = (int)std::max<int64_t>(1, std::min<int64_t>(want_items, no));
+    const double per = (double)nb / chunks;
+    int o = 0;
+    for (int ci = 0; ci < chunks && o < no; ++ci) {
+        const double goal = per * (ci + 1);
+        int e = o + 1;
+        while (e < no && (double)ctx->h_ooff[e] < goal && (no - e) > (chunks - ci - 1))
+            ++e;
+        if (ci == chunks - 1)
+            e = no;
+        items.push_back(WorkItem{o, e, 0, 0});
+        o = e;
+    }
+    return false;
+}
+
+template <int PREC, int NS, int JONES, int PER>
+static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
+{
+    diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
+        a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vflag, a.nv,
+        a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns, a.per_orient);
+}
+
+static void launch_diffract(int prec, int ns, int jones, int per, const KernelArgs &a,
+                            int blocks, hipStream_t s)
+{
+#define MBS_L(P, N, J, R)                                          \
+    if (prec == P && ns == N && jones == J && per == R) {         \
+        launch_one<P, N, J, R>(a, blocks, s);                      \
+        return;                                                    \
+    }
+    MBS_L(0, 0, 0, 0) MBS_L(0, 1, 0, 0) MBS_L(0, 0, 1, 0) MBS_L(0, 1, 1, 0)
+    MBS_L(0, 0, 0, 1) MBS_L(0, 1, 0, 1)
+    MBS_L(1, 0, 0, 0) MBS_L(1, 1, 0, 0) MBS_L(1, 0, 1, 0) MBS_L(1, 1, 1, 0)
+    MBS_L(1, 0, 0, 1) MBS_L(1, 1, 0, 1)
+#undef MBS_L
+}
+
+extern "C" {
+
+int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->grid_ok)
+        return fail(ctx, MBS_ERR_NO_GRID, "mbs_set_grid has not been called");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    const int prec = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
+    const int per = (flags & MBS_PER_ORIENT) ? 1 : 0;
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT))
+        return fail(ctx, MBS_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_tables(ctx);
+    if (rc)
+        return rc;
+    const int cells = ctx->cells;
+    const int n_tiles = (cells + 255) / 256;
+    const bool jones = plan_items(ctx, n_tiles, ctx->h_items);
+    const int n_items = (int)ctx->h_items.size();
+    const size_t mcount = (size_t)cells * 16;
+    HIP_TRY(ctx, ctx->res.ensure(mcount));
+    if (ns)
+        HIP_TRY(ctx, ctx->res_ns.ensure(mcount));
+    if (per)
+        HIP_TRY(ctx, ctx->per_orient.ensure((size_t)ctx->n_orient * mcount));
+    hipStream_t s = ctx->stream;
+    if (n_items == 0) {
+        HIP_TRY(ctx, hipMemsetAsync(ctx->res.p, 0, sizeof(double) * mcount, s));
+        if (ns)
+            HIP_TRY(ctx, hipMemsetAsync(ctx->res_ns.p, 0, sizeof(double) * mcount, s));
+        ctx->have_result = true;
+        ctx->have_ns = ns;
+        ctx->have_per = per;
+        return MBS_OK;
+    }
+    const size_t pstride = (size_t)cells * (jones ? 8 : 16);
+    HIP_TRY(ctx, ctx->part.ensure((size_t)n_items * pstride));
+    if (ns)
+        HIP_TRY(ctx, ctx->part_ns.ensure((size_t)n_items * pstride));
+    HIP_TRY(ctx, ctx->items.ensure(n_items));
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->items.p, ctx->h_items.data(), sizeof(WorkItem) * n_items,
+                                hipMemcpyHostToDevice, s));
+    KernelArgs a;
+    a.items = ctx->items.p;
+    a.n_items = n_items;
+    a.n_tiles = n_tiles;
+    a.cells = cells;
+    a.tab = ctx->tab.p;
+    a.hdr = ctx->hdr.p;
+    a.hdr32 = ctx->hdr32.p;
+    a.vtx = ctx->vtx.p;
+    a.vflag = ctx->vflag.p;
+    a.nv = ctx->nv.p;
+    a.voff = ctx->voff.p;
+    a.ext = ctx->ext.p;
+    a.ooff = ctx->ooff.p;
+    a.weights = ctx->weights.p;
+    a.k = ctx->consts.wave_index;
+    // The mixed kernel forms edge phasor differences in FP32, so its
+    // switch to the exact half-angle edge form happens at |C| <= 3e-5, the
+    // floor the reference applies to its FP32 build (GpuDiffraction.cu:498-512).
+    a.eps1 = prec ? std::max(ctx->consts.eps1, 3e-5) : ctx->consts.eps1;
+    a.cw_re = ctx->consts.compl_wave[0];
+    a.cw_im = ctx->consts.compl_wave[1];
+    a.out = ctx->part.p;
+    a.out_ns = ns ? ctx->part_ns.p : nullptr;
+    a.per_orient = (per && !jones) ? ctx->per_orient.p : nullptr;
+    const int64_t total = (int64_t)n_items * n_tiles;
+    if (total > (int64_t)1 << 30)
+        return fail(ctx, MBS_ERR_CAPACITY, "launch of %lld workgroups", (long long)total);
+    const int blocks = (int)((total + 7) / 8 * 8);
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+    launch_diffract(prec, ns, jones ? 1 : 0, (per && !jones) ? 1 : 0, a, blocks, s);
+    HIP_TRY(ctx, hipGetLastError());
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    ctx->last_items = n_items;
+    ctx->last_blocks = blocks;
+    const int rt = 256;
+    if (jones) {
+        jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
+            ctx->part.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res.p,
+            per ? ctx->per_orient.p : nullptr);
+        HIP_TRY(ctx, hipGetLastError());
+        if (ns) {
+            jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
+                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res_ns.p,
+                nullptr);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        const int rb = (int)((mcount + rt - 1) / rt);
+        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->res.p);
+        HIP_TRY(ctx, hipGetLastError());
+        if (ns) {
+            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells,
+                                                    ctx->res_ns.p);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    }
+    if (ctx->timing) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        ctx->ev_diff = true;
+    }
+    ctx->have_result = true;
+    ctx->have_ns = ns;
+    ctx->have_per = per;
+    return MBS_OK;
+}
+
+int mbs_set_timing(mbs_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (enable)
+        for (hipEvent_t &e : ctx->ev)
+            if (!e)
+                HIP_TRY(ctx, hipEventCreate(&e));
+    ctx->timing = enable != 0;
+    ctx->ev_up = ctx->ev_diff = false;
+    return MBS_OK;
+}
+
+int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_get_timing: NULL argument");
+    if (!ctx->timing)
+        return fail(ctx, MBS_ERR_INVALID, "timing is not enabled (mbs_set_timing)");
+    std::memset(out, 0, sizeof *out);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (ctx->ev_up) {
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+        HIP_TRY(ctx, hipEventElapsedTime(&out->upload_ms, ctx->ev[0], ctx->ev[1]));
+    }
+    if (ctx->ev_diff) {
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev[4]));
+        HIP_TRY(ctx, hipEventElapsedTime(&out->kernel_ms, ctx->ev[2], ctx->ev[3]));
+        HIP_TRY(ctx, hipEventElapsedTime(&out->reduce_ms, ctx->ev[3], ctx->ev[4]));
+    }
+    out->n_items = ctx->last_items;
+    out->n_blocks = ctx->last_blocks;
+    return MBS_OK;
+}
+
+}  // extern "C"
+
+static int fetch(mbs_ctx *ctx, const double *src, size_t n, double *dst_add, bool add)
+{
+    if (ctx->pinned_cap < n) {
+        if (ctx->pinned)
+            (void)hipHostFree(ctx->pinned);
+        ctx->pinned = nullptr;
+        ctx->pinned_cap = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&ctx->pinned, n * sizeof(double)));
+        ctx->pinned_cap = n;
+    }
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, src, n * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (add)
+        for (size_t i = 0; i < n; ++i)
+            dst_add[i] += ctx->pinned[i];
+    else
+        std::memcpy(dst_add, ctx->pinned, n * sizeof(double));
+    return MBS_OK;
+}
+
+extern "C" {
+
+int mbs_accumulate_mueller(mbs_ctx *ctx, double *m_full, double *m_noshadow)
+{
+    if (!ctx || !m_full)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_accumulate_mueller: NULL argument");
+    if (!ctx->have_result)
+        return fail(ctx, MBS_ERR_INVALID, "no result: call mbs_diffract first");
+    if (m_noshadow && !ctx->have_ns)
+        return fail(ctx, MBS_ERR_INVALID, "no-shadow result was not requested (MBS_NOSHADOW)");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->cells * 16;
+    int rc = fetch(ctx, ctx->res.p, n, m_full, true);
+    if (rc)
+        return rc;
+    if (m_noshadow)
+        return fetch(ctx, ctx->res_ns.p, n, m_noshadow, true);
+    return MBS_OK;
+}
+
+int mbs_fetch_per_orient(mbs_ctx *ctx, double *out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_fetch_per_orient: NULL argument");
+    if (!ctx->have_result || !ctx->have_per)
+        return fail(ctx, MBS_ERR_INVALID, "per-orientation result was not requested");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return fetch(ctx, ctx->per_orient.p, (size_t)ctx->n_orient * ctx->cells * 16, out, false);
+}
+
+int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid, const mbs_beam_batch *batch,
+                       const mbs_consts *consts, uint32_t flags, double *m_full,
+                       double *m_noshadow)
+{
+    int rc = mbs_set_grid(ctx, grid);
+    if (rc)
+        return rc;
+    rc = mbs_upload_beams(ctx, batch, consts);
+    if (rc)
+        return rc;
+    rc = mbs_diffract(ctx, flags | (m_noshadow ? MBS_NOSHADOW : 0u));
+    if (rc)
+        return rc;
+    return mbs_accumulate_mueller(ctx, m_full, m_noshadow);
+}
+
+int mbs_device_result(mbs_ctx *ctx, const double **m_full, const double **m_noshadow)
+{
+    if (!ctx || !ctx->have_result)
+        return fail(ctx, MBS_ERR_INVALID, "no result on the device");
+    if (m_full)
+        *m_full = ctx->res.p;
+    if (m_noshadow)
+        *m_noshadow = ctx->have_ns ? ctx->res_ns.p : nullptr;
+    return MBS_OK;
+}
+
+int mbs_batch_stats(mbs_ctx *ctx, int64_t *n_beams, double *units, int64_t *nv_hist32,
+                    double *mean_edges)
+{
+    if (!ctx || !ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "no beams uploaded");
+    if (n_beams)
+        *n_beams = ctx->n_beams;
+    if (units)
+        *units = (double)ctx->n_beams * (ctx->grid_ok ? ctx->cells : 0);
+    if (nv_hist32)
+        std::memcpy(nv_hist32, ctx->nv_hist, sizeof ctx->nv_hist);
+    if (mean_edges)
+        *mean_edges = ctx->mean_edges;
+    return MBS_OK;
+}
+
+int mbs_set_work_mode(mbs_ctx *ctx, int mode)
+{
+    if (!ctx || mode < 0 || mode > 2)
+        return fail(ctx, MBS_ERR_INVALID, "work mode must be 0, 1 or 2");
+    ctx->work_mode = mode;
+    return MBS_OK;
+}
+
+int mbs_set_target_blocks(mbs_ctx *ctx, int blocks)
+{
+    if (!ctx || blocks < 1)
+        return fail(ctx, MBS_ERR_INVALID, "bad target block count");
+    ctx->target_blocks = blocks;
+    return MBS_OK;
+}
+
+}  // extern "C"

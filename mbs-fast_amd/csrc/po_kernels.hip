@@ -1,0 +1,630 @@
+// po_kernels.hip -- gfx950 kernels of the PO diffraction engine.
+//
+// Work mapping.  One lane owns one scattering direction (grid cell); a
+// 256-lane workgroup owns 256 consecutive cells of one "work item".  A work
+// item is either a run of whole orientations (Mueller mode: the lane sums the
+// Jones matrices of every beam of an orientation in registers, converts to
+// Mueller, and accumulates weight*Mueller over the run) or one segment of one
+// orientation's beams (Jones mode, used when there are too few orientations
+// to fill the chip: the lane writes the partial Jones sum and a second kernel
+// adds the segments and converts).  Every beam of an item is the same for all
+// lanes, so beam records are read with wave-uniform loads (scalar cache), and
+// all arithmetic per (beam, direction) stays in VGPRs.  Results are reduced
+// in a fixed order by separate kernels, so output is bit-reproducible.
+//
+// Per (beam, direction) the lane evaluates, following the reference CPU loop
+// src/handler/HandlerPO.cpp:2632-2776:
+//   A = (b - d).hor, B = (b - d).ver                       (k-k0 projected)
+//   phi_v = A*k*x_v + B*k*y_v  (+ psi = -k d.c for internal beams, folded into
+//   the vertex phasors as the reference CUDA path does, GpuDiffraction.cu:
+//   1110-1122), F = complWave * sum_e (e^{i phi_e+1} - e^{i phi_e})/C_e / B
+//   (|B|>|A|) or ... / (-A), with the small-phase moment form when
+//   max|phi_v| <= 1e-3 and the half-angle form when |C_e| <= eps1;
+//   r_ij = u_i(d).T_j(beam) (the Jones rotation of HandlerPO_fast.h:580-683
+//   rewritten exactly with per-direction vectors u_i), d = F R J_phased.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "po_math.h"
+#include "po_layout.h"
+
+// Occupancy hint of the main kernel (waves per SIMD); the default lets the
+// compiler choose.  Builds may pass -DMBS_MIN_WAVES=N for tuning.
+#ifdef MBS_MIN_WAVES
+#define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
+#else
+#define MBS_OCCUPANCY
+#endif
+
+namespace mbs {
+
+// ---------------------------------------------------------------------------
+// Grid tables: per cell dir(3), u0(6), u1(6) as SoA [DT_COUNT][cells].
+// u0 = (vt x d, vt - d (d.vt)), u1 = (vf x d, vf - d (d.vf)) with
+// vt = (vf x d)/|vf x d| (kept unnormalised when |vf x d|^2 <= 1e-30, as
+// rotate_jones_inline does).  Then cpT.vt = NT.(vt x d) + nxDT.(vt - d(d.vt))
+// for cpT = d x NT + nxDT - d (d.nxDT), and likewise for cpP, vf.
+// ---------------------------------------------------------------------------
+__global__ void grid_tables_kernel(const double *__restrict__ theta,
+                                   const double *__restrict__ vf, int n_az,
+                                   int n_zen1, double az_step,
+                                   double *__restrict__ tab)
+{
+    const int cells = n_az * n_zen1;
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells)
+        return;
+    const int p = cell / n_zen1;
+    const int t = cell - p * n_zen1;
+    double st, ct, sp, cp;
+    sincos(theta[t], &st, &ct);
+    sincos(p * az_step, &sp, &cp);
+    const double d[3] = {st * cp, st * sp, -ct};
+    const double f[3] = {vf[3 * cell], vf[3 * cell + 1], vf[3 * cell + 2]};
+    double vt[3] = {f[1] * d[2] - f[2] * d[1], f[2] * d[0] - f[0] * d[2],
+                    f[0] * d[1] - f[1] * d[0]};
+    const double l2 = vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2];
+    if (l2 > 1e-30) {
+        const double inv = 1.0 / sqrt(l2);
+        vt[0] *= inv;
+        vt[1] *= inv;
+        vt[2] *= inv;
+    }
+    const double dvt = d[0] * vt[0] + d[1] * vt[1] + d[2] * vt[2];
+    const double dvf = d[0] * f[0] + d[1] * f[1] + d[2] * f[2];
+    double u[DT_COUNT];
+    u[DT_DIR + 0] = d[0];
+    u[DT_DIR + 1] = d[1];
+    u[DT_DIR + 2] = d[2];
+    // u0: (vt x d, vt - d (d.vt))
+    u[DT_U0 + 0] = vt[1] * d[2] - vt[2] * d[1];
+    u[DT_U0 + 1] = vt[2] * d[0] - vt[0] * d[2];
+    u[DT_U0 + 2] = vt[0] * d[1] - vt[1] * d[0];
+    u[DT_U0 + 3] = vt[0] - d[0] * dvt;
+    u[DT_U0 + 4] = vt[1] - d[1] * dvt;
+    u[DT_U0 + 5] = vt[2] - d[2] * dvt;
+    u[DT_U1 + 0] = f[1] * d[2] - f[2] * d[1];
+    u[DT_U1 + 1] = f[2] * d[0] - f[0] * d[2];
+    u[DT_U1 + 2] = f[0] * d[1] - f[1] * d[0];
+    u[DT_U1 + 3] = f[0] - d[0] * dvf;
+    u[DT_U1 + 4] = f[1] - d[1] * dvf;
+    u[DT_U1 + 5] = f[2] - d[2] * dvf;
+    for (int q = 0; q < DT_COUNT; ++q)
+        tab[(size_t)q * cells + cell] = u[q];
+}
+
+// ---------------------------------------------------------------------------
+// Beam packing: one thread per beam, from the C-ABI input arrays to the
+// kernel record (po_layout.h).  Small-phase moments follow
+// small_phase_polygon_factor (src/handler/HandlerPO_fast.h:154-216).
+// ---------------------------------------------------------------------------
+__global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
+                                  const int32_t *__restrict__ nv_in,
+                                  const int32_t *__restrict__ ext_in,
+                                  const int64_t *__restrict__ voff,
+                                  const double *__restrict__ vx,
+                                  const double *__restrict__ vy,
+                                  const double *__restrict__ syx,
+                                  const double *__restrict__ sxy,
+                                  const uint8_t *__restrict__ ef, double k,
+                                  double al_re, double al_im, double cw_re,
+                                  double cw_im,
+                                  double *__restrict__ hdr,
+                                  float *__restrict__ hdr32,
+                                  double *__restrict__ vtx,
+                                  int32_t *__restrict__ vflag)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_beams)
+        return;
+    const double *s = bs + (size_t)b * 36;
+    const int nv = nv_in[b];
+    const int64_t v0 = voff[b];
+    double *h = hdr + (size_t)b * HDR_STRIDE;
+    float *h32 = hdr32 + (size_t)b * HDR32_STRIDE;
+    const double *hor = s + 0, *ver = s + 3, *bd = s + 6, *cen = s + 9;
+    h[H_HOR + 0] = hor[0];
+    h[H_HOR + 1] = hor[1];
+    h[H_HOR + 2] = hor[2];
+    h[H_A0] = bd[0] * hor[0] + bd[1] * hor[1] + bd[2] * hor[2];
+    h[H_VER + 0] = ver[0];
+    h[H_VER + 1] = ver[1];
+    h[H_VER + 2] = ver[2];
+    h[H_B0] = bd[0] * ver[0] + bd[1] * ver[1] + bd[2] * ver[2];
+    const bool ext = ext_in[b] != 0;
+    // psi = -k d.c for internal beams; external beams carry no direction phase
+    h[H_KC + 0] = ext ? 0.0 : -k * cen[0];
+    h[H_KC + 1] = ext ? 0.0 : -k * cen[1];
+    h[H_KC + 2] = ext ? 0.0 : -k * cen[2];
+    h[H_EXT] = ext ? 1.0 : 0.0;
+    // complWave is folded into J_phased below; the small-phase area limit is
+    // therefore stored divided by complWave.
+    const double area = s[12];
+    const double cw2 = cw_re * cw_re + cw_im * cw_im;
+    const double lr = al_re * area, li = al_im * area;
+    h[H_AL + 0] = (lr * cw_re + li * cw_im) / cw2;
+    h[H_AL + 1] = (li * cw_re - lr * cw_im) / cw2;
+    // polygon moments about the aperture origin
+    double tw = 0, fx6 = 0, fy6 = 0, sx12 = 0, sy12 = 0, sxy24 = 0;
+    double minX = 1e300, maxX = -1e300, minY = 1e300, maxY = -1e300;
+    for (int i = 0; i < nv; ++i) {
+        const int j = (i + 1 < nv) ? i + 1 : 0;
+        const double xi = vx[v0 + i], yi = vy[v0 + i];
+        const double xj = vx[v0 + j], yj = vy[v0 + j];
+        const double cr = xi * yj - xj * yi;
+        tw += cr;
+        fx6 += (xi + xj) * cr;
+        fy6 += (yi + yj) * cr;
+        sx12 += (xi * xi + xi * xj + xj * xj) * cr;
+        sy12 += (yi * yi + yi * yj + yj * yj) * cr;
+        sxy24 += (2.0 * xi * yi + xi * yj + xj * yi + 2.0 * xj * yj) * cr;
+        minX = fmin(minX, xi);
+        maxX = fmax(maxX, xi);
+        minY = fmin(minY, yi);
+        maxY = fmax(maxY, yi);
+    }
+    const double areaScale = fmax((maxX - minX) * (maxY - minY), 2.2250738585072014e-308);
+    double mX = 0, mY = 0, mXX = 0, mYY = 0, mXY = 0;
+    if (!(fabs(tw) <= 0x1p-40 * areaScale)) {
+        const double inv = 2.0 / tw;
+        mX = (fx6 / 6.0) * inv;
+        mY = (fy6 / 6.0) * inv;
+        mXX = (sx12 / 12.0) * inv;
+        mYY = (sy12 / 12.0) * inv;
+        mXY = (sxy24 / 24.0) * inv;
+    }
+    h[H_MOM + 0] = k * mX;
+    h[H_MOM + 1] = k * mY;
+    h[H_MOM + 2] = k * k * mXX;
+    h[H_MOM + 3] = 2.0 * k * k * mXY;
+    h[H_MOM + 4] = k * k * mYY;
+    h[H_MOM + 5] = 0.0;
+    // rotation vectors, pre-scaled by 1/2 (exact)
+    for (int q = 0; q < 3; ++q) {
+        h[H_T0 + q] = 0.5 * s[13 + q];      // NT
+        h[H_T0 + 3 + q] = 0.5 * s[19 + q];  // nxDT
+        h[H_T1 + q] = 0.5 * s[16 + q];      // NP
+        h[H_T1 + 3 + q] = 0.5 * s[22 + q];  // nxDP
+    }
+    for (int q = 0; q < 4; ++q) {  // complWave * J_phased
+        const double jr = s[25 + 2 * q], ji = s[26 + 2 * q];
+        h[H_JP + 2 * q] = cw_re * jr - cw_im * ji;
+        h[H_JP + 2 * q + 1] = cw_re * ji + cw_im * jr;
+    }
+    // FP32 copy of what the mixed-precision kernel uses in FP32
+    for (int q = 0; q < 6; ++q) {
+        h32[H32_T0 + q] = (float)h[H_T0 + q];
+        h32[H32_T1 + q] = (float)h[H_T1 + q];
+    }
+    for (int q = 0; q < 8; ++q)
+        h32[H32_JP + q] = (float)h[H_JP + q];
+    h32[H32_AL + 0] = (float)h[H_AL + 0];
+    h32[H32_AL + 1] = (float)h[H_AL + 1];
+    for (int q = 0; q < 5; ++q)
+        h32[H32_MOM + q] = (float)h[H_MOM + q];
+    h32[H32_PAD] = 0.0f;
+    for (int i = 0; i < nv; ++i) {
+        double *v = vtx + (size_t)(v0 + i) * VTX_STRIDE;
+        const uint8_t f = ef[v0 + i];
+        v[V_KX] = k * vx[v0 + i];
+        v[V_KY] = k * vy[v0 + i];
+        v[V_SYX] = (f & 1) ? syx[v0 + i] : 0.0;
+        v[V_SXY] = (f & 2) ? sxy[v0 + i] : 0.0;
+        const int j = (i + 1 < nv) ? i + 1 : 0;
+        v[V_DX] = vx[v0 + j] - vx[v0 + i];
+        v[V_DY] = vy[v0 + j] - vy[v0 + i];
+        vflag[v0 + i] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-(beam, direction) evaluation, FP64.  Returns false when the aperture
+// integral is NaN (the reference skips the beam, HandlerPO.cpp:2742); else
+// adds d = F R Jp to j[8] (and to jn[8] when ACC_NS).
+//
+// The two edge-sum branches of the reference (|B| > |A|: sum over edges with
+// valid dx, coefficient A + (dy/dx) B, divide by B; otherwise edges with
+// valid dy, coefficient A (dx/dy) + B, divide by -A) are evaluated by one
+// loop whose operands are selected per lane, so lanes that take different
+// branches do not diverge.  complWave is folded into J_phased and the
+// small-phase area limit at packing time.
+// ---------------------------------------------------------------------------
+template <int ACC_NS>
+__device__ __forceinline__ void beam_dir_f64(
+    const double *__restrict__ h, const double *__restrict__ vtx,
+    const int32_t *__restrict__ vflag, int nv, const double *d,
+    const double *u0, const double *u1, double k, double eps1, double *j,
+    double *jn)
+{
+    // A, B: projections of k - k0 = b - d on the aperture axes
+    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
+    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
+    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
+    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
+    const double mul = useB ? B : A;   // coefficient = slope * mul + add
+    const double add = useB ? A : B;
+    const int vmask = useB ? 1 : 2;
+    // psi is folded into the vertex phasors (the integral is linear in each
+    // phasor), as the reference CUDA path does (GpuDiffraction.cu:1110-1122)
+    double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+    double maxph = __builtin_fabs(ph);
+    double s0, c0;
+    sincos_fast(ph + psi, s0, c0);
+    double sp = s0, cp = c0, sr = 0.0, si = 0.0;
+    for (int e = 0; e < nv; ++e) {
+        const double *v = vtx + (size_t)e * VTX_STRIDE;
+        const double slope_x = v[V_SYX], slope_y = v[V_SXY];
+        const double dxe = v[V_DX], dye = v[V_DY];
+        const int f = vflag[e];
+        double sn = s0, cn = c0;
+        if (e + 1 < nv) {
+            const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
+            maxph = fmax(maxph, __builtin_fabs(pn));
+            sincos_fast(pn + psi, sn, cn);
+        }
+        const double cc = __builtin_fma(useB ? slope_x : slope_y, mul, add);
+        const bool valid = (f & vmask) != 0;
+        const bool big = __builtin_fabs(cc) > eps1;
+        const double r = rcp_f64(cc);
+        const double inv = (valid && big) ? r : 0.0;
+        sr = __builtin_fma(cn - cp, inv, sr);
+        si = __builtin_fma(sn - sp, inv, si);
+        if (__builtin_expect(valid && !big, 0)) {
+            const double2 q = stable_edge_quotient(cp, sp, useB ? dxe : dye, cc, k);
+            sr += q.x;
+            si += q.y;
+        }
+        sp = sn;
+        cp = cn;
+    }
+    const double nrm = rcp_f64(useB ? B : -A);
+    // integral / complWave
+    double fr = sr * nrm, fi = si * nrm;
+    if (__builtin_expect(maxph <= 1e-3, 0)) {
+        // small-phase moment form (HandlerPO_fast.h:161-216); the area limit
+        // is stored divided by complWave
+        const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
+        const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
+        const double gr = 1.0 - 0.5 * p2, gi = p1;
+        fr = h[H_AL] * gr - h[H_AL + 1] * gi;
+        fi = h[H_AL] * gi + h[H_AL + 1] * gr;
+        double sd, cd;
+        sincos_fast(psi, sd, cd);
+        const double tr = fr * cd - fi * sd;
+        fi = fr * sd + fi * cd;
+        fr = tr;
+    }
+    if (__builtin_isnan(fr))
+        return;
+    const double *T0 = h + H_T0, *T1 = h + H_T1;
+    const double r00 = u0[0] * T0[0] + u0[1] * T0[1] + u0[2] * T0[2] + u0[3] * T0[3] + u0[4] * T0[4] + u0[5] * T0[5];
+    const double r01 = u0[0] * T1[0] + u0[1] * T1[1] + u0[2] * T1[2] + u0[3] * T1[3] + u0[4] * T1[4] + u0[5] * T1[5];
+    const double r10 = u1[0] * T0[0] + u1[1] * T0[1] + u1[2] * T0[2] + u1[3] * T0[3] + u1[4] * T0[4] + u1[5] * T0[5];
+    const double r11 = u1[0] * T1[0] + u1[1] * T1[1] + u1[2] * T1[2] + u1[3] * T1[3] + u1[4] * T1[4] + u1[5] * T1[5];
+    const double *jp = h + H_JP;  // complWave * J_phased
+    // q = R Jp (real 2x2 times complex 2x2), then d = F q
+    double q[8];
+    q[0] = r00 * jp[0] + r01 * jp[4];
+    q[1] = r00 * jp[1] + r01 * jp[5];
+    q[2] = r00 * jp[2] + r01 * jp[6];
+    q[3] = r00 * jp[3] + r01 * jp[7];
+    q[4] = r10 * jp[0] + r11 * jp[4];
+    q[5] = r10 * jp[1] + r11 * jp[5];
+    q[6] = r10 * jp[2] + r11 * jp[6];
+    q[7] = r10 * jp[3] + r11 * jp[7];
+    if constexpr (ACC_NS) {
+        for (int t = 0; t < 4; ++t) {
+            const double dr = fr * q[2 * t] - fi * q[2 * t + 1];
+            const double di = fr * q[2 * t + 1] + fi * q[2 * t];
+            j[2 * t] += dr;
+            j[2 * t + 1] += di;
+            jn[2 * t] += dr;
+            jn[2 * t + 1] += di;
+        }
+    } else {
+        for (int t = 0; t < 4; ++t) {
+            j[2 * t] = __builtin_fma(fr, q[2 * t], __builtin_fma(-fi, q[2 * t + 1], j[2 * t]));
+            j[2 * t + 1] = __builtin_fma(fr, q[2 * t + 1], __builtin_fma(fi, q[2 * t], j[2 * t + 1]));
+        }
+    }
+}
+
+// Per-(beam, direction) evaluation, mixed precision: A, B, the phases and
+// their reduction and the edge coefficients in FP64; phasors, edge sums,
+// rotation and Jones in FP32.
+template <int ACC_NS>
+__device__ __forceinline__ void beam_dir_mixed(
+    const double *__restrict__ h, const float *__restrict__ h32,
+    const double *__restrict__ vtx, const int32_t *__restrict__ vflag, int nv,
+    const double *d, const float *u0, const float *u1, double k, double eps1,
+    float *j, float *jn)
+{
+    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
+    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
+    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
+    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
+    const double mul = useB ? B : A;
+    const double add = useB ? A : B;
+    const int vmask = useB ? 1 : 2;
+    double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+    double maxph = __builtin_fabs(ph);
+    float s0, c0;
+    sincos_mixed(ph + psi, s0, c0);
+    float sp = s0, cp = c0, sr = 0.0f, si = 0.0f;
+    for (int e = 0; e < nv; ++e) {
+        const double *v = vtx + (size_t)e * VTX_STRIDE;
+        const double slope_x = v[V_SYX], slope_y = v[V_SXY];
+        const double dxe = v[V_DX], dye = v[V_DY];
+        const int f = vflag[e];
+        float sn = s0, cn = c0;
+        if (e + 1 < nv) {
+            const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
+            maxph = fmax(maxph, __builtin_fabs(pn));
+            sincos_mixed(pn + psi, sn, cn);
+        }
+        // the edge coefficient is cancellation-prone: FP64
+        const double cc = __builtin_fma(useB ? slope_x : slope_y, mul, add);
+        const bool valid = (f & vmask) != 0;
+        const bool big = __builtin_fabs(cc) > eps1;
+        const float r = rcp_f32((float)cc);
+        const float inv = (valid && big) ? r : 0.0f;
+        sr = __builtin_fmaf(cn - cp, inv, sr);
+        si = __builtin_fmaf(sn - sp, inv, si);
+        if (__builtin_expect(valid && !big, 0)) {
+            const double2 q = stable_edge_quotient((double)cp, (double)sp, useB ? dxe : dye,
+                                                   cc, k);
+            sr += (float)q.x;
+            si += (float)q.y;
+        }
+        sp = sn;
+        cp = cn;
+    }
+    const float nrm = rcp_f32((float)(useB ? B : -A));
+    float fr = sr * nrm, fi = si * nrm;
+    if (__builtin_expect(maxph <= 1e-3, 0)) {
+        const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
+        const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
+        const double gr = 1.0 - 0.5 * p2, gi = p1;
+        const double ar = h[H_AL] * gr - h[H_AL + 1] * gi;
+        const double ai = h[H_AL] * gi + h[H_AL + 1] * gr;
+        float sd, cd;
+        sincos_mixed(psi, sd, cd);
+        fr = (float)ar * cd - (float)ai * sd;
+        fi = (float)ar * sd + (float)ai * cd;
+    }
+    if (__builtin_isnan(fr))
+        return;
+    const float *T0 = h32 + H32_T0, *T1 = h32 + H32_T1;
+    const float r00 = u0[0] * T0[0] + u0[1] * T0[1] + u0[2] * T0[2] + u0[3] * T0[3] + u0[4] * T0[4] + u0[5] * T0[5];
+    const float r01 = u0[0] * T1[0] + u0[1] * T1[1] + u0[2] * T1[2] + u0[3] * T1[3] + u0[4] * T1[4] + u0[5] * T1[5];
+    const float r10 = u1[0] * T0[0] + u1[1] * T0[1] + u1[2] * T0[2] + u1[3] * T0[3] + u1[4] * T0[4] + u1[5] * T0[5];
+    const float r11 = u1[0] * T1[0] + u1[1] * T1[1] + u1[2] * T1[2] + u1[3] * T1[3] + u1[4] * T1[4] + u1[5] * T1[5];
+    const float *jp = h32 + H32_JP;
+    float q[8];
+    q[0] = r00 * jp[0] + r01 * jp[4];
+    q[1] = r00 * jp[1] + r01 * jp[5];
+    q[2] = r00 * jp[2] + r01 * jp[6];
+    q[3] = r00 * jp[3] + r01 * jp[7];
+    q[4] = r10 * jp[0] + r11 * jp[4];
+    q[5] = r10 * jp[1] + r11 * jp[5];
+    q[6] = r10 * jp[2] + r11 * jp[6];
+    q[7] = r10 * jp[3] + r11 * jp[7];
+    if constexpr (ACC_NS) {
+        for (int t = 0; t < 4; ++t) {
+            const float dr = fr * q[2 * t] - fi * q[2 * t + 1];
+            const float di = fr * q[2 * t + 1] + fi * q[2 * t];
+            j[2 * t] += dr;
+            j[2 * t + 1] += di;
+            jn[2 * t] += dr;
+            jn[2 * t + 1] += di;
+        }
+    } else {
+        for (int t = 0; t < 4; ++t) {
+            j[2 * t] = __builtin_fmaf(fr, q[2 * t], __builtin_fmaf(-fi, q[2 * t + 1], j[2 * t]));
+            j[2 * t + 1] = __builtin_fmaf(fr, q[2 * t + 1], __builtin_fmaf(fi, q[2 * t], j[2 * t + 1]));
+        }
+    }
+}
+
+// XCD-aware block -> (item, tile): consecutive workgroup ids land on the 8
+// XCDs round-robin; remapping gives each XCD a contiguous range of items so
+// the beam records of an item are shared through that XCD's L2.
+__device__ __forceinline__ bool map_block(int n_items, int n_tiles, int &item,
+                                          int &tile)
+{
+    const int total = n_items * n_tiles;
+    const int nblk = gridDim.x;
+    const int per_xcd = (nblk + 7) / 8;
+    const int bid = blockIdx.x;
+    const int logical = (bid % 8) * per_xcd + bid / 8;
+    if (logical >= total)
+        return false;
+    item = logical / n_tiles;
+    tile = logical - item * n_tiles;
+    return true;
+}
+
+struct KernelArgs {
+    const WorkItem *items;
+    int n_items, n_tiles, cells;
+    const double *tab;       // grid tables [DT_COUNT][cells]
+    const double *hdr;       // [n_beams][HDR_STRIDE]
+    const float *hdr32;      // [n_beams][HDR32_STRIDE]
+    const double *vtx;       // [n_vert][VTX_STRIDE]
+    const int32_t *vflag;    // [n_vert]
+    const int32_t *nv;       // [n_beams]
+    const int64_t *voff;     // [n_beams+1]
+    const int32_t *ext;      // [n_beams]
+    const int64_t *ooff;     // [n_orient+1]
+    const double *weights;   // [n_orient]
+    double k, eps1, cw_re, cw_im;
+    double *out;             // Mueller mode: [n_items][cells][16]; Jones: [n_items][cells][8]
+    double *out_ns;          // same shape, no-shadow (NS only)
+    double *per_orient;      // [n_orient][cells][16] (PER only, Mueller mode)
+};
+
+// Mueller of one orientation's Jones sum, weighted, stored to (first) or
+// added to (later orientations of the item) a lane-owned global cell: the
+// running orientation sum lives in memory, not in registers -- it is touched
+// once per orientation, against thousands of beams of work in between.
+template <typename T>
+__device__ __forceinline__ void flush_mueller(const T *j, double w, double *dst, bool first,
+                                              double *per)
+{
+    double mm[16];
+    for (int q = 0; q < 16; ++q)
+        mm[q] = 0.0;
+    add_mueller(mm, j, w);
+    if (per)
+        for (int q = 0; q < 16; ++q)
+            per[q] = mm[q];
+    if (first) {
+        for (int q = 0; q < 16; ++q)
+            dst[q] = mm[q];
+    } else {
+        for (int q = 0; q < 16; ++q)
+            dst[q] += mm[q];
+    }
+}
+
+// All pointers are separate __restrict__ kernel arguments: the per-orientation
+// stores into `out` must be provably disjoint from the beam records for the
+// compiler to keep reading those through the scalar cache.
+template <int PREC, int NS, int JONES, int PER>
+__global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
+    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
+    const double *__restrict__ tab, const double *__restrict__ hdr,
+    const float *__restrict__ hdr32, const double *__restrict__ vtx,
+    const int32_t *__restrict__ vflag, const int32_t *__restrict__ nv,
+    const int64_t *__restrict__ voff, const int32_t *__restrict__ ext,
+    const int64_t *__restrict__ ooff, const double *__restrict__ weights, double k,
+    double eps1, double *__restrict__ out, double *__restrict__ out_ns,
+    double *__restrict__ per_orient)
+{
+    int item, tile;
+    if (!map_block(n_items, n_tiles, item, tile))
+        return;
+    const int cell = tile * 256 + threadIdx.x;
+    const bool live = cell < cells;
+    const int c = live ? cell : cells - 1;  // dead lanes compute, never store
+    double d[3];
+    for (int q = 0; q < 3; ++q)
+        d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
+    using U = typename std::conditional<PREC == 1, float, double>::type;
+    U u0[6], u1[6];
+    for (int q = 0; q < 6; ++q) {
+        u0[q] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
+        u1[q] = (U)tab[(size_t)(DT_U1 + q) * cells + c];
+    }
+    const WorkItem it = items[item];
+    for (int o = it.o_begin; o < it.o_end; ++o) {
+        int64_t b0 = ooff[o], b1 = ooff[o + 1];
+        if constexpr (JONES) {
+            b0 = it.b_begin;
+            b1 = it.b_end;
+        }
+        U j[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        U jn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int64_t b = b0; b < b1; ++b) {
+            const double *h = hdr + (size_t)b * HDR_STRIDE;
+            const int64_t v0 = voff[b];
+            const int nvb = nv[b];
+            if constexpr (PREC == 0) {
+                if (NS && ext[b] == 0)
+                    beam_dir_f64<1>(h, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
+                                            nvb, d, u0, u1, k, eps1, j, jn);
+                else
+                    beam_dir_f64<0>(h, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
+                                            nvb, d, u0, u1, k, eps1, j, nullptr);
+            } else {
+                const float *h32 = hdr32 + (size_t)b * HDR32_STRIDE;
+                if (NS && ext[b] == 0)
+                    beam_dir_mixed<1>(h, h32, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
+                                      nvb, d, u0, u1, k, eps1, j, jn);
+                else
+                    beam_dir_mixed<0>(h, h32, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
+                                      nvb, d, u0, u1, k, eps1, j, nullptr);
+            }
+        }
+        if (!live)
+            continue;
+        if constexpr (JONES) {
+            double *dst = out + ((size_t)item * cells + cell) * 8;
+            for (int q = 0; q < 8; ++q)
+                dst[q] = j[q];
+            if constexpr (NS) {
+                double *dn = out_ns + ((size_t)item * cells + cell) * 8;
+                for (int q = 0; q < 8; ++q)
+                    dn[q] = jn[q];
+            }
+        } else {
+            const double w = weights[o];
+            const bool first = (o == it.o_begin);
+            double *per = PER ? per_orient + ((size_t)o * cells + cell) * 16 : nullptr;
+            flush_mueller(j, w, out + ((size_t)item * cells + cell) * 16, first, per);
+            if constexpr (NS)
+                flush_mueller(jn, w, out_ns + ((size_t)item * cells + cell) * 16, first,
+                              (double *)nullptr);
+        }
+    }
+}
+
+// Sum of per-item Mueller partials in item order: res[c] = sum_i part[i][c].
+__global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_items,
+                                      int cells, double *__restrict__ res)
+{
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)cells * 16;
+    if (idx >= n)
+        return;
+    double s = 0.0;
+    for (int i = 0; i < n_items; ++i)
+        s += part[(size_t)i * n + idx];
+    res[idx] = s;
+}
+
+// Jones segments -> Mueller: items are ordered by orientation; for each
+// orientation the segment Jones sums are added in segment order, converted
+// and accumulated with the orientation weight (TracerPOTotal.cpp:5985-5994).
+__global__ void jones_to_mueller_kernel(const double *__restrict__ part,
+                                        const WorkItem *__restrict__ items,
+                                        int n_items, int cells,
+                                        const double *__restrict__ weights,
+                                        double *__restrict__ res,
+                                        double *__restrict__ per_orient)
+{
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= cells)
+        return;
+    double m[16];
+    for (int q = 0; q < 16; ++q)
+        m[q] = 0.0;
+    int i = 0;
+    while (i < n_items) {
+        const int o = items[i].o_begin;
+        double j[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        while (i < n_items && items[i].o_begin == o) {
+            const double *src = part + ((size_t)i * cells + cell) * 8;
+            for (int q = 0; q < 8; ++q)
+                j[q] += src[q];
+            ++i;
+        }
+        add_mueller(m, j, weights[o]);
+        if (per_orient) {
+            double po[16];
+            for (int q = 0; q < 16; ++q)
+                po[q] = 0.0;
+            add_mueller(po, j, weights[o]);
+            double *dst = per_orient + ((size_t)o * cells + cell) * 16;
+            for (int q = 0; q < 16; ++q)
+                dst[q] = po[q];
+        }
+    }
+    double *dst = res + (size_t)cell * 16;
+    for (int q = 0; q < 16; ++q)
+        dst[q] = m[q];
+}
+
+}  // namespace mbs

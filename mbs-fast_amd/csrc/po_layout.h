@@ -1,0 +1,51 @@
+// po_layout.h -- device-side layout of packed beams and grid tables.
+#pragma once
+
+#include <stdint.h>
+
+namespace mbs {
+
+// Beam header, FP64, HDR_STRIDE doubles per beam (256 B aligned records so a
+// wave-uniform load of a header is a few 64-byte scalar loads).
+enum {
+    H_HOR = 0,   // aperture horizontal axis (3)
+    H_A0 = 3,    // beamDir . hor
+    H_VER = 4,   // aperture vertical axis (3)
+    H_B0 = 7,    // beamDir . ver
+    H_KC = 8,    // -k * centre (3), zero for external beams
+    H_EXT = 11,  // 1.0 for external beams
+    H_AL = 12,   // small-phase area limit / complWave (complex)
+    H_MOM = 14,  // k mX, k mY, k^2 mXX, 2 k^2 mXY, k^2 mYY, pad (6)
+    H_T0 = 20,   // 1/2 (NT, nxDT)   (6)
+    H_T1 = 26,   // 1/2 (NP, nxDP)   (6)
+    H_JP = 32,   // complWave * J_phased (8)
+    HDR_STRIDE = 40
+};
+
+// FP32 copy of the header fields used in FP32 by the mixed kernel.
+enum {
+    H32_T0 = 0,
+    H32_T1 = 6,
+    H32_JP = 12,
+    H32_AL = 20,
+    H32_MOM = 22,
+    H32_PAD = 27,
+    HDR32_STRIDE = 28
+};
+
+// Vertex records, FP64: k x, k y, slope dy/dx, slope dx/dy, edge dx, edge dy.
+enum { V_KX = 0, V_KY = 1, V_SYX = 2, V_SXY = 3, V_DX = 4, V_DY = 5, VTX_STRIDE = 6 };
+
+// Grid tables, SoA [DT_COUNT][cells].
+enum { DT_DIR = 0, DT_U0 = 3, DT_U1 = 9, DT_COUNT = 15 };
+
+// A unit of work: orientations [o_begin, o_end) whole (Mueller mode), or the
+// beam range [b_begin, b_end) of orientation o_begin (Jones mode).
+struct WorkItem {
+    int32_t o_begin;
+    int32_t o_end;
+    int64_t b_begin;
+    int64_t b_end;
+};
+
+}  // namespace mbs

@@ -1,0 +1,197 @@
+// po_math.h -- device math for the PO diffraction kernels (gfx950).
+//
+// sincos is evaluated here rather than through the device library so that the
+// hot loop costs ~30 VALU instructions per phasor and its accuracy is known:
+//   * reduction x = n*(pi/2) + r with n = rint(x*2/pi) and a two-constant
+//     FMA Cody-Waite step.  For |n| < 2^20 the first step fma(-n, P1, x) is
+//     exact (both x and n*P1 are multiples of 2^-52 or finer and the
+//     difference is below 1), the second adds one rounding;
+//   * Taylor polynomials of sin (to r^15) and cos (to r^16) on
+//     |r| <= pi/4 (+rounding slack), truncation below 7e-17 -- coefficients
+//     are exactly (-1)^j/(2j+1)! and (-1)^j/(2j)!, nothing else;
+//   * quadrant from n mod 4.
+// Phases reach ~1e4 rad for the largest particles, far inside |n| < 2^20.
+//
+// The FP32 variant used by the mixed-precision kernels reduces in FP64 the
+// same way and evaluates shorter Taylor polynomials (sin to r^9, cos to r^10)
+// in FP32 -- error ~1 ulp of FP32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mbs {
+
+constexpr double kTwoOverPi = 0.63661977236758134308;
+constexpr double kPio2Hi = 1.5707963267948966192;     // double(pi/2)
+constexpr double kPio2Lo = 6.123233995736766036e-17;  // pi/2 - kPio2Hi
+
+// Taylor coefficients (-1)^j / (2j+1)!  and  (-1)^j / (2j)!
+constexpr double kS1 = -1.0 / 6.0;
+constexpr double kS2 = 1.0 / 120.0;
+constexpr double kS3 = -1.0 / 5040.0;
+constexpr double kS4 = 1.0 / 362880.0;
+constexpr double kS5 = -1.0 / 39916800.0;
+constexpr double kS6 = 1.0 / 6227020800.0;
+constexpr double kS7 = -1.0 / 1307674368000.0;
+constexpr double kC1 = -1.0 / 2.0;
+constexpr double kC2 = 1.0 / 24.0;
+constexpr double kC3 = -1.0 / 720.0;
+constexpr double kC4 = 1.0 / 40320.0;
+constexpr double kC5 = -1.0 / 3628800.0;
+constexpr double kC6 = 1.0 / 479001600.0;
+constexpr double kC7 = -1.0 / 87178291200.0;
+constexpr double kC8 = 1.0 / 20922789888000.0;
+
+__device__ __forceinline__ void quadrant_fix(int q, double ps, double pc,
+                                             double &s, double &c)
+{
+    // sin(r + q*pi/2), cos(r + q*pi/2) from sin r, cos r
+    const bool swap = q & 1;
+    double ss = swap ? pc : ps;
+    double cc = swap ? ps : pc;
+    s = (q & 2) ? -ss : ss;
+    c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// One Horner step p*z + c as a single three-operand v_fma_f64 with the
+// coefficient in an SGPR pair.  Written as inline assembly because the
+// compiler otherwise keeps loop-invariant FP64 coefficients in VGPRs and
+// emits v_mov_b64 + v_fmac_f64 (two instructions) for every step.
+__device__ __forceinline__ double horner(double p, double z, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(z), "s"(c));
+    return r;
+}
+
+__device__ __forceinline__ void sincos_fast(double x, double &s, double &c)
+{
+    const double n = __builtin_rint(x * kTwoOverPi);
+    double r = __builtin_fma(-n, kPio2Hi, x);
+    r = __builtin_fma(-n, kPio2Lo, r);
+    const double z = r * r;
+    double ps = __builtin_fma(z, kS7, kS6);
+    ps = horner(ps, z, kS5);
+    ps = horner(ps, z, kS4);
+    ps = horner(ps, z, kS3);
+    ps = horner(ps, z, kS2);
+    ps = horner(ps, z, kS1);
+    ps = __builtin_fma(r * z, ps, r);
+    double pc = __builtin_fma(z, kC8, kC7);
+    pc = horner(pc, z, kC6);
+    pc = horner(pc, z, kC5);
+    pc = horner(pc, z, kC4);
+    pc = horner(pc, z, kC3);
+    pc = horner(pc, z, kC2);
+    pc = __builtin_fma(pc, z, kC1);
+    pc = __builtin_fma(pc, z, 1.0);
+    quadrant_fix((int)n, ps, pc, s, c);
+}
+
+// FP64 argument, FP32 result: FP64 reduction, FP32 Taylor polynomials.
+__device__ __forceinline__ void sincos_mixed(double x, float &s, float &c)
+{
+    const double n = __builtin_rint(x * kTwoOverPi);
+    double rd = __builtin_fma(-n, kPio2Hi, x);
+    rd = __builtin_fma(-n, kPio2Lo, rd);
+    const float r = (float)rd;
+    const float z = r * r;
+    float ps = __builtin_fmaf(z, (float)(1.0 / 362880.0), (float)(-1.0 / 5040.0));
+    ps = __builtin_fmaf(z, ps, (float)(1.0 / 120.0));
+    ps = __builtin_fmaf(z, ps, (float)(-1.0 / 6.0));
+    ps = __builtin_fmaf(r * z, ps, r);
+    float pc = __builtin_fmaf(z, (float)(-1.0 / 3628800.0), (float)(1.0 / 40320.0));
+    pc = __builtin_fmaf(z, pc, (float)(-1.0 / 720.0));
+    pc = __builtin_fmaf(z, pc, (float)(1.0 / 24.0));
+    pc = __builtin_fmaf(z, pc, -0.5f);
+    pc = __builtin_fmaf(z, pc, 1.0f);
+    const int q = (int)n;
+    const bool swap = q & 1;
+    const float ss = swap ? pc : ps;
+    const float cc = swap ? ps : pc;
+    s = (q & 2) ? -ss : ss;
+    c = ((q + 1) & 2) ? -cc : cc;
+}
+
+// 1/x to full FP64 precision: hardware estimate + two Newton steps.
+__device__ __forceinline__ double rcp_f64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+__device__ __forceinline__ float rcp_f32(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+
+// stable_sinc (src/handler/HandlerPO_fast.h:217-226)
+__device__ __forceinline__ double stable_sinc(double x)
+{
+    if (__builtin_fabs(x) < 1e-4) {
+        const double x2 = x * x;
+        return 1.0 - x2 / 6.0 + x2 * x2 / 120.0;
+    }
+    return sin(x) / x;
+}
+
+// add_stable_edge_quotient (src/handler/HandlerPO_fast.h:235-264): the exact
+// half-angle form of exp(i phaseStart)(exp(i k c delta) - 1)/c.  Returns the
+// increment (real part in .x, imaginary in .y) so the running sums stay in
+// registers; only lanes at a near-singular edge execute it.
+__device__ __forceinline__ double2 stable_edge_quotient(double pr, double pi,
+                                                        double delta, double cc,
+                                                        double k)
+{
+    const double z = k * cc * delta;
+    const double sh = stable_sinc(0.5 * z);
+    const double lr = -0.5 * k * k * cc * delta * delta * sh * sh;
+    const double li = k * delta * stable_sinc(z);
+    return make_double2(pr * lr - pi * li, pr * li + pi * lr);
+}
+
+// AddMuellerFromJones (src/math/Mueller.hpp:37-87): out += w * Mueller(J),
+// J given as 8 reals (00r 00i 01r 01i 10r 10i 11r 11i).
+template <typename T>
+__device__ __forceinline__ void add_mueller(double *m, const T *j, double w)
+{
+    const double j00r = j[0], j00i = j[1], j01r = j[2], j01i = j[3];
+    const double j10r = j[4], j10i = j[5], j11r = j[6], j11i = j[7];
+    const double a11 = j00r * j00r + j00i * j00i, a12 = j01r * j01r + j01i * j01i;
+    const double a21 = j10r * j10r + j10i * j10i, a22 = j11r * j11r + j11i * j11i;
+    double A1 = a11 + a21, A2 = a12 + a22;
+    m[0] += ((A1 + A2) * 0.5) * w;
+    m[1] += ((A1 - A2) * 0.5) * w;
+    A1 = a11 - a21;
+    A2 = a12 - a22;
+    m[4] += ((A1 + A2) * 0.5) * w;
+    m[5] += ((A1 - A2) * 0.5) * w;
+    double c1r = j00r * j01r + j00i * j01i, c1i = j00i * j01r - j00r * j01i;
+    double c2r = j11r * j10r + j11i * j10i, c2i = j11i * j10r - j11r * j10i;
+    m[2] += (-c1r - c2r) * w;
+    m[3] += (c2i - c1i) * w;
+    m[6] += (c2r - c1r) * w;
+    m[7] += (-c1i - c2i) * w;
+    c1r = j00r * j10r + j00i * j10i;
+    c1i = j00i * j10r - j00r * j10i;
+    c2r = j11r * j01r + j11i * j01i;
+    c2i = j11i * j01r - j11r * j01i;
+    m[8] += (-c1r - c2r) * w;
+    m[9] += (c2r - c1r) * w;
+    m[12] += (c1i - c2i) * w;
+    m[13] += (c2i + c1i) * w;
+    c1r = j00r * j11r + j00i * j11i;
+    c1i = j00i * j11r - j00r * j11i;
+    c2r = j01r * j10r + j01i * j10i;
+    c2i = j01i * j10r - j01r * j10i;
+    m[10] += (c1r + c2r) * w;
+    m[11] += (c1i - c2i) * w;
+    m[14] += (-c1i - c2i) * w;
+    m[15] += (c1r - c2r) * w;
+}
+
+}  // namespace mbs

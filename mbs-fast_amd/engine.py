@@ -1,0 +1,200 @@
+"""ctypes binding of libmbs_hip.so (the gfx950 engine, include/mbs_hip.h).
+
+There is no CPU fallback: if the library is missing, or no HIP device is
+visible, every entry point raises.  The library is loaded from this package
+directory (built in place by build.py), never from site-packages.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmbs_hip.so")
+_LIB = None
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("%s: %s" % (abi.STATUS_NAMES.get(status, status), message))
+        self.status = status
+
+
+def lib():
+    """Loads libmbs_hip.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError(abi.MBS_ERR_DEVICE,
+                              "libmbs_hip.so is not built (run __graft_entry__.build() "
+                              "or python mbs-fast_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.mbs_abi_version.restype = C.c_int
+        L.mbs_build_info.restype = C.c_char_p
+        L.mbs_visible_device_count.restype = C.c_int
+        L.mbs_check_runtime.argtypes = [C.POINTER(abi.mbs_device_info)]
+        L.mbs_check_runtime.restype = C.c_int
+        L.mbs_query_memory.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.mbs_query_memory.restype = C.c_int
+        L.mbs_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.mbs_ctx_create.restype = C.c_int
+        L.mbs_ctx_destroy.argtypes = [vp]
+        L.mbs_ctx_destroy.restype = C.c_int
+        L.mbs_last_error.argtypes = [vp]
+        L.mbs_last_error.restype = C.c_char_p
+        L.mbs_set_grid.argtypes = [vp, C.POINTER(abi.mbs_grid)]
+        L.mbs_set_grid.restype = C.c_int
+        L.mbs_upload_beams.argtypes = [vp, C.POINTER(abi.mbs_beam_batch),
+                                       C.POINTER(abi.mbs_consts)]
+        L.mbs_upload_beams.restype = C.c_int
+        L.mbs_diffract.argtypes = [vp, C.c_uint32]
+        L.mbs_diffract.restype = C.c_int
+        L.mbs_accumulate_mueller.argtypes = [vp, dp, dp]
+        L.mbs_accumulate_mueller.restype = C.c_int
+        L.mbs_fetch_per_orient.argtypes = [vp, dp]
+        L.mbs_fetch_per_orient.restype = C.c_int
+        L.mbs_diffract_batch.argtypes = [vp, C.POINTER(abi.mbs_grid),
+                                         C.POINTER(abi.mbs_beam_batch),
+                                         C.POINTER(abi.mbs_consts), C.c_uint32, dp, dp]
+        L.mbs_diffract_batch.restype = C.c_int
+        L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.mbs_device_result.restype = C.c_int
+        L.mbs_ctx_stream.argtypes = [vp]
+        L.mbs_ctx_stream.restype = vp
+        L.mbs_batch_stats.argtypes = [vp, C.POINTER(C.c_int64), dp, C.POINTER(C.c_int64), dp]
+        L.mbs_batch_stats.restype = C.c_int
+        L.mbs_set_target_blocks.argtypes = [vp, C.c_int]
+        L.mbs_set_target_blocks.restype = C.c_int
+        L.mbs_set_work_mode.argtypes = [vp, C.c_int]
+        L.mbs_set_work_mode.restype = C.c_int
+        L.mbs_set_timing.argtypes = [vp, C.c_int]
+        L.mbs_set_timing.restype = C.c_int
+        L.mbs_get_timing.argtypes = [vp, C.POINTER(abi.mbs_timing)]
+        L.mbs_get_timing.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def device_info():
+    info = abi.mbs_device_info()
+    rc = lib().mbs_check_runtime(C.byref(info))
+    if rc:
+        raise EngineError(rc, lib().mbs_last_error(None).decode())
+    return {"device_id": info.device_id, "visible": info.visible_devices,
+            "name": info.name.decode(), "arch": info.arch.decode(),
+            "cus": info.compute_units, "clock_khz": info.clock_khz,
+            "total_mem": info.total_mem, "free_mem": info.free_mem}
+
+
+class Context:
+    """One engine context (one HIP device, one stream)."""
+
+    def __init__(self, device=0):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.mbs_ctx_create(device, C.byref(h))
+        if rc:
+            raise EngineError(rc, self._L.mbs_last_error(None).decode())
+        self._h = h
+        self.grid = None
+        self.batch = None
+
+    def _check(self, rc):
+        if rc:
+            raise EngineError(rc, self._L.mbs_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.mbs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
+    @property
+    def stream(self):
+        return self._L.mbs_ctx_stream(self._h)
+
+    def set_grid(self, grid):
+        self._check(self._L.mbs_set_grid(self._h, C.byref(grid.struct())))
+        self.grid = grid
+
+    def upload(self, batch, consts):
+        self._check(self._L.mbs_upload_beams(self._h, C.byref(batch.struct()), C.byref(consts)))
+        self.batch = batch
+
+    def diffract(self, flags=0):
+        self._check(self._L.mbs_diffract(self._h, flags))
+
+    def accumulate(self, m_full, m_noshadow=None):
+        for a in (m_full, m_noshadow):
+            if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous
+                                  or a.size != self.grid.cells * 16):
+                raise ValueError("Mueller arrays must be C-contiguous float64 [cells*16]")
+        self._check(self._L.mbs_accumulate_mueller(
+            self._h, _d(m_full), _d(m_noshadow) if m_noshadow is not None else None))
+
+    def mueller(self, flags=0, noshadow=False):
+        """Runs the uploaded batch; returns (M, M_noshadow) as [n_az, n_zen+1, 16]."""
+        if noshadow:
+            flags |= abi.NOSHADOW
+        self.diffract(flags)
+        g = self.grid
+        m = np.zeros((g.n_az, g.n_zen + 1, 16))
+        mn = np.zeros_like(m) if noshadow else None
+        self.accumulate(m, mn)
+        return m, mn
+
+    def per_orient(self):
+        g = self.grid
+        out = np.zeros((self.batch.n_orient, g.n_az, g.n_zen + 1, 16))
+        self._check(self._L.mbs_fetch_per_orient(self._h, _d(out)))
+        return out
+
+    def device_result(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(self._L.mbs_device_result(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def stats(self):
+        nb, units, mean_e = C.c_int64(), C.c_double(), C.c_double()
+        hist = np.zeros(32, np.int64)
+        self._check(self._L.mbs_batch_stats(self._h, C.byref(nb), C.byref(units),
+                                            hist.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            C.byref(mean_e)))
+        return {"n_beams": nb.value, "units": units.value, "nv_hist": hist,
+                "mean_edges": mean_e.value}
+
+    def set_target_blocks(self, n):
+        self._check(self._L.mbs_set_target_blocks(self._h, int(n)))
+
+    def set_work_mode(self, mode):
+        """0 automatic, 1 whole orientations per item, 2 beam segments."""
+        self._check(self._L.mbs_set_work_mode(self._h, int(mode)))
+
+    def set_timing(self, enable=True):
+        self._check(self._L.mbs_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self):
+        t = abi.mbs_timing()
+        self._check(self._L.mbs_get_timing(self._h, C.byref(t)))
+        return {"upload_ms": t.upload_ms, "kernel_ms": t.kernel_ms, "reduce_ms": t.reduce_ms,
+                "n_items": t.n_items, "n_blocks": t.n_blocks}

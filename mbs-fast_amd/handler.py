@@ -1,0 +1,161 @@
+"""Host-side mirror of the reference's HandlerPO GPU interface.
+
+The reference exposes its GPU diffraction backend as member functions of
+HandlerPO (src/handler/HandlerPO.h:98-146), implemented in
+src/cuda/GpuDiffraction.cu and stubbed to ``false`` in CPU builds
+(src/handler/HandlerPO.cpp:2798-2883).  ``HandlerPOGpu`` offers the same
+members, argument meanings and error behaviour on top of the HIP engine:
+
+* results are ACCUMULATED (``+=``) into caller-owned grids, cell (p, t) for
+  p < nAz, t <= nZen (GpuDiffraction.cu:4751-4766);
+* the no-shadow grid is written only when ``ComputeNoShadow()`` is true
+  (``!m_fullOnly``, HandlerPO.cpp:982-985);
+* a batch the backend cannot take (incoherent mode, a beam with fewer than 3
+  or more than 31 aperture vertices) returns ``False`` and the caller raises,
+  as TracerPOTotal.cpp:5934-5939 does;
+* a device error prints a message and exits with status 2 unless
+  ``MBS_GPU_ALLOW_FALLBACK=1``, in which case the call returns ``False``
+  (GpuDiffraction.cu:694-710).
+
+``prepared`` is a :class:`abi.BeamBatch` (the flat form of
+``std::vector<PreparedOrientation>``); ``start``/``count`` select
+orientations as in the reference.
+"""
+import os
+import sys
+
+import numpy as np
+
+from . import abi
+from .engine import Context, EngineError
+
+
+class HandlerPOGpu:
+    def __init__(self, grid, consts, device=0, precision="fp64"):
+        self.grid = grid
+        self.consts = consts
+        self.isCoh = True
+        self.m_fullOnly = True
+        self.precision = precision
+        self._ctx = None
+        self._device = device
+
+    # -- HandlerPO settings mirrored from the reference -------------------
+    def SetFullOnly(self, value):
+        self.m_fullOnly = bool(value)
+
+    def IsFullOnly(self):
+        return self.m_fullOnly
+
+    def ComputeNoShadow(self):
+        return not self.m_fullOnly
+
+    # -- internals -----------------------------------------------------------
+    def _context(self):
+        if self._ctx is None:
+            self._ctx = Context(self._device)
+            self._ctx.set_grid(self.grid)
+        return self._ctx
+
+    def _flags(self):
+        f = abi.PREC_MIXED if self.precision == "mixed" else abi.PREC_FP64
+        if self.ComputeNoShadow():
+            f |= abi.NOSHADOW
+        return f
+
+    def _device_failure(self, err):
+        sys.stderr.write("GPU diffraction error: %s\n" % err)
+        if os.environ.get("MBS_GPU_ALLOW_FALLBACK") == "1":
+            return False
+        sys.exit(2)
+
+    @staticmethod
+    def _cells_view(m, n_az, n_zen1):
+        """Accepts Arr2D-shaped [nAz+1, nZen+1, 16] or [nAz, nZen+1, 16]
+        float64 arrays; returns the [nAz, nZen+1, 16] part."""
+        if m.dtype != np.float64 or m.ndim != 3 or m.shape[1:] != (n_zen1, 16):
+            raise ValueError("Mueller grid must be float64 [nAz(+1), nZen+1, 16]")
+        if m.shape[0] not in (n_az, n_az + 1):
+            raise ValueError("Mueller grid has %d phi rows, expected %d or %d"
+                             % (m.shape[0], n_az, n_az + 1))
+        return m[:n_az]
+
+    # -- HandlerPO GPU members -------------------------------------------------
+    def HandleOrientationsToLocalGpu(self, prepared, localM, localM_noshadow, start=0,
+                                     count=None, scale=1.0, waveIndex=0.0):
+        """HandlerPO::HandleOrientationsToLocalGpu (HandlerPO.h:109-120)."""
+        if not self.isCoh:
+            return False
+        if count is None:
+            count = prepared.n_orient - start
+        if start < 0 or count < 0 or start > prepared.n_orient:
+            return False
+        if scale != 1.0 or waveIndex != 0.0:
+            return False  # size-rescaled batches are the multi-size path
+        n = min(count, prepared.n_orient - start)
+        g = self.grid
+        if n == 0 or g.cells == 0:
+            return True
+        batch = prepared if (start == 0 and n == prepared.n_orient) else \
+            prepared.subset(list(range(start, start + n)))
+        mv = self._cells_view(localM, g.n_az, g.n_zen + 1)
+        mnv = self._cells_view(localM_noshadow, g.n_az, g.n_zen + 1) \
+            if self.ComputeNoShadow() else None
+        try:
+            ctx = self._context()
+            ctx.upload(batch, self.consts)
+            ctx.diffract(self._flags())
+            tmp = np.zeros((g.n_az, g.n_zen + 1, 16))
+            tmpn = np.zeros_like(tmp) if mnv is not None else None
+            ctx.accumulate(tmp, tmpn)
+        except EngineError as e:
+            if e.status in (abi.MBS_ERR_BAD_BEAM, abi.MBS_ERR_INVALID):
+                return False
+            return self._device_failure(e)
+        mv += tmp
+        if mnv is not None:
+            mnv += tmpn
+        return True
+
+    def HandleOrientationsToLocalGpuOnDevice(self, prepared, start, count, device, localM,
+                                             localM_noshadow, scale=1.0, waveIndex=0.0,
+                                             cacheToken=0):
+        """HandlerPO::HandleOrientationsToLocalGpuOnDevice (HandlerPO.h:121-130).
+        One GPU per process here: any device index other than the handler's is
+        refused."""
+        if device != self._device:
+            return False
+        return self.HandleOrientationsToLocalGpu(prepared, localM, localM_noshadow, start,
+                                                 count, scale, waveIndex)
+
+    def HandleBeamsToLocalGpu(self, prepared_orientation, localM, localM_noshadow):
+        """HandlerPO::HandleBeamsToLocalGpu (HandlerPO.h:103-105): one orientation."""
+        return self.HandleOrientationsToLocalGpu(prepared_orientation, localM,
+                                                 localM_noshadow, 0, 1)
+
+    def SelectGpuOrientationBatchSize(self, prepared, start, maxCount):
+        """HandlerPO::SelectGpuOrientationBatchSize (HandlerPO.h:145-147): the
+        largest count from ``start`` whose device working set fits in 80 % of
+        free device memory (partials + packed beams), at least 1."""
+        from .engine import lib
+        import ctypes as C
+        fr, tot = C.c_int64(), C.c_int64()
+        if lib().mbs_query_memory(C.byref(fr), C.byref(tot)) != 0:
+            return max(1, maxCount)
+        budget = 0.8 * fr.value
+        cells = self.grid.cells
+        n = max(1, maxCount)
+        oo = prepared.orient_offsets
+        while n > 1:
+            end = min(prepared.n_orient, start + n)
+            beams = int(oo[end] - oo[start])
+            need = beams * (36 + 40 + 14 + 6 * 31) * 8 + 64 * cells * 16 * 8 * 2
+            if need <= budget:
+                break
+            n //= 2
+        return n
+
+    def close(self):
+        if self._ctx is not None:
+            self._ctx.close()
+            self._ctx = None

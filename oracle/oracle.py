@@ -47,6 +47,7 @@ def lib():
         L.mbs_oracle_aperture.restype = None
         L.mbs_oracle_set_ref_sincos.argtypes = [C.c_int]
         L.mbs_oracle_set_ref_sincos.restype = None
+        L.mbs_oracle_threads.restype = C.c_int
         L.mbs_oracle_check_batch.argtypes = [vp]
         L.mbs_oracle_check_batch.restype = C.c_int
         _LIB = L
@@ -64,6 +65,10 @@ class ref_sincos:
     def __exit__(self, *exc):
         lib().mbs_oracle_set_ref_sincos(0)
         return False
+
+
+def threads():
+    return lib().mbs_oracle_threads()
 
 
 def _d(a):

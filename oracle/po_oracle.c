@@ -41,6 +41,19 @@ static int g_ref_sincos = 0;
 
 void mbs_oracle_set_ref_sincos(int mode) { g_ref_sincos = mode; }
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* Number of OpenMP threads the oracle's parallel loops use. */
+int mbs_oracle_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
 static void ref_poly_sincos(double x, double *s, double *c)
 {
     const double n = __builtin_round(x * 0.31830988618379067);

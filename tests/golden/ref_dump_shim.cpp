@@ -186,6 +186,11 @@ bool HandlerPO::HandleOrientationsToLocalGpu(
     localJ.push_back(tmp);
     localJns.push_back(tmp);
 
+    // MBS_FIXTURE_NO_EXPECTED=1: beams only (the expected grids are left
+    // zero), for callers that need the reference's beam sets but not the
+    // reference's CPU result for them.
+    const char *noExpected = std::getenv("MBS_FIXTURE_NO_EXPECTED");
+    const bool expected = !(noExpected && noExpected[0] == '1');
     const int end = std::min((int)prepared.size(), start + count);
     PutTag("BTCH");
     PutI32(end - start);
@@ -193,6 +198,8 @@ bool HandlerPO::HandleOrientationsToLocalGpu(
     {
         const PreparedOrientation &po = prepared[i];
         WriteOrientation(po);
+        if (!expected)
+            continue;
         if (!po.beams.empty())
             HandleBeamsToLocal(po, batchM, localJ, &localJns);
         AddToMuellerLocal(localJ, po.sinZenith, batchM, nAz, nZen);

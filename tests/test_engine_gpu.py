@@ -1,0 +1,221 @@
+"""Parity of the HIP engine with the FP64 oracle, and through it with the
+reference, on the reference's own prepared beams, through the C ABI.
+
+Bars (tests/metrics.py defines the measures):
+  FP64 engine vs oracle:    grid_l2 1e-10 (the north-star FP64 bar),
+                            row 2e-8, cell 2e-7
+  FP64 engine vs reference: grid_l2 / gate_l2 1e-6 (the reference CPU path
+                            evaluates phases with a polynomial whose cosine is
+                            off by up to 6.3e-9 and jumps by 1.3e-8 at odd
+                            multiples of pi/2 -- it is the reference that
+                            carries this error, the oracle with libm sincos
+                            does not)
+  mixed engine vs oracle:   P11 grid_l2 and P11 gate_l2 1e-4 (the north-star
+                            FP32 bar, in the reference's own gate norm);
+                            per-row and per-cell P11 errors are reported.
+The per-cell bound is looser than the aggregate ones because a coherent sum of
+hundreds of beams can cancel to a local M11 1e4 times below its row, and near
+an edge with |C| just above eps1 = 2.2e-7 the edge quotient amplifies the
+rounding of its phase arguments (|phase| ~ 1e2 rad) by up to 1/eps1 -- for the
+oracle as much as for the engine.
+"""
+import numpy as np
+import pytest
+
+import metrics as M
+from conftest import load_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(pkg):
+    c = pkg.engine.Context(0)
+    yield c
+    c.close()
+
+
+def _setup(pkg, fx):
+    abi = pkg.abi
+    return abi.Grid.from_fixture(fx), abi.BeamBatch.from_fixture(fx), abi.consts_from_fixture(fx)
+
+
+def _run(ctx, grid, batch, consts, flags=0, noshadow=False, target=None):
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_target_blocks(target or 4096)
+    return ctx.mueller(flags, noshadow)
+
+
+def check_fp64(a, b, theta, label):
+    print(label + ": " + M.summary(a, b, theta))
+    assert M.grid_l2(a, b, theta) < 1e-10, label
+    assert M.row(a, b) < 2e-8, label
+    assert M.cell(a, b) < 2e-7, label
+
+
+def test_device_is_gfx950(pkg):
+    info = pkg.engine.device_info()
+    print("device:", info)
+    assert info["arch"].startswith("gfx950")
+
+
+@pytest.mark.parametrize("name", ["config1_fixed", "fixed_absorbing"])
+def test_fixed_orientation_fp64(pkg, oracle, ctx, name):
+    fx = load_fixture(name)
+    grid, batch, consts = _setup(pkg, fx)
+    m, mn = _run(ctx, grid, batch, consts, noshadow=True)
+    o, on, _ = oracle.diffract(grid, batch, consts, noshadow=True)
+    check_fp64(m, o, grid.theta, name + " FP64 vs oracle")
+    check_fp64(mn, on, grid.theta, name + " FP64 no-shadow vs oracle")
+    # against the reference's own outputs on these beams
+    rows = fx.get("kept_theta_rows")
+    mr = m[:, rows] if rows is not None else m
+    th = grid.theta[rows] if rows is not None else grid.theta
+    for key in ("mueller_full", "mueller_fixed_dat"):
+        e = M.grid_l2(mr, fx[key], th)
+        print("%s FP64 vs reference %s: grid_l2 %.2e cell %.2e" % (name, key, e,
+                                                                    M.cell(mr, fx[key])))
+        assert e < 1e-6
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_orientation_average_both_work_modes(pkg, oracle, ctx, mode):
+    """Whole-orientation items (mode 1) and beam segments (mode 2) must both
+    equal the oracle, and the azimuth-averaged rows must pass the reference's
+    gate against the reference CLI's own .dat."""
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_work_mode(mode)
+    m, mn = _run(ctx, grid, batch, consts, noshadow=True)
+    ctx.set_work_mode(0)
+    o, on, _ = oracle.diffract(grid, batch, consts, noshadow=True)
+    check_fp64(m, o, grid.theta, "average mode=%d vs oracle" % mode)
+    check_fp64(mn, on, grid.theta, "average no-shadow mode=%d vs oracle" % mode)
+    for mm, key in ((m, "ref_total_dat"), (mn, "ref_total_dat_noshadow")):
+        avg = oracle.azimuth_average(grid, mm)
+        g = M.gate_l2(avg, fx[key])
+        print("average mode=%d %s gate_l2 vs reference .dat: %.2e" % (mode, key, g))
+        assert g < 1e-6
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_per_orientation_grids(pkg, oracle, ctx, mode):
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_work_mode(mode)
+    m, _ = ctx.mueller(pkg.abi.PER_ORIENT)
+    per = ctx.per_orient()
+    ctx.set_work_mode(0)
+    _, _, oper = oracle.diffract(grid, batch, consts, per_orient=True)
+    for o in range(batch.n_orient):
+        check_fp64(per[o], oper[o], grid.theta, "orientation %d" % o)
+    np.testing.assert_allclose(per.sum(axis=0), m, rtol=1e-12, atol=1e-15 * np.abs(m).max())
+
+
+def test_mixed_precision_p11(pkg, oracle, ctx):
+    for name in ("config1_fixed", "fixed_absorbing", "sobol_average"):
+        fx = load_fixture(name)
+        grid, batch, consts = _setup(pkg, fx)
+        m, _ = _run(ctx, grid, batch, consts, flags=pkg.abi.PREC_MIXED)
+        o, _, _ = oracle.diffract(grid, batch, consts)
+        a_m, a_o = oracle.azimuth_average(grid, m), oracle.azimuth_average(grid, o)
+        l2_11 = M.grid_l2(m[..., :1], o[..., :1], grid.theta)
+        avg11 = float(np.max(np.abs(a_m[:, 2] - a_o[:, 2]) / np.abs(a_o[:, 2])))
+        print("%s mixed vs oracle: P11 grid_l2 %.2e, azimuth-avg P11 max rel %.2e, "
+              "per-cell P11 max rel %.2e, all elements: %s"
+              % (name, l2_11, avg11, M.p11_rel(m, o), M.summary(m, o, grid.theta)))
+        g11 = M.gate_l2(a_m[:, :3], a_o[:, :3])
+        print("   azimuth-averaged P11 gate_l2 %.2e" % g11)
+        assert l2_11 < 1e-4
+        assert g11 < 1e-4
+
+
+def test_accumulates_and_is_deterministic(pkg, ctx):
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.diffract(0)
+    a = np.zeros(grid.cells * 16)
+    ctx.accumulate(a)
+    b = a.copy()
+    ctx.diffract(0)
+    ctx.accumulate(b)          # += semantics of HandleOrientationsToLocalGpu
+    np.testing.assert_array_equal(b, 2 * a)
+    ctx.diffract(0)
+    c = np.zeros_like(a)
+    ctx.accumulate(c)
+    np.testing.assert_array_equal(c, a)  # bit-reproducible
+
+
+def test_bad_beams_rejected(pkg, ctx):
+    fx = load_fixture("fixed_absorbing")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    for bad_nv in (2, 32):
+        nv = batch.beam_nv.copy()
+        nv[0] = bad_nv
+        vo = np.concatenate([[0], np.cumsum(nv)])
+        n = int(vo[-1])
+        bb = pkg.abi.BeamBatch(batch.orient_offsets, batch.weights, batch.beam_scalars, nv,
+                               batch.beam_external, vo, np.zeros(n), np.zeros(n), np.zeros(n),
+                               np.zeros(n), np.zeros(n, np.uint8))
+        with pytest.raises(pkg.engine.EngineError) as ei:
+            ctx.upload(bb, consts)
+        assert ei.value.status == pkg.abi.MBS_ERR_BAD_BEAM
+    with pytest.raises(pkg.engine.EngineError) as ei:
+        ctx.diffract(0x100)
+    assert ei.value.status in (pkg.abi.MBS_ERR_INVALID, pkg.abi.MBS_ERR_NO_BEAMS)
+
+
+def test_empty_batches(pkg, ctx):
+    """No orientations, and orientations without beams: zero result, no error."""
+    fx = load_fixture("fixed_absorbing")
+    grid, _, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    z = np.zeros(0)
+    for oo, w in (([0], []), ([0, 0, 0], [0.5, 0.5])):
+        bb = pkg.abi.BeamBatch(oo, w, np.zeros((0, 36)), np.zeros(0, np.int32),
+                               np.zeros(0, np.int32), [0], z, z, z, z, np.zeros(0, np.uint8))
+        ctx.upload(bb, consts)
+        m, _ = ctx.mueller(0)
+        assert not m.any()
+
+
+def test_handler_mirror_semantics(pkg, oracle):
+    """HandlerPOGpu: accumulate into Arr2D-shaped grids, no-shadow only when
+    requested, start/count selection, False for a batch it cannot take."""
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    h = pkg.handler.HandlerPOGpu(grid, consts)
+    Mg = np.zeros((grid.n_az + 1, grid.n_zen + 1, 16))
+    Mns = np.zeros_like(Mg)
+    assert h.HandleOrientationsToLocalGpu(batch, Mg, Mns, 2, 3)
+    assert not Mns.any()                      # full-only by default
+    o, _, _ = oracle.diffract(grid, batch.subset([2, 3, 4]), consts)
+    check_fp64(Mg[:-1], o, grid.theta, "handler mirror")
+    assert not Mg[-1].any()
+    h.SetFullOnly(False)
+    assert h.HandleOrientationsToLocalGpu(batch, Mg, Mns, 5, 100)   # count clipped
+    assert Mns.any()
+    assert h.SelectGpuOrientationBatchSize(batch, 0, 8) >= 1
+    h.isCoh = False
+    assert not h.HandleOrientationsToLocalGpu(batch, Mg, Mns)
+    h.close()
+
+
+def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
+    """A resampled batch at the orientation-average particle size (D=50 um) on
+    a reduced grid (the oracle is too slow for the full one)."""
+    syn = pkg.synthetic
+    fx = load_fixture("sobol_average")
+    src = pkg.abi.BeamBatch.from_fixture(fx)
+    consts = pkg.abi.consts_from_fixture(fx)
+    batch = syn.resample(src, [600, 900], scale=5.0, seed=3)
+    grid = syn.uniform_grid(90, 12)
+    m, _ = _run(ctx, grid, batch, consts)
+    o, _, _ = oracle.diffract(grid, batch, consts)
+    check_fp64(m, o, grid.theta, "synthetic D=50 FP64 vs oracle")
