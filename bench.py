@@ -84,7 +84,8 @@ def cpu_baseline_reference(budget_s=12.0):
         tgrid = os.path.join(work, "theta.grid")
         with open(tgrid, "w") as f:
             f.write("\n".join("%.10g" % t for t in th) + "\n")
-        n = max(threads, 4)
+        # the reference accepts power-of-two Sobol counts only
+        n = 1 << max(2, (threads - 1).bit_length())
         while True:
             d = os.path.join(work, "n%d" % n)
             os.makedirs(d)
@@ -98,7 +99,8 @@ def cpu_baseline_reference(budget_s=12.0):
             t2 = float(m.group(1))
             if t2 >= budget_s / 3 or n >= 256:
                 break
-            n = min(256, int(n * max(2.0, budget_s / 2 / max(t2, 0.05))))
+            grow = max(2.0, budget_s / 2 / max(t2, 0.05))
+            n = min(256, n << max(1, int(np.ceil(np.log2(grow)))))
         dump = os.path.join(d, "dump.bin")
         env = dict(os.environ, MBS_FIXTURE_DUMP=dump, MBS_FIXTURE_NO_EXPECTED="1")
         subprocess.run(_ref_cmd("mbs_po_dump", n, threads, "dmp", tgrid) + ["--backend", "gpu"],

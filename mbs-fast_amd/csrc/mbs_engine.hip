@@ -77,7 +77,7 @@ struct mbs_ctx {
     DevBuf<int32_t> nv, ext, vflag;
     DevBuf<int64_t> voff, ooff;
     DevBuf<uint8_t> ef_in;
-    DevBuf<double> hdr, vtx;
+    DevBuf<double> hdr, vtx, vdd;
     DevBuf<float> hdr32;
     // work and results
     DevBuf<WorkItem> items;
@@ -222,6 +222,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->ef_in.release();
     ctx->hdr.release();
     ctx->vtx.release();
+    ctx->vdd.release();
     ctx->hdr32.release();
     ctx->items.release();
     ctx->part.release();
@@ -345,6 +346,7 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
     HIP_TRY(ctx, ctx->hdr.ensure((size_t)nb * HDR_STRIDE));
     HIP_TRY(ctx, ctx->hdr32.ensure((size_t)nb * HDR32_STRIDE));
     HIP_TRY(ctx, ctx->vtx.ensure((size_t)nvert * VTX_STRIDE));
+    HIP_TRY(ctx, ctx->vdd.ensure((size_t)nvert * VDD_STRIDE));
     HIP_TRY(ctx, ctx->vflag.ensure(nvert));
     hipStream_t s = ctx->stream;
     if (ctx->timing)
@@ -384,7 +386,7 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
             ctx->syx_in.p, ctx->sxy_in.p, ctx->ef_in.p, c->wave_index,
             sg * c->inv_compl_wave[0], sg * c->inv_compl_wave[1], c->compl_wave[0],
             c->compl_wave[1], ctx->hdr.p, ctx->hdr32.p,
-            ctx->vtx.p, ctx->vflag.p);
+            ctx->vtx.p, ctx->vdd.p, ctx->vflag.p);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -464,7 +466,7 @@ template <int PREC, int NS, int JONES, int PER>
 static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
     diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
-        a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vflag, a.nv,
+        a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd, a.vflag, a.nv,
         a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns, a.per_orient);
 }
 
@@ -538,6 +540,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     a.hdr = ctx->hdr.p;
     a.hdr32 = ctx->hdr32.p;
     a.vtx = ctx->vtx.p;
+    a.vdd = ctx->vdd.p;
     a.vflag = ctx->vflag.p;
     a.nv = ctx->nv.p;
     a.voff = ctx->voff.p;

@@ -114,6 +114,7 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
                                   double *__restrict__ hdr,
                                   float *__restrict__ hdr32,
                                   double *__restrict__ vtx,
+                                  double *__restrict__ vdd,
                                   int32_t *__restrict__ vflag)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,8 +214,9 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
         v[V_SYX] = (f & 1) ? syx[v0 + i] : 0.0;
         v[V_SXY] = (f & 2) ? sxy[v0 + i] : 0.0;
         const int j = (i + 1 < nv) ? i + 1 : 0;
-        v[V_DX] = vx[v0 + j] - vx[v0 + i];
-        v[V_DY] = vy[v0 + j] - vy[v0 + i];
+        double *w = vdd + (size_t)(v0 + i) * VDD_STRIDE;
+        w[D_DX] = vx[v0 + j] - vx[v0 + i];
+        w[D_DY] = vy[v0 + j] - vy[v0 + i];
         vflag[v0 + i] = f;
     }
 }
@@ -234,9 +236,9 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
 template <int ACC_NS>
 __device__ __forceinline__ void beam_dir_f64(
     const double *__restrict__ h, const double *__restrict__ vtx,
-    const int32_t *__restrict__ vflag, int nv, const double *d,
-    const double *u0, const double *u1, double k, double eps1, double *j,
-    double *jn)
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
+    const double *d, const double *__restrict__ su, const double2 *__restrict__ tab,
+    double k, double eps1, double *j, double *jn)
 {
     // A, B: projections of k - k0 = b - d on the aperture axes
     const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
@@ -251,28 +253,28 @@ __device__ __forceinline__ void beam_dir_f64(
     double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
     double maxph = __builtin_fabs(ph);
     double s0, c0;
-    sincos_fast(ph + psi, s0, c0);
+    sincos_table(ph + psi, tab, s0, c0);
     double sp = s0, cp = c0, sr = 0.0, si = 0.0;
     for (int e = 0; e < nv; ++e) {
         const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope_x = v[V_SYX], slope_y = v[V_SXY];
-        const double dxe = v[V_DX], dye = v[V_DY];
+        const double slope = useB ? v[V_SYX] : v[V_SXY];
         const int f = vflag[e];
         double sn = s0, cn = c0;
         if (e + 1 < nv) {
             const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
             maxph = fmax(maxph, __builtin_fabs(pn));
-            sincos_fast(pn + psi, sn, cn);
+            sincos_table(pn + psi, tab, sn, cn);
         }
-        const double cc = __builtin_fma(useB ? slope_x : slope_y, mul, add);
+        const double cc = __builtin_fma(slope, mul, add);
         const bool valid = (f & vmask) != 0;
         const bool big = __builtin_fabs(cc) > eps1;
-        const double r = rcp_f64(cc);
+        const double r = rcp_f64_fast(cc);
         const double inv = (valid && big) ? r : 0.0;
         sr = __builtin_fma(cn - cp, inv, sr);
         si = __builtin_fma(sn - sp, inv, si);
         if (__builtin_expect(valid && !big, 0)) {
-            const double2 q = stable_edge_quotient(cp, sp, useB ? dxe : dye, cc, k);
+            const double *w = vdd + (size_t)e * VDD_STRIDE;
+            const double2 q = stable_edge_quotient(cp, sp, useB ? w[D_DX] : w[D_DY], cc, k);
             sr += q.x;
             si += q.y;
         }
@@ -291,13 +293,20 @@ __device__ __forceinline__ void beam_dir_f64(
         fr = h[H_AL] * gr - h[H_AL + 1] * gi;
         fi = h[H_AL] * gi + h[H_AL + 1] * gr;
         double sd, cd;
-        sincos_fast(psi, sd, cd);
+        sincos_table(psi, tab, sd, cd);
         const double tr = fr * cd - fi * sd;
         fi = fr * sd + fi * cd;
         fr = tr;
     }
     if (__builtin_isnan(fr))
         return;
+    // rotation vectors of this lane's direction, from LDS ([12][256])
+    const int l = threadIdx.x;
+    double u0[6], u1[6];
+    for (int q = 0; q < 6; ++q) {
+        u0[q] = su[q * 256 + l];
+        u1[q] = su[(6 + q) * 256 + l];
+    }
     const double *T0 = h + H_T0, *T1 = h + H_T1;
     const double r00 = u0[0] * T0[0] + u0[1] * T0[1] + u0[2] * T0[2] + u0[3] * T0[3] + u0[4] * T0[4] + u0[5] * T0[5];
     const double r01 = u0[0] * T1[0] + u0[1] * T1[1] + u0[2] * T1[2] + u0[3] * T1[3] + u0[4] * T1[4] + u0[5] * T1[5];
@@ -337,8 +346,9 @@ __device__ __forceinline__ void beam_dir_f64(
 template <int ACC_NS>
 __device__ __forceinline__ void beam_dir_mixed(
     const double *__restrict__ h, const float *__restrict__ h32,
-    const double *__restrict__ vtx, const int32_t *__restrict__ vflag, int nv,
-    const double *d, const float *u0, const float *u1, double k, double eps1,
+    const double *__restrict__ vtx, const double *__restrict__ vdd,
+    const int32_t *__restrict__ vflag, int nv, const double *d,
+    const float *__restrict__ su, const float2 *__restrict__ tab, double k, double eps1,
     float *j, float *jn)
 {
     const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
@@ -351,21 +361,20 @@ __device__ __forceinline__ void beam_dir_mixed(
     double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
     double maxph = __builtin_fabs(ph);
     float s0, c0;
-    sincos_mixed(ph + psi, s0, c0);
+    sincos_table_f32(ph + psi, tab, s0, c0);
     float sp = s0, cp = c0, sr = 0.0f, si = 0.0f;
     for (int e = 0; e < nv; ++e) {
         const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope_x = v[V_SYX], slope_y = v[V_SXY];
-        const double dxe = v[V_DX], dye = v[V_DY];
+        const double slope = useB ? v[V_SYX] : v[V_SXY];
         const int f = vflag[e];
         float sn = s0, cn = c0;
         if (e + 1 < nv) {
             const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
             maxph = fmax(maxph, __builtin_fabs(pn));
-            sincos_mixed(pn + psi, sn, cn);
+            sincos_table_f32(pn + psi, tab, sn, cn);
         }
         // the edge coefficient is cancellation-prone: FP64
-        const double cc = __builtin_fma(useB ? slope_x : slope_y, mul, add);
+        const double cc = __builtin_fma(slope, mul, add);
         const bool valid = (f & vmask) != 0;
         const bool big = __builtin_fabs(cc) > eps1;
         const float r = rcp_f32((float)cc);
@@ -373,8 +382,9 @@ __device__ __forceinline__ void beam_dir_mixed(
         sr = __builtin_fmaf(cn - cp, inv, sr);
         si = __builtin_fmaf(sn - sp, inv, si);
         if (__builtin_expect(valid && !big, 0)) {
-            const double2 q = stable_edge_quotient((double)cp, (double)sp, useB ? dxe : dye,
-                                                   cc, k);
+            const double *w = vdd + (size_t)e * VDD_STRIDE;
+            const double2 q = stable_edge_quotient((double)cp, (double)sp,
+                                                   useB ? w[D_DX] : w[D_DY], cc, k);
             sr += (float)q.x;
             si += (float)q.y;
         }
@@ -390,12 +400,18 @@ __device__ __forceinline__ void beam_dir_mixed(
         const double ar = h[H_AL] * gr - h[H_AL + 1] * gi;
         const double ai = h[H_AL] * gi + h[H_AL + 1] * gr;
         float sd, cd;
-        sincos_mixed(psi, sd, cd);
+        sincos_table_f32(psi, tab, sd, cd);
         fr = (float)ar * cd - (float)ai * sd;
         fi = (float)ar * sd + (float)ai * cd;
     }
     if (__builtin_isnan(fr))
         return;
+    const int l = threadIdx.x;
+    float u0[6], u1[6];
+    for (int q = 0; q < 6; ++q) {
+        u0[q] = su[q * 256 + l];
+        u1[q] = su[(6 + q) * 256 + l];
+    }
     const float *T0 = h32 + H32_T0, *T1 = h32 + H32_T1;
     const float r00 = u0[0] * T0[0] + u0[1] * T0[1] + u0[2] * T0[2] + u0[3] * T0[3] + u0[4] * T0[4] + u0[5] * T0[5];
     const float r01 = u0[0] * T1[0] + u0[1] * T1[1] + u0[2] * T1[2] + u0[3] * T1[3] + u0[4] * T1[4] + u0[5] * T1[5];
@@ -453,6 +469,7 @@ struct KernelArgs {
     const double *hdr;       // [n_beams][HDR_STRIDE]
     const float *hdr32;      // [n_beams][HDR32_STRIDE]
     const double *vtx;       // [n_vert][VTX_STRIDE]
+    const double *vdd;       // [n_vert][VDD_STRIDE]
     const int32_t *vflag;    // [n_vert]
     const int32_t *nv;       // [n_beams]
     const int64_t *voff;     // [n_beams+1]
@@ -497,7 +514,8 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
     const double *__restrict__ tab, const double *__restrict__ hdr,
     const float *__restrict__ hdr32, const double *__restrict__ vtx,
-    const int32_t *__restrict__ vflag, const int32_t *__restrict__ nv,
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag,
+    const int32_t *__restrict__ nv,
     const int64_t *__restrict__ voff, const int32_t *__restrict__ ext,
     const int64_t *__restrict__ ooff, const double *__restrict__ weights, double k,
     double eps1, double *__restrict__ out, double *__restrict__ out_ns,
@@ -513,11 +531,23 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
     using U = typename std::conditional<PREC == 1, float, double>::type;
-    U u0[6], u1[6];
-    for (int q = 0; q < 6; ++q) {
-        u0[q] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
-        u1[q] = (U)tab[(size_t)(DT_U1 + q) * cells + c];
+    using U2 = typename std::conditional<PREC == 1, float2, double2>::type;
+    // LDS: this lane's rotation vectors u0, u1 ([12][256], read once per beam
+    // after the vertex loop instead of occupying 12-24 VGPRs through it) and
+    // the 256-entry phasor table e^{i 2 pi j/256}
+    __shared__ U s_u[12 * 256];
+    __shared__ U2 s_tab[kTabN];
+    for (int q = 0; q < 12; ++q)
+        s_u[q * 256 + threadIdx.x] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
+    {
+        double ts, tc;
+        sincospi((double)threadIdx.x / 128.0, &ts, &tc);
+        if constexpr (PREC == 1)
+            s_tab[threadIdx.x] = make_float2((float)tc, (float)ts);
+        else
+            s_tab[threadIdx.x] = make_double2(tc, ts);
     }
+    __syncthreads();
     const WorkItem it = items[item];
     for (int o = it.o_begin; o < it.o_end; ++o) {
         int64_t b0 = ooff[o], b1 = ooff[o + 1];
@@ -531,21 +561,22 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
             const double *h = hdr + (size_t)b * HDR_STRIDE;
             const int64_t v0 = voff[b];
             const int nvb = nv[b];
+            const double *bv = vtx + (size_t)v0 * VTX_STRIDE;
+            const double *bd = vdd + (size_t)v0 * VDD_STRIDE;
             if constexpr (PREC == 0) {
                 if (NS && ext[b] == 0)
-                    beam_dir_f64<1>(h, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
-                                            nvb, d, u0, u1, k, eps1, j, jn);
+                    beam_dir_f64<1>(h, bv, bd, vflag + v0, nvb, d, s_u, s_tab, k, eps1, j, jn);
                 else
-                    beam_dir_f64<0>(h, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
-                                            nvb, d, u0, u1, k, eps1, j, nullptr);
+                    beam_dir_f64<0>(h, bv, bd, vflag + v0, nvb, d, s_u, s_tab, k, eps1, j,
+                                    nullptr);
             } else {
                 const float *h32 = hdr32 + (size_t)b * HDR32_STRIDE;
                 if (NS && ext[b] == 0)
-                    beam_dir_mixed<1>(h, h32, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
-                                      nvb, d, u0, u1, k, eps1, j, jn);
+                    beam_dir_mixed<1>(h, h32, bv, bd, vflag + v0, nvb, d, s_u, s_tab, k, eps1,
+                                      j, jn);
                 else
-                    beam_dir_mixed<0>(h, h32, vtx + (size_t)v0 * VTX_STRIDE, vflag + v0,
-                                      nvb, d, u0, u1, k, eps1, j, nullptr);
+                    beam_dir_mixed<0>(h, h32, bv, bd, vflag + v0, nvb, d, s_u, s_tab, k, eps1,
+                                      j, nullptr);
             }
         }
         if (!live)

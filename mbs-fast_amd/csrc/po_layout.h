@@ -33,8 +33,11 @@ enum {
     HDR32_STRIDE = 28
 };
 
-// Vertex records, FP64: k x, k y, slope dy/dx, slope dx/dy, edge dx, edge dy.
-enum { V_KX = 0, V_KY = 1, V_SYX = 2, V_SXY = 3, V_DX = 4, V_DY = 5, VTX_STRIDE = 6 };
+// Vertex records, FP64, 32 bytes: k x, k y, slope dy/dx, slope dx/dy (the
+// slopes of edge e = vertex e -> vertex e+1).  The edge deltas dx, dy, read
+// only by the rare near-singular branch, live in a separate array.
+enum { V_KX = 0, V_KY = 1, V_SYX = 2, V_SXY = 3, VTX_STRIDE = 4 };
+enum { D_DX = 0, D_DY = 1, VDD_STRIDE = 2 };
 
 // Grid tables, SoA [DT_COUNT][cells].
 enum { DT_DIR = 0, DT_U0 = 3, DT_U1 = 9, DT_COUNT = 15 };

@@ -89,6 +89,54 @@ __device__ __forceinline__ void sincos_fast(double x, double &s, double &c)
     quadrant_fix((int)n, ps, pc, s, c);
 }
 
+// ---------------------------------------------------------------------------
+// Table-driven phasor e^{i x}: x = j (2 pi/256) + d with j = rint(x 256/2pi),
+// |d| <= pi/256.  e^{i j 2pi/256} comes from a 256-entry table in LDS (filled
+// once per workgroup from the device library's sincos), e^{i d} from Taylor
+// polynomials (sin to d^7, cos to d^6, truncation < 1e-20), and one complex
+// product joins them.  The reduction is exact for |j| < 2^20 with
+// kTabStep1 = double(pi)/128 (an exact scaling of double(pi)): both x and
+// j*kTabStep1 are multiples of 2^-58 and |x - j*kTabStep1| < 2^-6.
+// ---------------------------------------------------------------------------
+constexpr int kTabN = 256;
+constexpr double kTabInvStep = 40.743665431525205956;  // 256 / (2 pi)
+constexpr double kTabStep1 = 3.141592653589793116 / 128.0;
+constexpr double kTabStep2 = 1.2246467991473532e-16 / 128.0;
+
+__device__ __forceinline__ void sincos_table(double x, const double2 *__restrict__ tab,
+                                             double &s, double &c)
+{
+    const double j = __builtin_rint(x * kTabInvStep);
+    double d = __builtin_fma(-j, kTabStep1, x);
+    d = __builtin_fma(-j, kTabStep2, d);
+    const double2 e = tab[(int)j & (kTabN - 1)];
+    const double z = d * d;
+    double ps = __builtin_fma(z, -1.0 / 5040.0, 1.0 / 120.0);
+    ps = horner(ps, z, -1.0 / 6.0);
+    const double sd = __builtin_fma(d * z, ps, d);
+    double pc = __builtin_fma(z, -1.0 / 720.0, 1.0 / 24.0);
+    pc = horner(pc, z, -0.5);
+    const double cd = __builtin_fma(pc, z, 1.0);
+    s = __builtin_fma(e.y, cd, e.x * sd);
+    c = __builtin_fma(e.x, cd, -(e.y * sd));
+}
+
+// FP32 result from an FP64 argument, same table (converted to FP32 values).
+__device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restrict__ tab,
+                                                 float &s, float &c)
+{
+    const double j = __builtin_rint(x * kTabInvStep);
+    double dd = __builtin_fma(-j, kTabStep1, x);
+    dd = __builtin_fma(-j, kTabStep2, dd);
+    const float d = (float)dd;
+    const float2 e = tab[(int)j & (kTabN - 1)];
+    const float z = d * d;
+    const float sd = __builtin_fmaf(d * z, __builtin_fmaf(z, 1.0f / 120.0f, -1.0f / 6.0f), d);
+    const float cd = __builtin_fmaf(z, __builtin_fmaf(z, 1.0f / 24.0f, -0.5f), 1.0f);
+    s = __builtin_fmaf(e.y, cd, e.x * sd);
+    c = __builtin_fmaf(e.x, cd, -(e.y * sd));
+}
+
 // FP64 argument, FP32 result: FP64 reduction, FP32 Taylor polynomials.
 __device__ __forceinline__ void sincos_mixed(double x, float &s, float &c)
 {
@@ -121,6 +169,15 @@ __device__ __forceinline__ double rcp_f64(double x)
     r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
     r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
     return r;
+}
+
+// 1/x from the hardware estimate and one Newton step: relative error below
+// (estimate error)^2, about 1e-14 -- used for the edge coefficients, whose
+// quotients carry the rounding of ~1e2 rad phase arguments anyway.
+__device__ __forceinline__ double rcp_f64_fast(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
 }
 
 __device__ __forceinline__ float rcp_f32(float x)
