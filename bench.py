@@ -20,8 +20,15 @@ mbs_diffract (main kernel + deterministic reduction).  The roofline figure
 uses the main kernel's duration measured with HIP events recorded on the
 engine's own stream (mbs_get_timing).
 
+HBM traffic per launch ("roofline.traffic") is read from the committed
+rocprofv3 summary TRAFFIC_FILE (FETCH_SIZE + WRITE_SIZE of the main kernel on
+this workload, each scaled by what the same counter reads for a plain device
+copy of known size, `bench.py --calibrate-copy`); counters cannot be read from
+inside the timed run.
+
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one
-GPU; the orientation sharding across GPUs is described in DESIGN.md.
+GPU (orientations would shard across GPUs, each GPU holding its own Mueller
+grid, with one sum-reduce of the grids at the end).
 """
 import argparse
 import json
@@ -37,6 +44,33 @@ sys.path.insert(0, ROOT)
 ORIENT_PER_STEP = 16
 BEAMS_MIN, BEAMS_MAX = 2000, 15000
 FP64_PEAK = 78.6e12   # MI355X FP64 vector (= FP64 MFMA) FLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "diffract_hbm_traffic_d50_541x96.txt")
+
+
+def measured_traffic():
+    """{"mixed": bytes, "fp64": bytes} per launch from TRAFFIC_FILE's
+    'traffic_bytes <kernel> <bytes>' lines, or {} if it is absent."""
+    out = {}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes":
+                    out[parts[1]] = float(parts[2])
+    return out
+
+
+def calibrate_copy():
+    """Plain device-to-device copy of 1 GiB, three times (torch's elementwise
+    copy kernel): the reference launch for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 1 << 30
+    a = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n, "launches": 3}))
 
 
 def unit_flops(batch):
@@ -185,6 +219,7 @@ def run(args):
     ctx.upload(batch, consts)
     ctx.set_timing(True)
     results = {}
+    traffic = measured_traffic()
     for name, flags in (("mixed", abi.PREC_MIXED), ("fp64", abi.PREC_FP64)):
         for _ in range(args.warmup):
             ctx.diffract(flags)
@@ -204,7 +239,8 @@ def run(args):
             "kernel_ms": kms,
             "roofline": {"bound": "mfma", "achieved": flops_launch / (kms * 1e-3) / 1e12,
                          "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                         "frac": flops_launch / (kms * 1e-3) / FP64_PEAK, "traffic": None},
+                         "frac": flops_launch / (kms * 1e-3) / FP64_PEAK,
+                         "traffic": traffic.get(name)},
             "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
         }
     ctx.close()
@@ -265,12 +301,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=12.0,
                     help="seconds of CPU time for the baseline sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (orientations "
-              "would shard across GPUs with one sum-reduce of the Mueller grid; see "
-              "DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (orientations "
+              "would shard across GPUs with one sum-reduce of the Mueller grid).  "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.steps < 1 or args.warmup < 0:
         sys.exit("--steps must be >= 1 and --warmup >= 0")
     run(args)

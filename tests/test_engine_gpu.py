@@ -2,7 +2,7 @@
 reference, on the reference's own prepared beams, through the C ABI.
 
 Bars (tests/metrics.py defines the measures):
-  FP64 engine vs oracle:    grid_l2 1e-10 (the north-star FP64 bar),
+  FP64 engine vs oracle:    grid_l2 1e-10 (the FP64 tolerance of the path),
                             row 2e-8, cell 2e-7
   FP64 engine vs reference: grid_l2 / gate_l2 1e-6 (the reference CPU path
                             evaluates phases with a polynomial whose cosine is
@@ -10,8 +10,8 @@ Bars (tests/metrics.py defines the measures):
                             multiples of pi/2 -- it is the reference that
                             carries this error, the oracle with libm sincos
                             does not)
-  mixed engine vs oracle:   P11 grid_l2 and P11 gate_l2 1e-4 (the north-star
-                            FP32 bar, in the reference's own gate norm);
+  mixed engine vs oracle:   P11 grid_l2 and P11 gate_l2 1e-4 (the FP32
+                            tolerance of the path, in the reference's own gate norm);
                             per-row and per-cell P11 errors are reported.
 The per-cell bound is looser than the aggregate ones because a coherent sum of
 hundreds of beams can cancel to a local M11 1e4 times below its row, and near
