@@ -183,6 +183,14 @@ int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid,
                        const mbs_beam_batch *batch, const mbs_consts *consts,
                        uint32_t flags, double *m_full, double *m_noshadow);
 
+/* Azimuth (laboratory-alpha) average of the last mbs_diffract result, on the
+ * device: the rows HandlerPOTotal::WriteMatricesToFile writes to its .dat
+ * (src/handler/HandlerPOTotal.cpp:23-122), with the pole forms of
+ * src/handler/PoleMueller.h:25-55.  rows: [n_zen+1][18] = theta in degrees,
+ * 2*pi*dcos, M11..M44 (row-major 4x4).  noshadow != 0 averages the no-shadow
+ * grid (requires MBS_NOSHADOW on the last mbs_diffract).  Overwrites rows. */
+int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows);
+
 /* Device pointer of the result grid of the last mbs_diffract (FP64,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
 int mbs_device_result(mbs_ctx *ctx, const double **m_full,

@@ -81,7 +81,7 @@ struct mbs_ctx {
     DevBuf<float> hdr32;
     // work and results
     DevBuf<WorkItem> items;
-    DevBuf<double> part, part_ns, res, res_ns, per_orient;
+    DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows;
     std::vector<WorkItem> h_items;
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
@@ -230,6 +230,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->res.release();
     ctx->res_ns.release();
     ctx->per_orient.release();
+    ctx->avg_rows.release();
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev)
@@ -704,6 +705,24 @@ int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid, const mbs_beam_batch 
     if (rc)
         return rc;
     return mbs_accumulate_mueller(ctx, m_full, m_noshadow);
+}
+
+int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows)
+{
+    if (!ctx || !rows)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_azimuth_average: NULL argument");
+    if (!ctx->have_result)
+        return fail(ctx, MBS_ERR_INVALID, "no result: call mbs_diffract first");
+    if (noshadow && !ctx->have_ns)
+        return fail(ctx, MBS_ERR_INVALID, "no-shadow result was not requested (MBS_NOSHADOW)");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->n_zen1 * 18;
+    HIP_TRY(ctx, ctx->avg_rows.ensure(n));
+    azimuth_average_kernel<<<ctx->n_zen1, 64, 0, ctx->stream>>>(
+        noshadow ? ctx->res_ns.p : ctx->res.p, ctx->theta.p, ctx->n_az, ctx->n_zen1 - 1,
+        ctx->az_step, ctx->avg_rows.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return fetch(ctx, ctx->avg_rows.p, n, rows, false);
 }
 
 int mbs_device_result(mbs_ctx *ctx, const double **m_full, const double **m_noshadow)

@@ -658,4 +658,81 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
         dst[q] = m[q];
 }
 
+// Azimuth (laboratory-alpha) average of an orientation-summed Mueller grid,
+// one row per zenith: HandlerPOTotal::WriteMatricesToFile
+// (src/handler/HandlerPOTotal.cpp:23-122).  Row t = [theta in degrees,
+// 2 pi dcos, M11..M44] with Mbar = (1/nAz) sum_p M(p,t) L(-phi_p), where
+// L(a) rotates the Stokes Q/U block by 2a (at the poles the plain mean is
+// taken); then the pole forms of PoleMueller (src/handler/PoleMueller.h:25-55)
+// and the solid-angle weight of ScatteringRange::Compute2PiDcos
+// (src/handler/Handler.h:70-125) in the stable sin*sin form.  One wave per
+// row; the 64 lanes stride over azimuths and combine by a fixed butterfly, so
+// the result is deterministic.
+__global__ void __launch_bounds__(64) azimuth_average_kernel(
+    const double *__restrict__ m, const double *__restrict__ theta, int n_az, int n_zen,
+    double az_step, double *__restrict__ rows)
+{
+    const int t = blockIdx.x;
+    const int nZ = n_zen + 1;
+    const double th = theta[t];
+    const double tol = 256.0 * 2.220446049250313e-16;
+    const bool fwd = __builtin_fabs(th) <= tol;
+    const bool bwd = __builtin_fabs(th - kPi) <= tol;
+    double s[16];
+    for (int q = 0; q < 16; ++q)
+        s[q] = 0.0;
+    for (int p = threadIdx.x; p < n_az; p += 64) {
+        const double *c = m + ((size_t)p * nZ + t) * 16;
+        double sn, cs;
+        sincos(2.0 * (-(double)p * az_step), &sn, &cs);
+        for (int r = 0; r < 4; ++r) {
+            const double *row = c + 4 * r;
+            if (fwd || bwd) {
+                for (int q = 0; q < 4; ++q)
+                    s[4 * r + q] += row[q];
+            } else {
+                s[4 * r + 0] += row[0];
+                s[4 * r + 1] += row[1] * cs - row[2] * sn;
+                s[4 * r + 2] += row[1] * sn + row[2] * cs;
+                s[4 * r + 3] += row[3];
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 16; ++q)
+            s[q] += __shfl_xor(s[q], off, 64);
+    if (threadIdx.x != 0)
+        return;
+    for (int q = 0; q < 16; ++q)
+        s[q] /= (double)n_az;
+    if (fwd || bwd) {
+        const double m00 = s[0], m33 = s[15], m03 = s[3];
+        const double m11 = fwd ? 0.5 * (s[5] + s[10]) : 0.5 * (s[5] - s[10]);
+        for (int q = 0; q < 16; ++q)
+            s[q] = 0.0;
+        s[0] = m00;
+        s[5] = m11;
+        s[10] = fwd ? m11 : -m11;
+        s[15] = m33;
+        s[3] = m03;
+        s[12] = m03;
+    }
+    double lo, hi;
+    if (t == 0) {
+        lo = theta[0];
+        hi = 0.5 * (theta[0] + theta[n_zen > 0 ? 1 : 0]);
+    } else if (t == n_zen) {
+        lo = 0.5 * (theta[n_zen - 1] + theta[n_zen]);
+        hi = theta[n_zen];
+    } else {
+        lo = 0.5 * (theta[t - 1] + theta[t]);
+        hi = 0.5 * (theta[t] + theta[t + 1]);
+    }
+    double *o = rows + (size_t)t * 18;
+    o[0] = (th * 180.0) / kPi;  // RadToDeg, src/common/common.cpp:140-143
+    o[1] = 2.0 * sin(0.5 * (lo + hi)) * sin(0.5 * (hi - lo)) * (2.0 * kPi);
+    for (int q = 0; q < 16; ++q)
+        o[2 + q] = s[q];
+}
+
 }  // namespace mbs

@@ -61,6 +61,8 @@ def lib():
                                          C.POINTER(abi.mbs_beam_batch),
                                          C.POINTER(abi.mbs_consts), C.c_uint32, dp, dp]
         L.mbs_diffract_batch.restype = C.c_int
+        L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
+        L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
         L.mbs_device_result.restype = C.c_int
         L.mbs_ctx_stream.argtypes = [vp]
@@ -162,6 +164,13 @@ class Context:
         mn = np.zeros_like(m) if noshadow else None
         self.accumulate(m, mn)
         return m, mn
+
+    def azimuth_average(self, noshadow=False):
+        """Rows [n_zen+1, 18] (theta deg, 2 pi dcos, M11..M44) of the last
+        result, averaged over azimuth on the device."""
+        out = np.zeros((self.grid.n_zen + 1, 18))
+        self._check(self._L.mbs_azimuth_average(self._h, 1 if noshadow else 0, _d(out)))
+        return out
 
     def per_orient(self):
         g = self.grid

@@ -39,7 +39,19 @@ SLOT_SRCS := $(REF)/src/handler/HandlerPO.cpp $(REF)/src/main.cpp
 SLOT_OBJS := $(patsubst $(REF)/%,$(OUT)/objslot/%.o,$(SLOT_SRCS))
 BASE_OBJS := $(filter-out $(patsubst $(REF)/%,$(OUT)/obj/%.o,$(SLOT_SRCS) $(REF)/cpu/GpuSupportStub.cpp),$(OBJS))
 
-all: $(OUT)/mbs_po $(OUT)/libmbsref.a $(OUT)/mbs_po_dump $(OUT)/ref_fixed_harness
+all: $(OUT)/mbs_po $(OUT)/libmbsref.a $(OUT)/mbs_po_dump $(OUT)/ref_fixed_harness \
+     $(OUT)/mbs_po_hip
+
+# The reference CLI with the MI355X backend: the GPU-slot objects plus the
+# product's drop-in translation unit mbs-fast_amd/host/HandlerPOHip.cpp,
+# linked against the engine library (found at run time through the rpath).
+ENGINE_SO := mbs-fast_amd/libmbs_hip.so
+$(OUT)/hip_shim.o: mbs-fast_amd/host/HandlerPOHip.cpp include/mbs_hip.h
+	$(CXX) $(CXXFLAGS) $(INC) -Iinclude -c $< -o $@
+
+$(OUT)/mbs_po_hip: $(BASE_OBJS) $(SLOT_OBJS) $(OUT)/hip_shim.o $(ENGINE_SO)
+	$(CXX) $(CXXFLAGS) $(BASE_OBJS) $(SLOT_OBJS) $(OUT)/hip_shim.o -o $@ \
+	    -L$(dir $(ENGINE_SO)) -lmbs_hip -Wl,-rpath,'$$ORIGIN/../../mbs-fast_amd' -lm -lgomp
 
 # Fixed-orientation fixture harness: the reference library plus the dump shim.
 $(OUT)/ref_fixed_harness: tests/golden/ref_fixed_harness.cpp $(filter-out $(OUT)/obj/src/main.cpp.o,$(BASE_OBJS)) $(OUT)/objslot/src/handler/HandlerPO.cpp.o $(OUT)/dump_shim.o

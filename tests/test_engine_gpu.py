@@ -99,6 +99,31 @@ def test_orientation_average_both_work_modes(pkg, oracle, ctx, mode):
         assert g < 1e-6
 
 
+def test_azimuth_average_on_device(pkg, oracle, ctx):
+    """The device azimuth average (poles included) equals the oracle's
+    restatement of HandlerPOTotal::WriteMatricesToFile on the same grid, and
+    its rows pass the reference gate against the reference CLI's .dat."""
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    m, mn = _run(ctx, grid, batch, consts, noshadow=True)
+    for mm, ns, key in ((m, False, "ref_total_dat"), (mn, True, "ref_total_dat_noshadow")):
+        rows = ctx.azimuth_average(noshadow=ns)
+        want = oracle.azimuth_average(grid, mm)
+        np.testing.assert_allclose(rows[:, 0], fx[key][:, 0], rtol=1e-15, atol=0)
+        np.testing.assert_allclose(rows[:, 1], want[:, 1], rtol=1e-13, atol=0)
+        scale = np.abs(want[:, 2:3])
+        err = float(np.max(np.abs(rows[:, 2:] - want[:, 2:]) / scale))
+        g = M.gate_l2(rows, fx[key])
+        print("%s: device vs oracle azimuth average max|d|/M11 %.2e; gate_l2 vs reference "
+              ".dat %.2e" % (key, err, g))
+        assert err < 1e-13
+        assert g < 1e-6
+        # the pole rows keep the pole form's zero pattern
+        for t in (0, grid.n_zen):
+            z = rows[t, 2:].reshape(4, 4)
+            assert z[0, 1] == z[0, 2] == z[1, 0] == z[2, 0] == z[1, 2] == z[2, 1] == 0.0
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_per_orientation_grids(pkg, oracle, ctx, mode):
     fx = load_fixture("sobol_average")
