@@ -191,6 +191,30 @@ int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid,
  * grid (requires MBS_NOSHADOW on the last mbs_diffract).  Overwrites rows. */
 int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows);
 
+/* Orientation sets of the orientation average, generated on the device
+ * (one row of MBS_ORIENT_STRIDE doubles per orientation: beta, gamma, weight,
+ * qx, qy, qz, qw, R00..R22 of the particle rotation):
+ *   MBS_ORIENT_SOBOL       Owen-scrambled 2-D Sobol (seed), beta = acos(1 -
+ *                          (1 - cos beta_sym) u), gamma = gamma_sym v, weight
+ *                          1/n: TracerPOTotal::TraceFromSobolSeed
+ *                          (src/scattering/TracerPOTotal.cpp:5465-5492) with
+ *                          Sobol2D (src/math/Sobol.cpp:1-106)
+ *   MBS_ORIENT_HAMMERSLEY  u = (i+1/2)/n, v = radical inverse base 2 of i:
+ *                          TraceFromSO3Quaternion (:5347-5426), unpaired
+ *   MBS_ORIENT_SHOEMAKE    full-SO(3) quaternions from (u, RI2(i), RI3(i)):
+ *                          TraceFromSO3FullQuaternion (:5428-5463); beta and
+ *                          gamma are NaN, beta_sym/gamma_sym are ignored
+ * R follows Particle::SetRotateMatrix(beta, gamma, 0) /
+ * SetRotateMatrixFromQuaternion (src/particle/Particle.cpp:1593-1642).
+ * out: [n][MBS_ORIENT_STRIDE]. */
+#define MBS_ORIENT_SOBOL 0
+#define MBS_ORIENT_HAMMERSLEY 1
+#define MBS_ORIENT_SHOEMAKE 2
+#define MBS_ORIENT_STRIDE 16
+#define MBS_MAX_ORIENT_SET (1 << 26)
+int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double beta_sym,
+                        double gamma_sym, double *out);
+
 /* Device pointer of the result grid of the last mbs_diffract (FP64,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
 int mbs_device_result(mbs_ctx *ctx, const double **m_full,

@@ -31,6 +31,9 @@ PREC_MIXED = 1
 NOSHADOW = 4
 PER_ORIENT = 8
 
+ORIENT_SOBOL, ORIENT_HAMMERSLEY, ORIENT_SHOEMAKE = 0, 1, 2
+ORIENT_STRIDE = 16
+
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)

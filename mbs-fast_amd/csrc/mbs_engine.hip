@@ -19,6 +19,7 @@
 
 #include "../../include/mbs_hip.h"
 #include "po_kernels.hip"
+#include "orient_kernels.hip"
 
 using namespace mbs;
 
@@ -81,7 +82,7 @@ struct mbs_ctx {
     DevBuf<float> hdr32;
     // work and results
     DevBuf<WorkItem> items;
-    DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows;
+    DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows, orient;
     std::vector<WorkItem> h_items;
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
@@ -231,6 +232,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->res_ns.release();
     ctx->per_orient.release();
     ctx->avg_rows.release();
+    ctx->orient.release();
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev)
@@ -723,6 +725,31 @@ int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows)
         ctx->az_step, ctx->avg_rows.p);
     HIP_TRY(ctx, hipGetLastError());
     return fetch(ctx, ctx->avg_rows.p, n, rows, false);
+}
+
+int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double beta_sym,
+                        double gamma_sym, double *out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_orientation_set: NULL argument");
+    if (kind != MBS_ORIENT_SOBOL && kind != MBS_ORIENT_HAMMERSLEY && kind != MBS_ORIENT_SHOEMAKE)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_orientation_set: unknown kind %d", kind);
+    if (n <= 0)
+        return fail(ctx, MBS_ERR_INVALID, "orientation count must be positive");
+    if (n > MBS_MAX_ORIENT_SET)
+        return fail(ctx, MBS_ERR_CAPACITY, "orientation count %d exceeds %d", n,
+                    MBS_MAX_ORIENT_SET);
+    if (kind != MBS_ORIENT_SHOEMAKE &&
+        !(beta_sym > 0.0 && beta_sym <= M_PI && gamma_sym > 0.0 && gamma_sym <= 2.0 * M_PI))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "symmetry domain must satisfy 0 < beta <= pi and 0 < gamma <= 2*pi");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t count = (size_t)n * MBS_ORIENT_STRIDE;
+    HIP_TRY(ctx, ctx->orient.ensure(count));
+    orientation_set_kernel<<<(n + 255) / 256, 256, 0, ctx->stream>>>(
+        kind, n, seed, std::cos(beta_sym), gamma_sym, ctx->orient.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return fetch(ctx, ctx->orient.p, count, out, false);
 }
 
 int mbs_device_result(mbs_ctx *ctx, const double **m_full, const double **m_noshadow)

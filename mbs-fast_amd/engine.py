@@ -61,6 +61,9 @@ def lib():
                                          C.POINTER(abi.mbs_beam_batch),
                                          C.POINTER(abi.mbs_consts), C.c_uint32, dp, dp]
         L.mbs_diffract_batch.restype = C.c_int
+        L.mbs_orientation_set.argtypes = [vp, C.c_int, C.c_int32, C.c_uint32, C.c_double,
+                                          C.c_double, dp]
+        L.mbs_orientation_set.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -170,6 +173,15 @@ class Context:
         result, averaged over azimuth on the device."""
         out = np.zeros((self.grid.n_zen + 1, 18))
         self._check(self._L.mbs_azimuth_average(self._h, 1 if noshadow else 0, _d(out)))
+        return out
+
+    def orientation_set(self, kind, n, seed=1, beta_sym=np.pi, gamma_sym=2 * np.pi):
+        """[n, 16] rows (beta, gamma, weight, qx, qy, qz, qw, R row-major),
+        generated on the device; kind is abi.ORIENT_SOBOL, ORIENT_HAMMERSLEY
+        or ORIENT_SHOEMAKE."""
+        out = np.zeros((int(n), abi.ORIENT_STRIDE))
+        self._check(self._L.mbs_orientation_set(self._h, int(kind), int(n), int(seed),
+                                                float(beta_sym), float(gamma_sym), _d(out)))
         return out
 
     def per_orient(self):

@@ -37,6 +37,11 @@ def lib():
         L.mbs_oracle_add_mueller.restype = None
         L.mbs_oracle_azimuth_average.argtypes = [vp, dp, dp]
         L.mbs_oracle_azimuth_average.restype = None
+        L.mbs_oracle_sobol.argtypes = [C.c_int, C.c_uint32, dp, dp]
+        L.mbs_oracle_sobol.restype = None
+        L.mbs_oracle_orientation_set.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_double,
+                                                 C.c_double, dp]
+        L.mbs_oracle_orientation_set.restype = None
         L.mbs_oracle_2pi_dcos.argtypes = [dp, C.c_int, C.c_int]
         L.mbs_oracle_2pi_dcos.restype = C.c_double
         L.mbs_oracle_small_phase_factor.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double,
@@ -130,3 +135,19 @@ def aperture(vx, vy, slope_yx, slope_xy, flags, A, B, consts, area):
                               fl.ctypes.data_as(C.POINTER(C.c_uint8)), len(vx),
                               A, B, C.byref(consts), area, C.byref(fr), C.byref(fi))
     return complex(fr.value, fi.value)
+
+
+def sobol(n, seed):
+    """First n Owen-scrambled 2-D Sobol points (x, y) of Sobol2D(seed)."""
+    x, y = np.zeros(n), np.zeros(n)
+    lib().mbs_oracle_sobol(int(n), int(seed), _d(x), _d(y))
+    return x, y
+
+
+def orientation_set(kind, n, seed=1, beta_sym=np.pi, gamma_sym=2 * np.pi):
+    """[n, 16] orientation rows (beta, gamma, weight, q (zero for Euler sets),
+    R row-major) as mbs_orientation_set defines them."""
+    out = np.zeros((int(n), 16))
+    lib().mbs_oracle_orientation_set(int(kind), int(n), int(seed), float(beta_sym),
+                                     float(gamma_sym), _d(out))
+    return out

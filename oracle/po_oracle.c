@@ -505,3 +505,147 @@ void mbs_oracle_aperture(const double *vx, const double *vy, const double *syx,
         sincos(c->wave_index * (A * vx[v] + B * vy[v]), &vs[v], &vc[v]);
     aperture_integral(vx, vy, syx, sxy, ef, nv, A, B, vc, vs, c, area, fr, fi);
 }
+
+/* ---- orientation sets ----------------------------------------------------
+ * Sobol2D restated in its sequential form (src/math/Sobol.cpp:1-106): the
+ * state XORs direction number c = index of the lowest zero bit of the point
+ * counter, then each coordinate is Owen-scrambled bit by bit with a hash of
+ * (seed, dimension, level, prefix of original bits). */
+static uint32_t sobol_hash(uint32_t seed, uint32_t dim, uint32_t level, uint32_t prefix)
+{
+    uint32_t h = seed ^ 0x9e3779b9u;
+    h ^= dim + 0x85ebca6bu + (h << 6) + (h >> 2);
+    h ^= level + 0xc2b2ae35u + (h << 6) + (h >> 2);
+    h ^= prefix + 0x27d4eb2fu + (h << 6) + (h >> 2);
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+static uint32_t sobol_owen(uint32_t seed, uint32_t value, uint32_t dim)
+{
+    uint32_t out = 0, prefix = 0;
+    for (uint32_t level = 0; level < 32; ++level) {
+        const uint32_t shift = 31u - level, bit = (value >> shift) & 1u;
+        out |= (bit ^ (sobol_hash(seed, dim, level, prefix) & 1u)) << shift;
+        prefix = (prefix << 1) | bit;
+    }
+    return out;
+}
+
+void mbs_oracle_sobol(int n, uint32_t seed, double *x, double *y)
+{
+    uint32_t dir[2][32];
+    for (int i = 0; i < 32; ++i)
+        dir[0][i] = 1u << (31 - i);
+    dir[1][0] = 1u << 31;
+    for (int i = 1; i < 32; ++i)
+        dir[1][i] = dir[1][i - 1] ^ (dir[1][i - 1] >> 1);
+    uint32_t s0 = 0, s1 = 0;
+    for (uint32_t k = 0; k < (uint32_t)n; ++k) {
+        uint32_t c = 0, v = k;
+        while (v & 1u) {
+            v >>= 1;
+            ++c;
+        }
+        if (c >= 32)
+            c = 0;
+        s0 ^= dir[0][c];
+        s1 ^= dir[1][c];
+        x[k] = (double)sobol_owen(seed, s0, 0) / 4294967296.0;
+        y[k] = (double)sobol_owen(seed, s1, 1) / 4294967296.0;
+    }
+}
+
+static double radical_inverse(uint32_t v, uint32_t base)
+{
+    const double ib = 1.0 / (double)base;
+    double inv = ib, r = 0.0;
+    while (v > 0) {
+        r += (double)(v % base) * inv;
+        v /= base;
+        inv *= ib;
+    }
+    return r;
+}
+
+/* Particle::SetRotateMatrix(beta, gamma, alpha) and
+ * SetRotateMatrixFromQuaternion (src/particle/Particle.cpp:1593-1642). */
+static void oracle_euler_matrix(double beta, double gamma, double alpha, double *R)
+{
+    double sA, cA, sB, cB, sG, cG;
+    sincos(alpha, &sA, &cA);
+    sincos(beta, &sB, &cB);
+    sincos(gamma, &sG, &cG);
+    R[0] = cA * cB * cG - sA * sG;
+    R[1] = -(cA * cB * sG + sA * cG);
+    R[2] = cA * sB;
+    R[3] = sA * cG * cB + cA * sG;
+    R[4] = cA * cG - sA * sG * cB;
+    R[5] = sA * sB;
+    R[6] = -sB * cG;
+    R[7] = sB * sG;
+    R[8] = cB;
+}
+
+static void oracle_quat_matrix(double x, double y, double z, double w, double *R)
+{
+    const double n = sqrt(x * x + y * y + z * z + w * w);
+    if (n <= DBL_EPSILON) {
+        x = y = z = 0.0;
+        w = 1.0;
+    } else {
+        x /= n; y /= n; z /= n; w /= n;
+    }
+    R[0] = 1.0 - 2.0 * (y * y + z * z);
+    R[1] = 2.0 * (x * y - w * z);
+    R[2] = 2.0 * (x * z + w * y);
+    R[3] = 2.0 * (x * y + w * z);
+    R[4] = 1.0 - 2.0 * (x * x + z * z);
+    R[5] = 2.0 * (y * z - w * x);
+    R[6] = 2.0 * (x * z - w * y);
+    R[7] = 2.0 * (y * z + w * x);
+    R[8] = 1.0 - 2.0 * (x * x + y * y);
+}
+
+/* Orientation sets, rows of MBS_ORIENT_STRIDE doubles (see mbs_orientation_set
+ * in include/mbs_hip.h): Sobol (TracerPOTotal.cpp:5465-5492), symmetry
+ * Hammersley (:5347-5426, unpaired) and full-SO(3) Shoemake quaternions
+ * (:5428-5463).  The quaternion of an Euler row is left zero here (the tests
+ * check the engine's through its rotation matrix). */
+void mbs_oracle_orientation_set(int kind, int n, uint32_t seed, double beta_sym,
+                                double gamma_sym, double *out)
+{
+    double *x = malloc(sizeof(double) * (size_t)n), *y = malloc(sizeof(double) * (size_t)n);
+    if (kind == MBS_ORIENT_SOBOL)
+        mbs_oracle_sobol(n, seed, x, y);
+    const double cbs = cos(beta_sym);
+    for (int i = 0; i < n; ++i) {
+        double *o = out + (size_t)i * MBS_ORIENT_STRIDE;
+        memset(o, 0, sizeof(double) * MBS_ORIENT_STRIDE);
+        o[2] = 1.0 / (double)n;
+        if (kind == MBS_ORIENT_SHOEMAKE) {
+            const double u1 = (i + 0.5) / n, u2 = radical_inverse((uint32_t)i, 2),
+                         u3 = radical_inverse((uint32_t)i, 3);
+            const double r1 = sqrt(fmax(0.0, 1.0 - u1)), r2 = sqrt(u1);
+            const double a = 2.0 * M_PI * u2, b = 2.0 * M_PI * u3;
+            o[0] = o[1] = NAN;
+            o[3] = r1 * sin(a);
+            o[4] = r1 * cos(a);
+            o[5] = r2 * sin(b);
+            o[6] = r2 * cos(b);
+            oracle_quat_matrix(o[3], o[4], o[5], o[6], o + 7);
+        } else {
+            const double u = kind == MBS_ORIENT_SOBOL ? x[i] : (i + 0.5) / (double)n;
+            const double v = kind == MBS_ORIENT_SOBOL ? y[i] : radical_inverse((uint32_t)i, 2);
+            o[0] = acos(1.0 - (1.0 - cbs) * u);
+            o[1] = gamma_sym * v;
+            oracle_euler_matrix(o[0], o[1], 0.0, o + 7);
+        }
+    }
+    free(x);
+    free(y);
+}

@@ -40,7 +40,11 @@ SLOT_OBJS := $(patsubst $(REF)/%,$(OUT)/objslot/%.o,$(SLOT_SRCS))
 BASE_OBJS := $(filter-out $(patsubst $(REF)/%,$(OUT)/obj/%.o,$(SLOT_SRCS) $(REF)/cpu/GpuSupportStub.cpp),$(OBJS))
 
 all: $(OUT)/mbs_po $(OUT)/libmbsref.a $(OUT)/mbs_po_dump $(OUT)/ref_fixed_harness \
-     $(OUT)/mbs_po_hip
+     $(OUT)/mbs_po_hip $(OUT)/ref_sobol_dump
+
+# The reference's Sobol2D points, for the orientation-set fixture.
+$(OUT)/ref_sobol_dump: tests/golden/ref_sobol_dump.cpp $(OUT)/obj/src/math/Sobol.cpp.o
+	$(CXX) $(CXXFLAGS) $(INC) $^ -o $@
 
 # The reference CLI with the MI355X backend: the GPU-slot objects plus the
 # product's drop-in translation unit mbs-fast_amd/host/HandlerPOHip.cpp,

@@ -242,31 +242,51 @@ def sobol_case(name, L, D, n_re, n_im, wave, refl, n_orient, seed, nphi, nth,
     return fx
 
 
+def sobol_points(n=4096, seeds=(1, 7, 42, 4096, 123456789)):
+    """The reference's own Sobol2D points (x then y per seed), dumped by
+    oracle/_ref/ref_sobol_dump (tests/golden/ref_sobol_dump.cpp)."""
+    wd = fresh_dir("sobol_points")
+    out = os.path.join(wd, "sobol.bin")
+    run([os.path.join(REF_BIN, "ref_sobol_dump"), out, str(n)] + [str(s) for s in seeds], wd)
+    raw = np.fromfile(out, dtype=np.float64).reshape(len(seeds), 2, n)
+    print("sobol_points: %d seeds x %d points" % (len(seeds), n))
+    return {"seeds": np.array(seeds, dtype=np.uint32), "x": raw[:, 0], "y": raw[:, 1]}
+
+
 def save(name, fx):
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **fx)
     print("wrote %s (%.1f KiB)" % (path, os.path.getsize(path) / 1024.0))
 
 
-def main():
+def main(only=()):
+    """Writes every fixture, or only the named ones (command-line arguments)."""
     os.makedirs(WORK, exist_ok=True)
+    want = lambda name: not only or name in only  # noqa: E731
+    if want("sobol_points"):
+        save("sobol_points", sobol_points())
+    if not (want("config1_fixed") or want("fixed_absorbing") or want("sobol_average")):
+        return
     # Configuration 1: hexagonal column L=20 D=10 um, m=1.3116, lambda=0.532,
     # 8 reflections, beta=30 gamma=10, theta 0..180 in 720 steps x 48 phi.
     # Inputs are kept whole; the expected grid every 8th theta row (+ last).
     rows = list(range(0, 721, 8)) + [720]
-    save("config1_fixed", fixed_case("config1_fixed", 20, 10, 1.3116, 0.0, 0.532, 8,
-                                     30, 10, 0, 180, 48, 720, keep_rows=rows))
+    if want("config1_fixed"):
+        save("config1_fixed", fixed_case("config1_fixed", 20, 10, 1.3116, 0.0, 0.532, 8,
+                                         30, 10, 0, 180, 48, 720, keep_rows=rows))
     # Absorbing fixed case (complex refractive index; exercises the absorption
     # factors folded into J_phased), small grid.
-    save("fixed_absorbing", fixed_case("fixed_absorbing", 12, 8, 1.3116, 0.01, 0.532, 6,
-                                       47, 23, 0, 180, 16, 90))
+    if want("fixed_absorbing"):
+        save("fixed_absorbing", fixed_case("fixed_absorbing", 12, 8, 1.3116, 0.01, 0.532, 6,
+                                           47, 23, 0, 180, 16, 90))
     # Orientation average: reference Owen-Sobol (seed 1) orientations of the
     # configuration-2 particle shape scaled down (L=20 D=10 um) with absorption,
     # the non-uniform theta grid of configuration 2 on 24 phi bins.
     tg = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
-    save("sobol_average", sobol_case("sobol_average", 20, 10, 1.3116, 1e-4, 0.532, 6,
-                                     8, 1, 24, 540, tgrid=tg))
+    if want("sobol_average"):
+        save("sobol_average", sobol_case("sobol_average", 20, 10, 1.3116, 1e-4, 0.532, 6,
+                                         8, 1, 24, 540, tgrid=tg))
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

@@ -244,3 +244,38 @@ def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
     m, _ = _run(ctx, grid, batch, consts)
     o, _, _ = oracle.diffract(grid, batch, consts)
     check_fp64(m, o, grid.theta, "synthetic D=50 FP64 vs oracle")
+
+
+def _quat_matrix(q):
+    x, y, z, w = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2], ids=["sobol", "hammersley", "shoemake"])
+def test_orientation_sets_on_device(pkg, oracle, ctx, kind):
+    """Device orientation sets equal the oracle's (Sobol bits exactly: gamma is
+    gamma_sym * v with identical v), and the reference's own Sobol2D points."""
+    for n, seed, bs, gs in ((4096, 1, np.pi / 2, np.pi / 3), (1000, 42, np.pi, 2 * np.pi)):
+        rows = ctx.orientation_set(kind, n, seed=seed, beta_sym=bs, gamma_sym=gs)
+        want = oracle.orientation_set(kind, n, seed=seed, beta_sym=bs, gamma_sym=gs)
+        np.testing.assert_array_equal(rows[:, 2], want[:, 2])
+        if kind == 2:
+            np.testing.assert_allclose(rows[:, 3:7], want[:, 3:7], rtol=0, atol=5e-16)
+        else:
+            np.testing.assert_array_equal(rows[:, 1], want[:, 1])
+            np.testing.assert_allclose(rows[:, 0], want[:, 0], rtol=4e-16, atol=0)
+            assert np.all(rows[:, 6] >= 0)
+        np.testing.assert_allclose(rows[:, 7:], want[:, 7:], rtol=0, atol=1e-15)
+        for r in rows[::97]:
+            np.testing.assert_allclose(_quat_matrix(r[3:7]), r[7:].reshape(3, 3), atol=2e-15)
+    if kind == 0:
+        fx = load_fixture("sobol_points")
+        rows = ctx.orientation_set(0, 4096, seed=int(fx["seeds"][0]), beta_sym=np.pi,
+                                   gamma_sym=2 * np.pi)
+        np.testing.assert_allclose(rows[:, 1] / (2 * np.pi), fx["y"][0], rtol=2e-16, atol=0)
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.orientation_set(kind, 0)
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.orientation_set(7, 10)

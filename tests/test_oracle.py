@@ -257,3 +257,66 @@ def test_azimuth_average_undoes_frame_rotation(pkg, oracle):
             m[p, t] = (m0[t] @ np.linalg.inv(L)).ravel()
     avg = oracle.azimuth_average(grid, m)
     np.testing.assert_allclose(avg[:, 2:].reshape(-1, 4, 4), m0, atol=1e-13)
+
+
+# ---- orientation sets --------------------------------------------------------
+
+def _rz(a):
+    c, s = np.cos(a), np.sin(a)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _ry(b):
+    c, s = np.cos(b), np.sin(b)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def test_sobol_matches_reference_points(oracle):
+    """The oracle's sequential restatement of Sobol2D reproduces the points the
+    reference's own Sobol2D (compiled from src/math/Sobol.cpp) wrote, bit for
+    bit, for five seeds x 4096 points."""
+    fx = load_fixture("sobol_points")
+    n = fx["x"].shape[1]
+    for k, seed in enumerate(fx["seeds"]):
+        x, y = oracle.sobol(n, int(seed))
+        np.testing.assert_array_equal(x, fx["x"][k])
+        np.testing.assert_array_equal(y, fx["y"][k])
+    # Owen scrambling keeps the (0, m, 2)-net property: every elementary
+    # interval of area 1/256 holds exactly one point of an aligned block of
+    # 256 sequence points.  Output k is sequence point k+1 (the first point
+    # is emitted after one Gray-code step), so outputs 255..510 are points
+    # 256..511.
+    x, y = fx["x"][0, 255:511], fx["y"][0, 255:511]
+    for a in range(9):
+        ix = np.floor(x * 2 ** a).astype(int)
+        iy = np.floor(y * 2 ** (8 - a)).astype(int)
+        assert len(set(zip(ix, iy))) == 256
+
+
+def test_orientation_sets_geometry(oracle):
+    n = 512
+    for kind in (0, 1):
+        rows = oracle.orientation_set(kind, n, seed=42, beta_sym=np.pi / 2, gamma_sym=np.pi / 3)
+        assert np.all(rows[:, 2] == 1.0 / n)
+        assert np.all((rows[:, 0] >= 0) & (rows[:, 0] <= np.pi / 2 + 1e-15))
+        assert np.all((rows[:, 1] >= 0) & (rows[:, 1] < np.pi / 3))
+        for r in rows[::37]:
+            R = r[7:16].reshape(3, 3)
+            np.testing.assert_allclose(R, _ry(r[0]) @ _rz(r[1]), atol=1e-15)
+        # uniform solid-angle measure: cos(beta) is uniform on [0, 1]
+        assert abs(np.mean(np.cos(rows[:, 0])) - 0.5) < 2e-3
+    ham = oracle.orientation_set(1, 8, beta_sym=np.pi, gamma_sym=2 * np.pi)
+    np.testing.assert_allclose(ham[:, 1] / (2 * np.pi),
+                               [0, .5, .25, .75, .125, .625, .375, .875], atol=1e-16)
+    np.testing.assert_allclose(np.cos(ham[:, 0]), 1 - 2 * (np.arange(8) + .5) / 8, atol=1e-15)
+    sh = oracle.orientation_set(2, 9)
+    q = sh[:, 3:7]
+    np.testing.assert_allclose(np.linalg.norm(q, axis=1), 1.0, atol=1e-15)
+    # u3 = radical inverse base 3: 0, 1/3, 2/3, 1/9, 4/9, 7/9, 2/9, 5/9, 8/9
+    u3 = np.arctan2(q[:, 2], q[:, 3]) % (2 * np.pi) / (2 * np.pi)
+    np.testing.assert_allclose(u3[1:], np.array([1, 2, 1 / 3, 4 / 3, 7 / 3, 2 / 3, 5 / 3, 8 / 3])
+                               / 3, atol=1e-14)
+    for r in sh:
+        R = r[7:16].reshape(3, 3)
+        np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1) < 1e-14
