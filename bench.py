@@ -222,7 +222,9 @@ def run(args):
     upload_ms = 1e3 * (time.perf_counter() - t_up)
     results = {}
     traffic = measured_traffic()
-    for name, flags in (("mixed", abi.PREC_MIXED), ("fp64", abi.PREC_FP64)):
+    legs = (("mixed", abi.PREC_MIXED), ("mixed_valu", abi.PREC_MIXED | abi.NO_MFMA),
+            ("fp64", abi.PREC_FP64))
+    for name, flags in legs:
         for _ in range(args.warmup):
             ctx.diffract(flags)
         hip.sync()
@@ -279,6 +281,8 @@ def run(args):
         "roofline": results["mixed"]["roofline"],
         "fp64": {k: results["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms",
                                                  "roofline")},
+        "mixed_valu": {k: results["mixed_valu"][k] for k in ("value", "ms_per_step", "kernel_ms",
+                                                             "roofline")},
     }
     if not args.no_cpu:
         port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)

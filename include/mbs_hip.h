@@ -117,6 +117,9 @@ typedef struct mbs_consts {
                                  reference FP32 build does                    */
 #define MBS_NOSHADOW 4u       /* also accumulate the no-shadow Mueller grid     */
 #define MBS_PER_ORIENT 8u     /* also keep one Mueller grid per orientation     */
+#define MBS_NO_MFMA 16u       /* mixed precision: rotation and coherent sum on
+                                 the vector ALU instead of the MFMA contraction
+                                 (same arithmetic, for comparison)              */
 
 typedef struct mbs_ctx mbs_ctx;
 
@@ -214,6 +217,34 @@ int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows);
 #define MBS_MAX_ORIENT_SET (1 << 26)
 int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double beta_sym,
                         double gamma_sym, double *out);
+
+/* Aperture FFT (near-forward path), after mbs_upload_beams: for each listed
+ * beam of the uploaded batch, the aperture polygon (vx, vy) is sampled on an
+ * n x n grid of pixel centres spanning pad times its bounding box, Fourier
+ * transformed in FP32 on the device, and the transform
+ *   I(A, B) = integral over the aperture of exp(i k (A x + B y)) dx dy
+ * (k = wave_index of the upload) is interpolated at each (A, B) point with a
+ * phase-corrected Lanczos kernel of the given radius.  The method of the
+ * reference's aperture probe (tools/fft_aperture_probe.cu:130-169, :549-551,
+ * :50-102), for convex polygons.  (Multiply by -i/lambda for the aperture
+ * factor of mbs_diffract's integral.)
+ *   beams   [n_beams] indices into the uploaded batch, 1 <= n_beams <= 64
+ *   n       grid side, a power of two in MBS_FFT_MIN_N..MBS_FFT_MAX_N
+ *   ab      [n_points][2] (A, B);  out [n_beams][n_points][2] (re, im)
+ * n_points = 0 computes the spectra only (kept on the device until the next
+ * call).  Device workspace: n_beams * n * n * 8 bytes. */
+#define MBS_FFT_MAX_BATCH 64
+#define MBS_FFT_MIN_N 64
+#define MBS_FFT_MAX_N 4096
+int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_t n,
+                     double pad, int32_t n_points, const double *ab, int32_t radius,
+                     double *out);
+/* Spectrum `index` of the last mbs_aperture_fft batch: the DFT of its raster,
+ * [n rows (y)][n columns (x)] of (re, im) FP32, unnormalised. */
+int mbs_aperture_fft_spectrum(mbs_ctx *ctx, int32_t index, float *out);
+/* With timing enabled: device times of the last mbs_aperture_fft's raster+row
+ * pass, column pass and sampling. */
+int mbs_aperture_fft_timing(mbs_ctx *ctx, float *rows_ms, float *cols_ms, float *sample_ms);
 
 /* Device pointer of the result grid of the last mbs_diffract (FP64,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */

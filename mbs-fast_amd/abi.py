@@ -30,6 +30,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 NOSHADOW = 4
 PER_ORIENT = 8
+NO_MFMA = 16
 
 ORIENT_SOBOL, ORIENT_HAMMERSLEY, ORIENT_SHOEMAKE = 0, 1, 2
 ORIENT_STRIDE = 16

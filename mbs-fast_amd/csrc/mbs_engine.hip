@@ -20,6 +20,7 @@
 #include "../../include/mbs_hip.h"
 #include "po_kernels.hip"
 #include "orient_kernels.hip"
+#include "fft_kernels.hip"
 
 using namespace mbs;
 
@@ -79,10 +80,17 @@ struct mbs_ctx {
     DevBuf<int64_t> voff, ooff;
     DevBuf<uint8_t> ef_in;
     DevBuf<double> hdr, vtx, vdd;
-    DevBuf<float> hdr32;
+    DevBuf<float> hdr32, wb;
     // work and results
     DevBuf<WorkItem> items;
     DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows, orient;
+    // aperture FFT
+    DevBuf<float2> fft_grid;
+    DevBuf<double> fft_rec, fft_ab, fft_out;
+    DevBuf<int64_t> fft_beams;
+    int fft_n = 0, fft_batch = 0;
+    float fft_ms[3] = {0, 0, 0};
+    hipEvent_t fft_ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<WorkItem> h_items;
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
@@ -224,6 +232,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->hdr.release();
     ctx->vtx.release();
     ctx->vdd.release();
+    ctx->wb.release();
     ctx->hdr32.release();
     ctx->items.release();
     ctx->part.release();
@@ -233,6 +242,14 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->per_orient.release();
     ctx->avg_rows.release();
     ctx->orient.release();
+    ctx->fft_grid.release();
+    ctx->fft_rec.release();
+    ctx->fft_ab.release();
+    ctx->fft_out.release();
+    ctx->fft_beams.release();
+    for (hipEvent_t &e : ctx->fft_ev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
     for (hipEvent_t &e : ctx->ev)
@@ -350,6 +367,7 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
     HIP_TRY(ctx, ctx->hdr32.ensure((size_t)nb * HDR32_STRIDE));
     HIP_TRY(ctx, ctx->vtx.ensure((size_t)nvert * VTX_STRIDE));
     HIP_TRY(ctx, ctx->vdd.ensure((size_t)nvert * VDD_STRIDE));
+    HIP_TRY(ctx, ctx->wb.ensure((size_t)std::max(nb, 1) * WB_STRIDE));
     HIP_TRY(ctx, ctx->vflag.ensure(nvert));
     hipStream_t s = ctx->stream;
     if (ctx->timing)
@@ -389,7 +407,7 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
             ctx->syx_in.p, ctx->sxy_in.p, ctx->ef_in.p, c->wave_index,
             sg * c->inv_compl_wave[0], sg * c->inv_compl_wave[1], c->compl_wave[0],
             c->compl_wave[1], ctx->hdr.p, ctx->hdr32.p,
-            ctx->vtx.p, ctx->vdd.p, ctx->vflag.p);
+            ctx->vtx.p, ctx->vdd.p, ctx->vflag.p, ctx->wb.p);
         HIP_TRY(ctx, hipGetLastError());
     }
     if (ctx->timing) {
@@ -468,9 +486,16 @@ static bool plan_items(mbs_ctx *ctx, int n_tiles, std::vector<WorkItem> &items)
 template <int PREC, int NS, int JONES, int PER>
 static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
-    diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
-        a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd, a.vflag, a.nv,
-        a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns, a.per_orient);
+    if constexpr (PREC == 2)
+        diffract_mfma_kernel<NS, JONES, PER><<<blocks, 256, 0, s>>>(
+            a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.wb, a.vtx, a.vdd, a.vflag,
+            a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
+            a.per_orient);
+    else
+        diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
+            a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd,
+            a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
+            a.per_orient);
 }
 
 static void launch_diffract(int prec, int ns, int jones, int per, const KernelArgs &a,
@@ -485,6 +510,8 @@ static void launch_diffract(int prec, int ns, int jones, int per, const KernelAr
     MBS_L(0, 0, 0, 1) MBS_L(0, 1, 0, 1)
     MBS_L(1, 0, 0, 0) MBS_L(1, 1, 0, 0) MBS_L(1, 0, 1, 0) MBS_L(1, 1, 1, 0)
     MBS_L(1, 0, 0, 1) MBS_L(1, 1, 0, 1)
+    MBS_L(2, 0, 0, 0) MBS_L(2, 1, 0, 0) MBS_L(2, 0, 1, 0) MBS_L(2, 1, 1, 0)
+    MBS_L(2, 0, 0, 1) MBS_L(2, 1, 0, 1)
 #undef MBS_L
 }
 
@@ -498,10 +525,11 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         return fail(ctx, MBS_ERR_NO_GRID, "mbs_set_grid has not been called");
     if (!ctx->beams_ok)
         return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
-    const int prec = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    // mixed precision runs the coherent sum on MFMA unless MBS_NO_MFMA
+    const int prec = (flags & MBS_PREC_MIXED) ? ((flags & MBS_NO_MFMA) ? 1 : 2) : 0;
     const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
     const int per = (flags & MBS_PER_ORIENT) ? 1 : 0;
-    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT))
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT | MBS_NO_MFMA))
         return fail(ctx, MBS_ERR_INVALID, "unknown flag bits 0x%x", flags);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int rc = ensure_tables(ctx);
@@ -542,6 +570,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     a.tab = ctx->tab.p;
     a.hdr = ctx->hdr.p;
     a.hdr32 = ctx->hdr32.p;
+    a.wb = ctx->wb.p;
     a.vtx = ctx->vtx.p;
     a.vdd = ctx->vdd.p;
     a.vflag = ctx->vflag.p;
@@ -641,6 +670,40 @@ int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out)
 }
 
 }  // extern "C"
+
+static int fft_lds_limit(mbs_ctx *ctx, int *bytes)
+{
+    HIP_TRY(ctx, hipDeviceGetAttribute(bytes, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                       ctx->device));
+    return MBS_OK;
+}
+
+template <int R>
+static int launch_fft_rows(mbs_ctx *ctx, int nb, int n, int logn)
+{
+    const size_t lds = (size_t)(R * (n + FFT_PAD) + n / 2) * sizeof(float2);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<R>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int units = nb * (n / R);
+    const int blocks = (units + 7) / 8 * 8;
+    fft_rows_kernel<R><<<blocks, FFT_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_rec.p,
+                                                                ctx->fft_grid.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
+template <int C>
+static int launch_fft_cols(mbs_ctx *ctx, int nb, int n, int logn)
+{
+    const size_t lds = (size_t)(C * (n + FFT_PAD) + n / 2) * sizeof(float2);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<C>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int units = nb * (n / C);
+    const int blocks = (units + 7) / 8 * 8;
+    fft_cols_kernel<C><<<blocks, FFT_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_grid.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
 
 static int fetch(mbs_ctx *ctx, const double *src, size_t n, double *dst_add, bool add)
 {
@@ -750,6 +813,127 @@ int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double
         kind, n, seed, std::cos(beta_sym), gamma_sym, ctx->orient.p);
     HIP_TRY(ctx, hipGetLastError());
     return fetch(ctx, ctx->orient.p, count, out, false);
+}
+
+int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_t n,
+                     double pad, int32_t n_points, const double *ab, int32_t radius,
+                     double *out)
+{
+    if (!ctx || !beams || (n_points > 0 && (!ab || !out)))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_aperture_fft: NULL argument");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    if (n_beams < 1 || n_beams > MBS_FFT_MAX_BATCH)
+        return fail(ctx, MBS_ERR_CAPACITY, "FFT batch of %d beams (1..%d)", n_beams,
+                    MBS_FFT_MAX_BATCH);
+    if (n < MBS_FFT_MIN_N || n > MBS_FFT_MAX_N || (n & (n - 1)))
+        return fail(ctx, MBS_ERR_INVALID, "FFT size %d: a power of two in %d..%d", n,
+                    MBS_FFT_MIN_N, MBS_FFT_MAX_N);
+    if (!(pad >= 1.0) || radius < 1 || radius > 8 || n_points < 0)
+        return fail(ctx, MBS_ERR_INVALID, "pad >= 1, 1 <= radius <= 8, n_points >= 0");
+    for (int i = 0; i < n_beams; ++i)
+        if (beams[i] < 0 || beams[i] >= ctx->n_beams)
+            return fail(ctx, MBS_ERR_INVALID, "beam index %lld outside the uploaded batch",
+                        (long long)beams[i]);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int limit = 0;
+    int rc = fft_lds_limit(ctx, &limit);
+    if (rc)
+        return rc;
+    auto lds_for = [n](int r) { return (size_t)(r * (n + FFT_PAD) + n / 2) * sizeof(float2); };
+    int logn = 0;
+    while ((1 << logn) < n)
+        ++logn;
+    HIP_TRY(ctx, ctx->fft_grid.ensure((size_t)n_beams * n * n));
+    HIP_TRY(ctx, ctx->fft_rec.ensure((size_t)n_beams * FB_STRIDE));
+    HIP_TRY(ctx, ctx->fft_beams.ensure(n_beams));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->fft_beams.p, beams, sizeof(int64_t) * n_beams,
+                                hipMemcpyHostToDevice, s));
+    fft_setup_kernel<<<(n_beams + 63) / 64, 64, 0, s>>>(n_beams, ctx->fft_beams.p, ctx->nv.p,
+                                                       ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
+                                                       n, pad, ctx->fft_rec.p);
+    HIP_TRY(ctx, hipGetLastError());
+    if (ctx->timing)
+        for (hipEvent_t &e : ctx->fft_ev)
+            if (!e)
+                HIP_TRY(ctx, hipEventCreate(&e));
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[0], s));
+    if ((size_t)limit >= lds_for(4))
+        rc = launch_fft_rows<4>(ctx, n_beams, n, logn);
+    else if ((size_t)limit >= lds_for(2))
+        rc = launch_fft_rows<2>(ctx, n_beams, n, logn);
+    else
+        rc = launch_fft_rows<1>(ctx, n_beams, n, logn);
+    if (rc)
+        return rc;
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
+    if ((size_t)limit >= lds_for(8))
+        rc = launch_fft_cols<8>(ctx, n_beams, n, logn);
+    else if ((size_t)limit >= lds_for(4))
+        rc = launch_fft_cols<4>(ctx, n_beams, n, logn);
+    else
+        rc = launch_fft_cols<2>(ctx, n_beams, n, logn);
+    if (rc)
+        return rc;
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[2], s));
+    ctx->fft_n = n;
+    ctx->fft_batch = n_beams;
+    if (n_points > 0) {
+        const size_t np = (size_t)n_beams * n_points;
+        HIP_TRY(ctx, ctx->fft_ab.ensure((size_t)n_points * 2));
+        HIP_TRY(ctx, ctx->fft_out.ensure(np * 2));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->fft_ab.p, ab, sizeof(double) * 2 * n_points,
+                                    hipMemcpyHostToDevice, s));
+        fft_sample_kernel<<<(unsigned)((np + 255) / 256), 256, 0, s>>>(
+            n_beams, n_points, n, ctx->fft_rec.p, ctx->fft_grid.p, ctx->fft_ab.p,
+            ctx->consts.wave_index, radius, ctx->fft_out.p);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    if (ctx->timing) {
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[3], s));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->fft_ev[3]));
+        HIP_TRY(ctx, hipEventElapsedTime(&ctx->fft_ms[0], ctx->fft_ev[0], ctx->fft_ev[1]));
+        HIP_TRY(ctx, hipEventElapsedTime(&ctx->fft_ms[1], ctx->fft_ev[1], ctx->fft_ev[2]));
+        HIP_TRY(ctx, hipEventElapsedTime(&ctx->fft_ms[2], ctx->fft_ev[2], ctx->fft_ev[3]));
+    }
+    if (n_points > 0)
+        return fetch(ctx, ctx->fft_out.p, (size_t)n_beams * n_points * 2, out, false);
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return MBS_OK;
+}
+
+int mbs_aperture_fft_spectrum(mbs_ctx *ctx, int32_t index, float *out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_aperture_fft_spectrum: NULL argument");
+    if (index < 0 || index >= ctx->fft_batch)
+        return fail(ctx, MBS_ERR_INVALID, "no spectrum %d (last batch holds %d)", index,
+                    ctx->fft_batch);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n2 = (size_t)ctx->fft_n * ctx->fft_n;
+    HIP_TRY(ctx, hipMemcpyAsync(out, ctx->fft_grid.p + (size_t)index * n2, n2 * sizeof(float2),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBS_OK;
+}
+
+int mbs_aperture_fft_timing(mbs_ctx *ctx, float *rows_ms, float *cols_ms, float *sample_ms)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->timing)
+        return fail(ctx, MBS_ERR_INVALID, "timing is not enabled (mbs_set_timing)");
+    if (rows_ms)
+        *rows_ms = ctx->fft_ms[0];
+    if (cols_ms)
+        *cols_ms = ctx->fft_ms[1];
+    if (sample_ms)
+        *sample_ms = ctx->fft_ms[2];
+    return MBS_OK;
 }
 
 int mbs_device_result(mbs_ctx *ctx, const double **m_full, const double **m_noshadow)

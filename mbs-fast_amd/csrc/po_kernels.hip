@@ -37,6 +37,9 @@
 #else
 #define MBS_OCCUPANCY
 #endif
+// The MFMA kernel is held to 4 waves per SIMD (128 VGPRs + AGPRs) when it
+// keeps one accumulator set; the no-shadow variant (two sets) gets 2.
+#define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : 4, 8)))
 
 namespace mbs {
 
@@ -115,7 +118,8 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
                                   float *__restrict__ hdr32,
                                   double *__restrict__ vtx,
                                   double *__restrict__ vdd,
-                                  int32_t *__restrict__ vflag)
+                                  int32_t *__restrict__ vflag,
+                                  float *__restrict__ wb)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n_beams)
@@ -194,6 +198,22 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
         h[H_JP + 2 * q] = cw_re * jr - cw_im * ji;
         h[H_JP + 2 * q + 1] = cw_re * ji + cw_im * jr;
     }
+    // MFMA operand: W[m][c] = T0[m] Jp[0][c] + T1[m] Jp[1][c] (FP64, then FP32)
+    float *w = wb + (size_t)b * WB_STRIDE;
+    for (int n = 0; n < 32; ++n) {
+        double re = 0.0, im = 0.0;
+        if (n < WB_COLS) {
+            const int m = n >> 2, c = (n >> 1) & 1;
+            const double t0 = h[H_T0 + m], t1 = h[H_T1 + m];
+            const double wr = t0 * h[H_JP + 2 * c] + t1 * h[H_JP + 4 + 2 * c];
+            const double wi = t0 * h[H_JP + 2 * c + 1] + t1 * h[H_JP + 4 + 2 * c + 1];
+            const bool imag_col = n & 1;
+            re = imag_col ? wi : wr;   // row 0 multiplies Re F
+            im = imag_col ? wr : -wi;  // row 1 multiplies Im F
+        }
+        w[n] = (float)re;
+        w[32 + n] = (float)im;
+    }
     // FP32 copy of what the mixed-precision kernel uses in FP32
     for (int q = 0; q < 6; ++q) {
         h32[H32_T0 + q] = (float)h[H_T0 + q];
@@ -233,6 +253,18 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
 // branches do not diverge.  complWave is folded into J_phased and the
 // small-phase area limit at packing time.
 // ---------------------------------------------------------------------------
+// A wave-uniform FP64 value kept in scalar registers.  Both edge slopes are
+// read as scalars and the per-lane choice is made between registers; without
+// this the compiler may select the address per lane instead, turning two
+// scalar loads into one vector load on every edge.
+__device__ __forceinline__ double uniform_f64(double x)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 template <int ACC_NS>
 __device__ __forceinline__ void beam_dir_f64(
     const double *__restrict__ h, const double *__restrict__ vtx,
@@ -257,7 +289,7 @@ __device__ __forceinline__ void beam_dir_f64(
     double sp = s0, cp = c0, sr = 0.0, si = 0.0;
     for (int e = 0; e < nv; ++e) {
         const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope = useB ? v[V_SYX] : v[V_SXY];
+        const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
         const int f = vflag[e];
         double sn = s0, cn = c0;
         if (e + 1 < nv) {
@@ -340,16 +372,15 @@ __device__ __forceinline__ void beam_dir_f64(
     }
 }
 
-// Per-(beam, direction) evaluation, mixed precision: A, B, the phases and
-// their reduction and the edge coefficients in FP64; phasors, edge sums,
-// rotation and Jones in FP32.
-template <int ACC_NS>
-__device__ __forceinline__ void beam_dir_mixed(
-    const double *__restrict__ h, const float *__restrict__ h32,
-    const double *__restrict__ vtx, const double *__restrict__ vdd,
-    const int32_t *__restrict__ vflag, int nv, const double *d,
-    const float *__restrict__ su, const float2 *__restrict__ tab, double k, double eps1,
-    float *j, float *jn)
+// Aperture integral F(beam, direction) / complWave, mixed precision: A, B,
+// the phases and their reduction, the edge coefficients and the moments in
+// FP64; phasors and edge sums in FP32.  A NaN result (degenerate beam) is
+// returned as 0, i.e. the beam is skipped.
+__device__ __forceinline__ void aperture_mixed(
+    const double *__restrict__ h, const double *__restrict__ vtx,
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
+    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
+    float &out_i)
 {
     const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
     const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
@@ -365,7 +396,7 @@ __device__ __forceinline__ void beam_dir_mixed(
     float sp = s0, cp = c0, sr = 0.0f, si = 0.0f;
     for (int e = 0; e < nv; ++e) {
         const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope = useB ? v[V_SYX] : v[V_SXY];
+        const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
         const int f = vflag[e];
         float sn = s0, cn = c0;
         if (e + 1 < nv) {
@@ -404,7 +435,23 @@ __device__ __forceinline__ void beam_dir_mixed(
         fr = (float)ar * cd - (float)ai * sd;
         fi = (float)ar * sd + (float)ai * cd;
     }
-    if (__builtin_isnan(fr))
+    const bool bad = __builtin_isnan(fr);
+    out_r = bad ? 0.0f : fr;
+    out_i = bad ? 0.0f : fi;
+}
+
+// Mixed precision with the rotation and coherent sum on the vector ALU.
+template <int ACC_NS>
+__device__ __forceinline__ void beam_dir_mixed(
+    const double *__restrict__ h, const float *__restrict__ h32,
+    const double *__restrict__ vtx, const double *__restrict__ vdd,
+    const int32_t *__restrict__ vflag, int nv, const double *d,
+    const float *__restrict__ su, const float2 *__restrict__ tab, double k, double eps1,
+    float *j, float *jn)
+{
+    float fr, fi;
+    aperture_mixed(h, vtx, vdd, vflag, nv, d, tab, k, eps1, fr, fi);
+    if (fr == 0.0f && fi == 0.0f)
         return;
     const int l = threadIdx.x;
     float u0[6], u1[6];
@@ -468,6 +515,7 @@ struct KernelArgs {
     const double *tab;       // grid tables [DT_COUNT][cells]
     const double *hdr;       // [n_beams][HDR_STRIDE]
     const float *hdr32;      // [n_beams][HDR32_STRIDE]
+    const float *wb;         // [n_beams][WB_STRIDE]
     const double *vtx;       // [n_vert][VTX_STRIDE]
     const double *vdd;       // [n_vert][VDD_STRIDE]
     const int32_t *vflag;    // [n_vert]
@@ -579,6 +627,141 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
                                       j, nullptr);
             }
         }
+        if (!live)
+            continue;
+        if constexpr (JONES) {
+            double *dst = out + ((size_t)item * cells + cell) * 8;
+            for (int q = 0; q < 8; ++q)
+                dst[q] = j[q];
+            if constexpr (NS) {
+                double *dn = out_ns + ((size_t)item * cells + cell) * 8;
+                for (int q = 0; q < 8; ++q)
+                    dn[q] = jn[q];
+            }
+        } else {
+            const double w = weights[o];
+            const bool first = (o == it.o_begin);
+            double *per = PER ? per_orient + ((size_t)o * cells + cell) * 16 : nullptr;
+            flush_mueller(j, w, out + ((size_t)item * cells + cell) * 16, first, per);
+            if constexpr (NS)
+                flush_mueller(jn, w, out_ns + ((size_t)item * cells + cell) * 16, first,
+                              (double *)nullptr);
+        }
+    }
+}
+
+// ---- coherent sum as a matrix contraction on MFMA (mixed precision) -------
+//
+// The rotation is linear in the beam's polarisation data: with the
+// direction-only vectors u0, u1 (6 components each) and the per-beam complex
+// W[m][c] = T0[m] Jp[0][c] + T1[m] Jp[1][c],
+//   J[i][c](dir) = sum_b F_b(dir) (R_b Jp_b)[i][c]
+//                = sum_m u_i[m](dir) G[m][c](dir),  G = sum_b F_b(dir) W_b[m][c].
+// G is a complex [64 directions x beams] . [beams x 12] product per wave,
+// done as a real one with K = 2 per beam (Re F, Im F) and N = 24 columns on
+// v_mfma_f32_32x32x2f32 (two 32-direction blocks per wave).  The vector ALU
+// keeps only the aperture integral; the rotation (4 six-term dot products),
+// R.Jp and the accumulation (56 operations per beam and direction) move to
+// the matrix cores, which run beside the vector ALU.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+enum { G_LD = 25 };  // LDS row stride (floats) of the per-wave G transpose
+
+// G (this wave's accumulators) -> this lane's Jones sum.  Accumulator v of
+// lane l holds G[dir][n] with n = l % 32, dir = 8 (v/4) + 4 (l/32) + v % 4
+// (+32 for the second block).  Called by every thread of the block.
+__device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1,
+                                              float *__restrict__ sg,
+                                              const float *__restrict__ su, float *j)
+{
+    const int lane = threadIdx.x & 63;
+    const int n = lane & 31;
+    if (n < WB_COLS) {
+        for (int v = 0; v < 16; ++v) {
+            const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            sg[i * G_LD + n] = g0[v];
+            sg[(32 + i) * G_LD + n] = g1[v];
+        }
+    }
+    __syncthreads();
+    const float *g = sg + lane * G_LD;
+    for (int t = 0; t < 4; ++t) {  // t = 2 i + c
+        const int i = t >> 1, c = t & 1;
+        float re = 0.0f, im = 0.0f;
+        for (int m = 0; m < 6; ++m) {
+            const float um = su[(6 * i + m) * 256 + threadIdx.x];
+            re = __builtin_fmaf(um, g[4 * m + 2 * c], re);
+            im = __builtin_fmaf(um, g[4 * m + 2 * c + 1], im);
+        }
+        j[2 * t] = re;
+        j[2 * t + 1] = im;
+    }
+    __syncthreads();
+}
+
+template <int NS, int JONES, int PER>
+__global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
+    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
+    const double *__restrict__ tab, const double *__restrict__ hdr,
+    const float *__restrict__ wb, const double *__restrict__ vtx,
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag,
+    const int32_t *__restrict__ nv, const int64_t *__restrict__ voff,
+    const int32_t *__restrict__ ext, const int64_t *__restrict__ ooff,
+    const double *__restrict__ weights, double k, double eps1, double *__restrict__ out,
+    double *__restrict__ out_ns, double *__restrict__ per_orient)
+{
+    int item, tile;
+    if (!map_block(n_items, n_tiles, item, tile))
+        return;
+    const int cell = tile * 256 + threadIdx.x;
+    const bool live = cell < cells;
+    const int c = live ? cell : cells - 1;  // dead lanes compute, never store
+    double d[3];
+    for (int q = 0; q < 3; ++q)
+        d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
+    __shared__ float2 s_tab[kTabN];
+    __shared__ float s_g[4 * 64 * G_LD];
+    __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
+    for (int q = 0; q < 12; ++q)
+        s_u[q * 256 + threadIdx.x] = (float)tab[(size_t)(DT_U0 + q) * cells + c];
+    {
+        double ts, tc;
+        sincospi((double)threadIdx.x / 128.0, &ts, &tc);
+        s_tab[threadIdx.x] = make_float2((float)tc, (float)ts);
+    }
+    __syncthreads();
+    float *sg = s_g + (threadIdx.x >> 6) * 64 * G_LD;
+    const int lane = threadIdx.x & 63;
+    const WorkItem it = items[item];
+    for (int o = it.o_begin; o < it.o_end; ++o) {
+        int64_t b0 = ooff[o], b1 = ooff[o + 1];
+        if constexpr (JONES) {
+            b0 = it.b_begin;
+            b1 = it.b_end;
+        }
+        f32x16 g0 = {}, g1 = {}, n0 = {}, n1 = {};
+        for (int64_t b = b0; b < b1; ++b) {
+            const float bop = wb[(size_t)b * WB_STRIDE + lane];
+            const int64_t v0 = voff[b];
+            float fr, fi;
+            aperture_mixed(hdr + (size_t)b * HDR_STRIDE, vtx + (size_t)v0 * VTX_STRIDE,
+                           vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nv[b], d, s_tab, k,
+                           eps1, fr, fi);
+            // A operands: lanes 0-31 (Re F), lanes 32-63 (Im F) of the
+            // wave's directions 0-31, then the same for directions 32-63
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(fr),
+                                                             __float_as_uint(fi), false, false);
+            const float a0 = __uint_as_float(sw[0]), a1 = __uint_as_float(sw[1]);
+            g0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bop, g0, 0, 0, 0);
+            g1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bop, g1, 0, 0, 0);
+            if (NS && ext[b] == 0) {
+                n0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, bop, n0, 0, 0, 0);
+                n1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, bop, n1, 0, 0, 0);
+            }
+        }
+        float j[8], jn[8];
+        mfma_epilogue(g0, g1, sg, s_u, j);
+        if constexpr (NS)
+            mfma_epilogue(n0, n1, sg, s_u, jn);
         if (!live)
             continue;
         if constexpr (JONES) {

@@ -33,6 +33,15 @@ enum {
     HDR32_STRIDE = 28
 };
 
+// MFMA B operand of one beam (mixed kernel), WB_STRIDE floats in lane order:
+// lane l holds B[l/32][l%32] of the complex contraction
+//   G[dir][n] += Re F(dir) * B[0][n] + Im F(dir) * B[1][n],
+// with column n = 4m + 2c + r for rotation component m (0..5), Jones column c
+// and r = 0 (real part) / 1 (imaginary part) of W[m][c] = T0[m] Jp[0][c] +
+// T1[m] Jp[1][c]: B[0][n] = (Re W, Im W), B[1][n] = (-Im W, Re W); columns
+// 24..31 are zero.
+enum { WB_COLS = 24, WB_STRIDE = 64 };
+
 // Vertex records, FP64, 32 bytes: k x, k y, slope dy/dx, slope dx/dy (the
 // slopes of edge e = vertex e -> vertex e+1).  The edge deltas dx, dy, read
 // only by the rare near-singular branch, live in a separate array.

@@ -64,6 +64,14 @@ def lib():
         L.mbs_orientation_set.argtypes = [vp, C.c_int, C.c_int32, C.c_uint32, C.c_double,
                                           C.c_double, dp]
         L.mbs_orientation_set.restype = C.c_int
+        L.mbs_aperture_fft.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.c_int32,
+                                       C.c_double, C.c_int32, dp, C.c_int32, dp]
+        L.mbs_aperture_fft.restype = C.c_int
+        L.mbs_aperture_fft_spectrum.argtypes = [vp, C.c_int32, C.POINTER(C.c_float)]
+        L.mbs_aperture_fft_spectrum.restype = C.c_int
+        L.mbs_aperture_fft_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                              C.POINTER(C.c_float)]
+        L.mbs_aperture_fft_timing.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -183,6 +191,28 @@ class Context:
         self._check(self._L.mbs_orientation_set(self._h, int(kind), int(n), int(seed),
                                                 float(beta_sym), float(gamma_sym), _d(out)))
         return out
+
+    def aperture_fft(self, beams, n, pad=4.0, ab=None, radius=4):
+        """I(A, B) of each listed beam at the (A, B) points, [n_beams, n_points]
+        complex, from the device aperture FFT (see mbs_aperture_fft)."""
+        beams = np.ascontiguousarray(beams, dtype=np.int64)
+        pts = np.zeros((0, 2)) if ab is None else np.ascontiguousarray(ab, dtype=np.float64)
+        out = np.zeros((len(beams), len(pts), 2))
+        self._check(self._L.mbs_aperture_fft(
+            self._h, beams.ctypes.data_as(C.POINTER(C.c_int64)), len(beams), int(n),
+            float(pad), len(pts), _d(pts), int(radius), _d(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def aperture_fft_spectrum(self, index, n):
+        out = np.zeros((n, n, 2), dtype=np.float32)
+        self._check(self._L.mbs_aperture_fft_spectrum(
+            self._h, int(index), out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out[..., 0] + 1j * out[..., 1].astype(np.complex64)
+
+    def aperture_fft_timing(self):
+        r, c, s = C.c_float(), C.c_float(), C.c_float()
+        self._check(self._L.mbs_aperture_fft_timing(self._h, C.byref(r), C.byref(c), C.byref(s)))
+        return {"rows_ms": r.value, "cols_ms": c.value, "sample_ms": s.value}
 
     def per_orient(self):
         g = self.grid

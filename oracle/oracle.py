@@ -42,6 +42,8 @@ def lib():
         L.mbs_oracle_orientation_set.argtypes = [C.c_int, C.c_int, C.c_uint32, C.c_double,
                                                  C.c_double, dp]
         L.mbs_oracle_orientation_set.restype = None
+        L.mbs_oracle_aperture_raster.argtypes = [dp, dp, C.c_int, C.c_int, C.c_double, dp, dp]
+        L.mbs_oracle_aperture_raster.restype = None
         L.mbs_oracle_2pi_dcos.argtypes = [dp, C.c_int, C.c_int]
         L.mbs_oracle_2pi_dcos.restype = C.c_double
         L.mbs_oracle_small_phase_factor.argtypes = [dp, dp, C.c_int, C.c_double, C.c_double,
@@ -151,3 +153,45 @@ def orientation_set(kind, n, seed=1, beta_sym=np.pi, gamma_sym=2 * np.pi):
     lib().mbs_oracle_orientation_set(int(kind), int(n), int(seed), float(beta_sym),
                                      float(gamma_sym), _d(out))
     return out
+
+
+def aperture_raster(vx, vy, n, pad):
+    """(raster [n, n] of 0/1 as float64, (xc, yc, dx, dy)) of a convex polygon,
+    as the device aperture FFT rasterises it."""
+    vx, vy = [np.ascontiguousarray(a, dtype=np.float64) for a in (vx, vy)]
+    grid = np.zeros((n, n))
+    geom = np.zeros(4)
+    lib().mbs_oracle_aperture_raster(_d(vx), _d(vy), len(vx), int(n), float(pad), _d(grid),
+                                     _d(geom))
+    return grid, geom
+
+
+def _lanczos(x, radius):
+    x = np.asarray(x, dtype=np.float64)
+    out = np.sinc(x) * np.sinc(x / radius)
+    return np.where(np.abs(x) >= radius, 0.0, out)
+
+
+def aperture_fft(vx, vy, n, pad, ab, k, radius=4):
+    """I(A, B) = integral of exp(ik(Ax + By)) over the polygon, from the FFT
+    of its raster (numpy FFT, FP64) sampled by phase-corrected Lanczos
+    interpolation -- the restatement of the device path
+    (tools/fft_aperture_probe.cu:50-102 for the sampling).  Returns
+    (values [n_points] complex, spectrum [n, n] complex)."""
+    grid, (xc, yc, dx, dy) = aperture_raster(vx, vy, n, pad)
+    X = np.fft.fft2(grid)  # rows = y, columns = x; exp(-2 pi i j m / n)
+    ab = np.asarray(ab, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros(len(ab), dtype=complex)
+    corr = np.pi * (n - 1) / n
+    for p, (A, B) in enumerate(ab):
+        fx, fy = -k * A / (2 * np.pi), -k * B / (2 * np.pi)
+        ux, uy = fx * n * dx, fy * n * dy
+        cx, cy = int(np.floor(ux)), int(np.floor(uy))
+        jx = np.arange(cx - radius + 1, cx + radius + 1)
+        jy = np.arange(cy - radius + 1, cy + radius + 1)
+        wx, wy = _lanczos(ux - jx, radius), _lanczos(uy - jy, radius)
+        w = np.outer(wy, wx)
+        z = X[np.ix_(jy % n, jx % n)] * np.exp(1j * corr * (jy[:, None] + jx[None, :]))
+        val = np.sum(w * z) / np.sum(w)
+        out[p] = dx * dy * val * np.exp(-2j * np.pi * (fx * xc + fy * yc))
+    return out, X

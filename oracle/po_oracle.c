@@ -649,3 +649,50 @@ void mbs_oracle_orientation_set(int kind, int n, uint32_t seed, double beta_sym,
     free(x);
     free(y);
 }
+
+/* ---- aperture FFT raster -------------------------------------------------
+ * The 0/1 raster of a convex aperture polygon on an n x n grid of pixel
+ * centres spanning pad times its bounding box, as the device aperture FFT
+ * builds it (the raster of tools/fft_aperture_probe.cu:130-169 for polygons):
+ * pixel centre r = c + ((2j + 1 - n)/2) d per axis; inside iff a x + b y + c
+ * > 0 on every edge line (oriented so that the inside is positive), every
+ * product formed with the same explicit FMAs as the device.  grid: [n][n]
+ * row-major (y, x); geom: xc, yc, dx, dy. */
+void mbs_oracle_aperture_raster(const double *vx, const double *vy, int nv, int n, double pad,
+                                double *grid, double *geom)
+{
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300, area2 = 0.0;
+    for (int e = 0; e < nv; ++e) {
+        const int f = e + 1 < nv ? e + 1 : 0;
+        x0 = fmin(x0, vx[e]);
+        x1 = fmax(x1, vx[e]);
+        y0 = fmin(y0, vy[e]);
+        y1 = fmax(y1, vy[e]);
+        area2 += vx[e] * vy[f] - vx[f] * vy[e];
+    }
+    const double xc = 0.5 * (x0 + x1), yc = 0.5 * (y0 + y1);
+    const double dx = pad * (x1 - x0) / n, dy = pad * (y1 - y0) / n;
+    const double sg = area2 >= 0.0 ? 1.0 : -1.0;
+    double L[3 * ORACLE_MAXV];
+    for (int e = 0; e < nv; ++e) {
+        const int f = e + 1 < nv ? e + 1 : 0;
+        const double a = -(vy[f] - vy[e]) * sg, b = (vx[f] - vx[e]) * sg;
+        L[3 * e] = a;
+        L[3 * e + 1] = b;
+        L[3 * e + 2] = -fma(a, vx[e], b * vy[e]);
+    }
+    for (int y = 0; y < n; ++y) {
+        const double py = fma((double)(2 * y + 1 - n), 0.5 * dy, yc);
+        for (int x = 0; x < n; ++x) {
+            const double px = fma((double)(2 * x + 1 - n), 0.5 * dx, xc);
+            int in = 1;
+            for (int e = 0; e < nv; ++e)
+                in = in && (fma(L[3 * e], px, fma(L[3 * e + 1], py, L[3 * e + 2])) > 0.0);
+            grid[(size_t)y * n + x] = in ? 1.0 : 0.0;
+        }
+    }
+    geom[0] = xc;
+    geom[1] = yc;
+    geom[2] = dx;
+    geom[3] = dy;
+}

@@ -158,6 +158,35 @@ def test_mixed_precision_p11(pkg, oracle, ctx):
         assert g11 < 1e-4
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_mfma_contraction_equals_vector_sum(pkg, oracle, ctx, mode):
+    """The mixed kernel's MFMA contraction (default) and its vector-ALU form
+    (MBS_NO_MFMA) compute the same FP32 sums in a different order: they agree
+    to FP32 rounding, full and no-shadow, per orientation, in both work modes."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_work_mode(mode)
+    out = {}
+    for name, extra in (("mfma", 0), ("valu", abi.NO_MFMA)):
+        m, mn = ctx.mueller(abi.PREC_MIXED | abi.PER_ORIENT | extra, noshadow=True)
+        out[name] = (m, mn, ctx.per_orient())
+    ctx.set_work_mode(0)
+    o, on, _ = oracle.diffract(grid, batch, consts, noshadow=True)
+    for k, (a, b) in enumerate(zip(out["mfma"], out["valu"])):
+        e = M.grid_l2(a, b, grid.theta) if k < 2 else max(
+            M.grid_l2(a[i], b[i], grid.theta) for i in range(len(a)))
+        print("mode %d part %d: MFMA vs VALU grid_l2 %.2e" % (mode, k, e))
+        assert e < 2e-5
+    for name in ("mfma", "valu"):
+        e = M.grid_l2(out[name][0][..., :1], o[..., :1], grid.theta)
+        en = M.grid_l2(out[name][1][..., :1], on[..., :1], grid.theta)
+        print("%s P11 vs oracle: full %.2e no-shadow %.2e" % (name, e, en))
+        assert e < 1e-4 and en < 1e-4
+
+
 def test_accumulates_and_is_deterministic(pkg, ctx):
     fx = load_fixture("sobol_average")
     grid, batch, consts = _setup(pkg, fx)
@@ -279,3 +308,45 @@ def test_orientation_sets_on_device(pkg, oracle, ctx, kind):
         ctx.orientation_set(kind, 0)
     with pytest.raises(pkg.engine.EngineError):
         ctx.orientation_set(7, 10)
+
+
+@pytest.mark.parametrize("n", [256, 2048])
+def test_aperture_fft_on_device(pkg, oracle, ctx, n):
+    """Device aperture FFT (FP32) of real beam apertures against the oracle's
+    FP64 FFT of the identical raster: spectra to FP32 accuracy, sampled
+    values to 1e-5 of the aperture area; and, through the oracle, against the
+    exact edge-sum transform.  n = 2048 runs a full 64-beam batch (2 GiB)."""
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    k = consts.wave_index
+    rng = np.random.default_rng(n)
+    nb = 64 if n == 2048 else 8
+    beams = rng.choice(batch.n_beams, nb, replace=False)
+    ab = rng.uniform(-0.02, 0.02, (32, 2))
+    ctx.set_timing(True)
+    got = ctx.aperture_fft(beams, n, pad=4.0, ab=ab)
+    t = ctx.aperture_fft_timing()
+    ctx.set_timing(False)
+    print("n=%d batch=%d: rows %.3f ms, cols %.3f ms, sample %.3f ms"
+          % (n, nb, t["rows_ms"], t["cols_ms"], t["sample_ms"]))
+    vo = batch.vert_offsets
+    for i in (0, nb - 1):
+        b = beams[i]
+        vx, vy = batch.vx[vo[b]:vo[b + 1]], batch.vy[vo[b]:vo[b + 1]]
+        want, X = oracle.aperture_fft(vx, vy, n, 4.0, ab, k)
+        S = ctx.aperture_fft_spectrum(i, n)
+        rel = np.linalg.norm(S - X) / np.linalg.norm(X)
+        area = abs(X[0, 0]) * (oracle.aperture_raster(vx, vy, n, 4.0)[1][2]
+                               * oracle.aperture_raster(vx, vy, n, 4.0)[1][3])
+        err = np.max(np.abs(got[i] - want)) / area
+        print("beam %d: spectrum rel L2 %.2e, sampled max |d|/area %.2e" % (b, rel, err))
+        assert rel < 1e-5
+        assert err < 1e-5
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.aperture_fft(beams[:1], 1000)
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.aperture_fft(np.zeros(65, np.int64), 256)
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.aperture_fft([batch.n_beams], 256)
