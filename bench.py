@@ -198,6 +198,35 @@ class Hip:
             raise RuntimeError("hipDeviceSynchronize: %d" % rc)
 
 
+FFT_N, FFT_BATCH = 2048, 64
+
+
+def bench_aperture_fft(ctx, batch, args):
+    """Aperture FFT of FFT_BATCH beams on FFT_N^2 FP32 grids: raster + row
+    pass (writes n^2 x 8 B per beam) and column pass (reads and writes n^2 x
+    8 B per beam); HBM-bound, reported against 8 TB/s."""
+    beams = np.arange(FFT_BATCH) % batch.n_beams
+    ctx.set_timing(True)
+    for _ in range(max(1, args.warmup)):
+        ctx.aperture_fft(beams, FFT_N)
+    rows, cols = [], []
+    for _ in range(args.steps):
+        ctx.aperture_fft(beams, FFT_N)
+        t = ctx.aperture_fft_timing()
+        rows.append(t["rows_ms"])
+        cols.append(t["cols_ms"])
+    grid_bytes = float(FFT_N) * FFT_N * 8 * FFT_BATCH
+    r, c = float(np.mean(rows)), float(np.mean(cols))
+    return {"n": FFT_N, "batch": FFT_BATCH, "dtype": "c64",
+            "grids_per_s": FFT_BATCH / ((r + c) * 1e-3),
+            "rows_ms": r, "cols_ms": c,
+            "rows_GBps": grid_bytes / (r * 1e-3) / 1e9,
+            "cols_GBps": 2 * grid_bytes / (c * 1e-3) / 1e9,
+            "roofline": {"bound": "hbm", "achieved": 3 * grid_bytes / ((r + c) * 1e-3) / 1e9,
+                         "peak": 8000.0, "unit": "GB/s",
+                         "frac": 3 * grid_bytes / ((r + c) * 1e-3) / 8e12, "traffic": None}}
+
+
 def run(args):
     import mbs_loader
     pkg = mbs_loader.load()
@@ -247,6 +276,7 @@ def run(args):
                          "traffic": traffic.get(name)},
             "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
         }
+    fft = bench_aperture_fft(ctx, batch, args)
     ctx.close()
     hist = np.bincount(batch.beam_nv, minlength=32)
     out = {
@@ -283,6 +313,7 @@ def run(args):
                                                  "roofline")},
         "mixed_valu": {k: results["mixed_valu"][k] for k in ("value", "ms_per_step", "kernel_ms",
                                                              "roofline")},
+        "aperture_fft": fft,
     }
     if not args.no_cpu:
         port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)

@@ -279,6 +279,30 @@ int mbs_set_target_blocks(mbs_ctx *ctx, int blocks);
 #define MBS_WORK_SEGMENTS 2
 int mbs_set_work_mode(mbs_ctx *ctx, int mode);
 
+/* Autotuner of the launch shape (no counterpart in the reference, whose CUDA
+ * path sizes its launch from fixed heuristics, GpuDiffraction.cu:2948-3029):
+ * times mbs_diffract(flags) on the uploaded batch for every candidate
+ * (target workgroups 1024..32768 x whole-orientation / beam-segment work
+ * items x, for mixed precision, MFMA / vector-ALU coherent sum), keeps the
+ * fastest for later mbs_diffract calls of the same precision, and reports
+ * what it timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.
+ * flags: MBS_PREC_MIXED and/or MBS_NOSHADOW only.  Leaves a result like
+ * mbs_diffract(flags). */
+typedef struct mbs_tune_entry {
+    int32_t target_blocks;
+    int32_t work_mode;  /* 1 whole orientations, 2 beam segments */
+    int32_t mfma;       /* 1 MFMA contraction, 0 vector ALU (mixed only) */
+    float ms;           /* main kernel + reduction, device time */
+} mbs_tune_entry;
+int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t max_entries,
+                 int32_t *n_entries);
+
+/* Graph replay: with enable != 0, mbs_diffract captures its kernel sequence
+ * (diffraction + reduction, no copies or memsets) into a HIP graph and
+ * replays it while launch shape, buffers and sizes are unchanged (the work
+ * list is re-uploaded before each replay); re-captured otherwise. */
+int mbs_set_graph(mbs_ctx *ctx, int enable);
+
 /* The HIP stream the context launches on (a hipStream_t, as void*), for
  * timing with events recorded on the same stream. */
 void *mbs_ctx_stream(mbs_ctx *ctx);

@@ -74,6 +74,11 @@ class mbs_timing(C.Structure):
                 ("reduce_ms", C.c_float), ("n_items", C.c_int32), ("n_blocks", C.c_int32)]
 
 
+class mbs_tune_entry(C.Structure):
+    _fields_ = [("target_blocks", C.c_int32), ("work_mode", C.c_int32), ("mfma", C.c_int32),
+                ("ms", C.c_float)]
+
+
 def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 

@@ -97,6 +97,15 @@ struct mbs_ctx {
     size_t pinned_cap = 0;
     int target_blocks = 4096;
     int work_mode = 0;  // 0 auto, 1 whole orientations, 2 beam segments
+    // launch shape chosen by mbs_autotune, per precision (0 FP64, 1 mixed)
+    struct Tuned {
+        bool valid = false;
+        int target_blocks = 0, work_mode = 0, mfma = 1;
+    } tuned[2];
+    // graph replay of the diffraction kernels (mbs_set_graph)
+    bool use_graph = false;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<unsigned char> graph_key;
     // stage timing
     bool timing = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -242,6 +251,8 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->per_orient.release();
     ctx->avg_rows.release();
     ctx->orient.release();
+    if (ctx->graph_exec)
+        (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->fft_grid.release();
     ctx->fft_rec.release();
     ctx->fft_ab.release();
@@ -438,12 +449,12 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
 //    the partial Jones grids of segment mode would exceed the memory budget.
 static const size_t kPartialBudget = (size_t)8 << 30;  // bytes
 
-static bool plan_items(mbs_ctx *ctx, int n_tiles, std::vector<WorkItem> &items)
+static bool plan_items(mbs_ctx *ctx, int n_tiles, int target, int work_mode,
+                       std::vector<WorkItem> &items)
 {
     items.clear();
     const int no = ctx->n_orient;
     const int64_t nb = ctx->n_beams;
-    const int target = ctx->target_blocks;
     const size_t cells = (size_t)ctx->cells;
     const int64_t want_items = std::max<int64_t>(1, (target + n_tiles - 1) / n_tiles);
     // segment size in beams: enough items to fill the device, but at least
@@ -456,7 +467,7 @@ static bool plan_items(mbs_ctx *ctx, int n_tiles, std::vector<WorkItem> &items)
         n_seg += std::max<int64_t>(1, (n + seg - 1) / seg);
     }
     const bool segments_fit = (size_t)n_seg * cells * 8 * sizeof(double) <= kPartialBudget;
-    const bool jones = ctx->work_mode == 2 || (ctx->work_mode == 0 && segments_fit && nb > 0);
+    const bool jones = work_mode == 2 || (work_mode == 0 && segments_fit && nb > 0);
     if (jones) {
         for (int o = 0; o < no; ++o) {
             const int64_t b0 = ctx->h_ooff[o], b1 = ctx->h_ooff[o + 1];
@@ -515,6 +526,46 @@ static void launch_diffract(int prec, int ns, int jones, int per, const KernelAr
 #undef MBS_L
 }
 
+// Launches the diffraction kernel and its reduction on the context's stream:
+// kernels only (no copies, no memsets), so the sequence can be captured in a
+// graph.  With `events`, ev[2..3] bracket the main kernel.
+static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, int per, bool jones, int per_k,
+                             const KernelArgs &a, int blocks, int n_items, bool events)
+{
+    hipStream_t s = ctx->stream;
+    const int cells = ctx->cells;
+    const size_t mcount = (size_t)cells * 16;
+    if (events)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+    launch_diffract(prec, ns, jones ? 1 : 0, per_k, a, blocks, s);
+    HIP_TRY(ctx, hipGetLastError());
+    if (events)
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    const int rt = 256;
+    if (jones) {
+        jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
+            ctx->part.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res.p,
+            per ? ctx->per_orient.p : nullptr);
+        HIP_TRY(ctx, hipGetLastError());
+        if (ns) {
+            jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
+                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res_ns.p,
+                nullptr);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    } else {
+        const int rb = (int)((mcount + rt - 1) / rt);
+        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->res.p);
+        HIP_TRY(ctx, hipGetLastError());
+        if (ns) {
+            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells,
+                                                    ctx->res_ns.p);
+            HIP_TRY(ctx, hipGetLastError());
+        }
+    }
+    return MBS_OK;
+}
+
 extern "C" {
 
 int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
@@ -525,8 +576,12 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         return fail(ctx, MBS_ERR_NO_GRID, "mbs_set_grid has not been called");
     if (!ctx->beams_ok)
         return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
-    // mixed precision runs the coherent sum on MFMA unless MBS_NO_MFMA
-    const int prec = (flags & MBS_PREC_MIXED) ? ((flags & MBS_NO_MFMA) ? 1 : 2) : 0;
+    // mixed precision runs the coherent sum on MFMA unless MBS_NO_MFMA, or
+    // unless the autotuner found the vector-ALU form faster on this batch
+    const int base = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    const mbs_ctx::Tuned &tn = ctx->tuned[base];
+    const bool mfma = !(flags & MBS_NO_MFMA) && !(tn.valid && !tn.mfma);
+    const int prec = base ? (mfma ? 2 : 1) : 0;
     const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
     const int per = (flags & MBS_PER_ORIENT) ? 1 : 0;
     if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT | MBS_NO_MFMA))
@@ -537,7 +592,8 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         return rc;
     const int cells = ctx->cells;
     const int n_tiles = (cells + 255) / 256;
-    const bool jones = plan_items(ctx, n_tiles, ctx->h_items);
+    const bool jones = plan_items(ctx, n_tiles, tn.valid ? tn.target_blocks : ctx->target_blocks,
+                                  tn.valid ? tn.work_mode : ctx->work_mode, ctx->h_items);
     const int n_items = (int)ctx->h_items.size();
     const size_t mcount = (size_t)cells * 16;
     HIP_TRY(ctx, ctx->res.ensure(mcount));
@@ -547,9 +603,11 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         HIP_TRY(ctx, ctx->per_orient.ensure((size_t)ctx->n_orient * mcount));
     hipStream_t s = ctx->stream;
     if (n_items == 0) {
-        HIP_TRY(ctx, hipMemsetAsync(ctx->res.p, 0, sizeof(double) * mcount, s));
+        const int zb = (int)((mcount + 255) / 256);
+        zero_kernel<<<zb, 256, 0, s>>>(ctx->res.p, mcount);
         if (ns)
-            HIP_TRY(ctx, hipMemsetAsync(ctx->res_ns.p, 0, sizeof(double) * mcount, s));
+            zero_kernel<<<zb, 256, 0, s>>>(ctx->res_ns.p, mcount);
+        HIP_TRY(ctx, hipGetLastError());
         ctx->have_result = true;
         ctx->have_ns = ns;
         ctx->have_per = per;
@@ -563,6 +621,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     HIP_TRY(ctx, hipMemcpyAsync(ctx->items.p, ctx->h_items.data(), sizeof(WorkItem) * n_items,
                                 hipMemcpyHostToDevice, s));
     KernelArgs a;
+    std::memset(&a, 0, sizeof a);  // padding too: the struct is a graph-cache key
     a.items = ctx->items.p;
     a.n_items = n_items;
     a.n_tiles = n_tiles;
@@ -593,35 +652,50 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     if (total > (int64_t)1 << 30)
         return fail(ctx, MBS_ERR_CAPACITY, "launch of %lld workgroups", (long long)total);
     const int blocks = (int)((total + 7) / 8 * 8);
-    if (ctx->timing)
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-    launch_diffract(prec, ns, jones ? 1 : 0, (per && !jones) ? 1 : 0, a, blocks, s);
-    HIP_TRY(ctx, hipGetLastError());
-    if (ctx->timing)
-        HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     ctx->last_items = n_items;
     ctx->last_blocks = blocks;
-    const int rt = 256;
-    if (jones) {
-        jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-            ctx->part.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res.p,
-            per ? ctx->per_orient.p : nullptr);
-        HIP_TRY(ctx, hipGetLastError());
-        if (ns) {
-            jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res_ns.p,
-                nullptr);
-            HIP_TRY(ctx, hipGetLastError());
+    const int per_k = (per && !jones) ? 1 : 0;
+    if (ctx->use_graph) {
+        // Replay the captured kernel sequence when nothing it depends on
+        // (launch shape, buffers, sizes, constants) has changed.
+        const int tail[8] = {prec, ns, per, jones ? 1 : 0, blocks, n_items, cells, per_k};
+        const void *outs[3] = {ctx->res.p, ctx->res_ns.p, ctx->per_orient.p};
+        std::vector<unsigned char> key(sizeof(KernelArgs) + sizeof tail + sizeof outs);
+        std::memcpy(key.data(), &a, sizeof(KernelArgs));
+        std::memcpy(key.data() + sizeof(KernelArgs), tail, sizeof tail);
+        std::memcpy(key.data() + sizeof(KernelArgs) + sizeof tail, outs, sizeof outs);
+        if (!ctx->graph_exec || key != ctx->graph_key) {
+            if (ctx->graph_exec) {
+                (void)hipGraphExecDestroy(ctx->graph_exec);
+                ctx->graph_exec = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const int rc2 = issue_diffraction(ctx, prec, ns, per, jones, per_k, a, blocks,
+                                              n_items, false);
+            hipError_t ce = hipStreamEndCapture(s, &g);
+            if (rc2)
+                return rc2;
+            if (ce != hipSuccess)
+                return fail(ctx, MBS_ERR_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ce != hipSuccess) {
+                ctx->graph_exec = nullptr;
+                return fail(ctx, MBS_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ce));
+            }
+            ctx->graph_key.swap(key);
         }
+        if (ctx->timing)
+            HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+        HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, s));
+        if (ctx->timing)
+            HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     } else {
-        const int rb = (int)((mcount + rt - 1) / rt);
-        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->res.p);
-        HIP_TRY(ctx, hipGetLastError());
-        if (ns) {
-            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells,
-                                                    ctx->res_ns.p);
-            HIP_TRY(ctx, hipGetLastError());
-        }
+        const int rc2 = issue_diffraction(ctx, prec, ns, per, jones, per_k, a, blocks, n_items,
+                                          ctx->timing);
+        if (rc2)
+            return rc2;
     }
     if (ctx->timing) {
         HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
@@ -968,6 +1042,7 @@ int mbs_set_work_mode(mbs_ctx *ctx, int mode)
     if (!ctx || mode < 0 || mode > 2)
         return fail(ctx, MBS_ERR_INVALID, "work mode must be 0, 1 or 2");
     ctx->work_mode = mode;
+    ctx->tuned[0].valid = ctx->tuned[1].valid = false;
     return MBS_OK;
 }
 
@@ -976,6 +1051,90 @@ int mbs_set_target_blocks(mbs_ctx *ctx, int blocks)
     if (!ctx || blocks < 1)
         return fail(ctx, MBS_ERR_INVALID, "bad target block count");
     ctx->target_blocks = blocks;
+    ctx->tuned[0].valid = ctx->tuned[1].valid = false;
+    return MBS_OK;
+}
+
+int mbs_set_graph(mbs_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    ctx->use_graph = enable != 0;
+    if (!ctx->use_graph && ctx->graph_exec) {
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = nullptr;
+        ctx->graph_key.clear();
+    }
+    return MBS_OK;
+}
+
+int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t max_entries,
+                 int32_t *n_entries)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_autotune takes MBS_PREC_MIXED and MBS_NOSHADOW");
+    if (!ctx->grid_ok || !ctx->beams_ok)
+        return fail(ctx, ctx->grid_ok ? MBS_ERR_NO_BEAMS : MBS_ERR_NO_GRID,
+                    "mbs_autotune needs a grid and an uploaded batch");
+    const int base = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    const bool had_timing = ctx->timing;
+    int rc = mbs_set_timing(ctx, 1);
+    if (rc)
+        return rc;
+    static const int kBlocks[] = {1024, 2048, 4096, 8192, 16384, 32768};
+    const int n_tiles = (ctx->cells + 255) / 256;
+    std::vector<mbs_tune_entry> tried;
+    mbs_ctx::Tuned best;
+    float best_ms = 0.0f;
+    for (int mfma = (base ? 1 : 0); mfma >= 0; --mfma)
+        for (int mode = 2; mode >= 1; --mode)
+            for (int tb : kBlocks) {
+                std::vector<WorkItem> items;
+                const bool jones = plan_items(ctx, n_tiles, tb, mode, items);
+                if (jones && items.size() * (size_t)ctx->cells * 8 * sizeof(double) > kPartialBudget)
+                    continue;
+                mbs_ctx::Tuned t;
+                t.valid = true;
+                t.target_blocks = tb;
+                t.work_mode = mode;
+                t.mfma = mfma;
+                ctx->tuned[base] = t;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && !rc; ++rep) {  // the first run allocates
+                    rc = mbs_diffract(ctx, flags);
+                    mbs_timing tm;
+                    if (!rc)
+                        rc = mbs_get_timing(ctx, &tm);
+                    ms = tm.kernel_ms + tm.reduce_ms;
+                }
+                if (rc)
+                    break;
+                mbs_tune_entry e;
+                std::memset(&e, 0, sizeof e);
+                e.target_blocks = tb;
+                e.work_mode = mode;
+                e.mfma = mfma;
+                e.ms = ms;
+                tried.push_back(e);
+                if (!best.valid || ms < best_ms) {
+                    best = t;
+                    best_ms = ms;
+                }
+            }
+    ctx->tuned[base] = best;
+    if (!had_timing)
+        mbs_set_timing(ctx, 0);
+    if (rc)
+        return rc;
+    if (n_entries)
+        *n_entries = (int32_t)tried.size();
+    if (entries)
+        for (size_t i = 0; i < tried.size() && (int32_t)i < max_entries; ++i)
+            entries[i] = tried[i];
     return MBS_OK;
 }
 

@@ -785,6 +785,15 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     }
 }
 
+// Zero fill (used instead of a memset so that no memset node is ever recorded
+// into a captured graph).
+__global__ void zero_kernel(double *__restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0.0;
+}
+
 // Sum of per-item Mueller partials in item order: res[c] = sum_i part[i][c].
 __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_items,
                                       int cells, double *__restrict__ res)

@@ -72,6 +72,11 @@ def lib():
         L.mbs_aperture_fft_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                               C.POINTER(C.c_float)]
         L.mbs_aperture_fft_timing.restype = C.c_int
+        L.mbs_autotune.argtypes = [vp, C.c_uint32, C.POINTER(abi.mbs_tune_entry), C.c_int32,
+                                   C.POINTER(C.c_int32)]
+        L.mbs_autotune.restype = C.c_int
+        L.mbs_set_graph.argtypes = [vp, C.c_int]
+        L.mbs_set_graph.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -240,6 +245,17 @@ class Context:
     def set_work_mode(self, mode):
         """0 automatic, 1 whole orientations per item, 2 beam segments."""
         self._check(self._L.mbs_set_work_mode(self._h, int(mode)))
+
+    def autotune(self, flags=0):
+        """Times candidate launch shapes, keeps the fastest; returns the list
+        of (target_blocks, work_mode, mfma, ms)."""
+        buf = (abi.mbs_tune_entry * 64)()
+        n = C.c_int32()
+        self._check(self._L.mbs_autotune(self._h, flags, buf, 64, C.byref(n)))
+        return [(e.target_blocks, e.work_mode, e.mfma, e.ms) for e in buf[:n.value]]
+
+    def set_graph(self, enable=True):
+        self._check(self._L.mbs_set_graph(self._h, 1 if enable else 0))
 
     def set_timing(self, enable=True):
         self._check(self._L.mbs_set_timing(self._h, 1 if enable else 0))

@@ -350,3 +350,35 @@ def test_aperture_fft_on_device(pkg, oracle, ctx, n):
         ctx.aperture_fft(np.zeros(65, np.int64), 256)
     with pytest.raises(pkg.engine.EngineError):
         ctx.aperture_fft([batch.n_beams], 256)
+
+
+def test_graph_replay_and_autotune(pkg, oracle, ctx):
+    """Graph replays equal the eager result on every replay, with eager work
+    and synchronisations between replays; the autotuner's choice gives the
+    same Mueller grid as the default launch shape (to FP32 rounding for the
+    mixed kernel, whose summation order changes with the shape)."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    for flags in (abi.PREC_FP64, abi.PREC_MIXED):
+        eager, eager_ns = ctx.mueller(flags, noshadow=True)
+        ctx.set_graph(True)
+        try:
+            for rep in range(4):
+                m, mn = ctx.mueller(flags, noshadow=True)
+                np.testing.assert_array_equal(m, eager)
+                np.testing.assert_array_equal(mn, eager_ns)
+                ctx.orientation_set(0, 1000)  # eager work and a sync between replays
+        finally:
+            ctx.set_graph(False)
+    for flags, tol in ((abi.PREC_FP64, 1e-12), (abi.PREC_MIXED, 2e-5)):
+        ref, _ = ctx.mueller(flags)
+        table = ctx.autotune(flags)
+        best = min(table, key=lambda e: e[3])
+        print("autotune flags=%d: %d shapes, best %s" % (flags, len(table), best))
+        assert len(table) >= 6
+        m, _ = ctx.mueller(flags)
+        assert M.grid_l2(m, ref, grid.theta) < tol
+    ctx.set_target_blocks(4096)  # back to the default shape for later tests
