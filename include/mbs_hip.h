@@ -246,6 +246,32 @@ int mbs_aperture_fft_spectrum(mbs_ctx *ctx, int32_t index, float *out);
  * pass, column pass and sampling. */
 int mbs_aperture_fft_timing(mbs_ctx *ctx, float *rows_ms, float *cols_ms, float *sample_ms);
 
+/* Learned control variate: a multilayer perceptron trained on the device
+ * (no counterpart in the reference).  Layers: n_in -> depth x width (GELU,
+ * exact erf form) -> n_out (linear).  GEMMs run on BF16 MFMA with FP32
+ * accumulation; weights are FP32 masters, rounded to BF16 as the GEMMs load
+ * them.  Parameters are packed per layer as W [in][out] row-major then b
+ * [out].  Created with He-uniform weights from `seed`, zero biases.  The
+ * model lives on ctx's device and stream. */
+typedef struct mbs_mlp mbs_mlp;
+int mbs_mlp_create(mbs_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out,
+                   uint64_t seed, mbs_mlp **out);
+int mbs_mlp_destroy(mbs_mlp *m);
+int64_t mbs_mlp_param_count(const mbs_mlp *m);
+int mbs_mlp_set_params(mbs_mlp *m, const float *params);
+int mbs_mlp_get_params(mbs_mlp *m, float *params);
+/* y [rows][n_out] = MLP(x [rows][n_in]). */
+int mbs_mlp_forward(mbs_mlp *m, const float *x, int32_t rows, float *y);
+/* Mean-squared-error loss over rows x n_out and its gradient w.r.t. every
+ * parameter (packed like the parameters). */
+int mbs_mlp_gradient(mbs_mlp *m, const float *x, const float *t, int32_t rows, double *loss,
+                     float *grad);
+/* Adam (beta 0.9, 0.999, eps 1e-8) over `epochs` passes of shuffled
+ * minibatches of `batch` rows (a trailing partial batch is dropped);
+ * final_loss = mean loss of the last epoch's batches. */
+int mbs_mlp_train(mbs_mlp *m, const float *x, const float *t, int32_t n, int32_t batch,
+                  int32_t epochs, float lr, uint64_t seed, double *final_loss);
+
 /* Device pointer of the result grid of the last mbs_diffract (FP64,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
 int mbs_device_result(mbs_ctx *ctx, const double **m_full,

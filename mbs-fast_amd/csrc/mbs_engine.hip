@@ -21,6 +21,7 @@
 #include "po_kernels.hip"
 #include "orient_kernels.hip"
 #include "fft_kernels.hip"
+#include "mlp_kernels.hip"
 
 using namespace mbs;
 
@@ -1135,6 +1136,350 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
     if (entries)
         for (size_t i = 0; i < tried.size() && (int32_t)i < max_entries; ++i)
             entries[i] = tried[i];
+    return MBS_OK;
+}
+
+}  // extern "C"
+
+// ---- learned control variate: the MLP ------------------------------------
+
+struct mbs_mlp {
+    mbs_ctx *ctx = nullptr;
+    int n_in = 0, width = 0, depth = 0, n_out = 0;
+    std::vector<int64_t> w_off, b_off;  // parameter offsets per layer
+    int64_t n_params = 0;
+    DevBuf<float> params, grads, adam_m, adam_v;
+    int64_t adam_t = 0;
+    // activations of the last forward pass: Z_l (pre-activation), A_l (input of layer l)
+    int cap_rows = 0;
+    std::vector<DevBuf<float>> z, a;
+    DevBuf<float> y, t, dy, da, dz, row_sq, x_all, t_all;
+    DevBuf<int32_t> idx;
+};
+
+namespace {
+
+int mlp_layer_in(const mbs_mlp *m, int l) { return l == 0 ? m->n_in : m->width; }
+int mlp_layer_out(const mbs_mlp *m, int l) { return l == m->depth ? m->n_out : m->width; }
+
+int mlp_ensure_rows(mbs_mlp *m, int rows)
+{
+    mbs_ctx *ctx = m->ctx;
+    if (rows <= m->cap_rows)
+        return MBS_OK;
+    m->z.resize(m->depth);
+    m->a.resize(m->depth + 1);
+    for (int l = 0; l <= m->depth; ++l)
+        HIP_TRY(ctx, m->a[l].ensure((size_t)rows * mlp_layer_in(m, l)));
+    for (int l = 0; l < m->depth; ++l)
+        HIP_TRY(ctx, m->z[l].ensure((size_t)rows * m->width));
+    HIP_TRY(ctx, m->y.ensure((size_t)rows * m->n_out));
+    HIP_TRY(ctx, m->t.ensure((size_t)rows * m->n_out));
+    HIP_TRY(ctx, m->dy.ensure((size_t)rows * m->n_out));
+    HIP_TRY(ctx, m->da.ensure((size_t)rows * m->width));
+    HIP_TRY(ctx, m->dz.ensure((size_t)rows * m->width));
+    HIP_TRY(ctx, m->row_sq.ensure(rows));
+    m->cap_rows = rows;
+    return MBS_OK;
+}
+
+template <int TA, int TB>
+void mlp_gemm(hipStream_t s, int M, int N, int K, const float *A, int lda, const float *B,
+              int ldb, float *C, int ldc, const float *bias, int beta)
+{
+    const dim3 grid((N + 63) / 64, (M + 63) / 64);
+    mlp_gemm_kernel<TA, TB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, bias, beta);
+}
+
+// Forward pass of `rows` inputs already in a[0]; output in y.
+int mlp_forward_dev(mbs_mlp *m, int rows)
+{
+    mbs_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    const float *P = m->params.p;
+    for (int l = 0; l <= m->depth; ++l) {
+        const int in = mlp_layer_in(m, l), out = mlp_layer_out(m, l);
+        float *dst = l < m->depth ? m->z[l].p : m->y.p;
+        mlp_gemm<0, 0>(s, rows, out, in, m->a[l].p, in, P + m->w_off[l], out, dst, out,
+                       P + m->b_off[l], 0);
+        if (l < m->depth) {
+            const size_t n = (size_t)rows * out;
+            mlp_gelu_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, m->z[l].p,
+                                                                        m->a[l + 1].p);
+        }
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
+// Loss (sum of squares) partials and parameter gradients of the batch whose
+// forward pass is in place and whose targets are in t.
+int mlp_backward_dev(mbs_mlp *m, int rows)
+{
+    mbs_ctx *ctx = m->ctx;
+    hipStream_t s = ctx->stream;
+    const float *P = m->params.p;
+    float *G = m->grads.p;
+    mlp_mse_grad_kernel<<<rows, 256, 0, s>>>(rows, m->n_out, m->y.p, m->t.p, m->dy.p,
+                                             m->row_sq.p);
+    const float *dcur = m->dy.p;  // gradient w.r.t. the output of layer l
+    for (int l = m->depth; l >= 0; --l) {
+        const int in = mlp_layer_in(m, l), out = mlp_layer_out(m, l);
+        const float *dout = dcur;
+        if (l < m->depth) {
+            const size_t n = (size_t)rows * out;
+            mlp_gelu_grad_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, m->z[l].p, dcur,
+                                                                             m->dz.p);
+            dout = m->dz.p;
+        }
+        // dW_l = A_l^T dOut, db_l = column sums of dOut
+        mlp_gemm<1, 0>(s, in, out, rows, m->a[l].p, in, dout, out, G + m->w_off[l], out,
+                       nullptr, 0);
+        mlp_colsum_kernel<<<(out + 255) / 256, 256, 0, s>>>(rows, out, dout, G + m->b_off[l]);
+        if (l > 0) {  // dA_l = dOut W_l^T
+            mlp_gemm<0, 1>(s, rows, in, out, dout, out, P + m->w_off[l], out, m->da.p, in,
+                           nullptr, 0);
+            dcur = m->da.p;
+        }
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
+uint64_t mlp_splitmix(uint64_t &x)
+{
+    uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbs_mlp_create(mbs_ctx *ctx, int32_t n_in, int32_t width, int32_t depth, int32_t n_out,
+                   uint64_t seed, mbs_mlp **out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_mlp_create: NULL argument");
+    *out = nullptr;
+    if (n_in < 1 || n_in > 64 || width < 8 || width > 4096 || depth < 1 || depth > 16 ||
+        n_out < 1 || n_out > 65536)
+        return fail(ctx, MBS_ERR_INVALID, "MLP shape out of range");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    mbs_mlp *m = new mbs_mlp();
+    m->ctx = ctx;
+    m->n_in = n_in;
+    m->width = width;
+    m->depth = depth;
+    m->n_out = n_out;
+    int64_t off = 0;
+    for (int l = 0; l <= depth; ++l) {
+        m->w_off.push_back(off);
+        off += (int64_t)mlp_layer_in(m, l) * mlp_layer_out(m, l);
+        m->b_off.push_back(off);
+        off += mlp_layer_out(m, l);
+    }
+    m->n_params = off;
+    // He-uniform weights (bound sqrt(6 / fan_in)), zero biases, from splitmix64
+    std::vector<float> h(off, 0.0f);
+    uint64_t st = seed;
+    for (int l = 0; l <= depth; ++l) {
+        const int in = mlp_layer_in(m, l), o = mlp_layer_out(m, l);
+        const double bound = std::sqrt(6.0 / in) * (l == depth ? 0.5 : 1.0);
+        for (int64_t i = 0; i < (int64_t)in * o; ++i) {
+            const double u = (double)(mlp_splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+            h[m->w_off[l] + i] = (float)((2.0 * u - 1.0) * bound);
+        }
+    }
+    hipError_t e = m->params.ensure(off);
+    if (e == hipSuccess)
+        e = m->grads.ensure(off);
+    if (e == hipSuccess)
+        e = m->adam_m.ensure(off);
+    if (e == hipSuccess)
+        e = m->adam_v.ensure(off);
+    if (e == hipSuccess)
+        e = hipMemcpy(m->params.p, h.data(), sizeof(float) * off, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemset(m->adam_m.p, 0, sizeof(float) * off);
+    if (e == hipSuccess)
+        e = hipMemset(m->adam_v.p, 0, sizeof(float) * off);
+    if (e != hipSuccess) {
+        delete m;
+        return fail(ctx, MBS_ERR_DEVICE, "MLP allocation: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return MBS_OK;
+}
+
+int mbs_mlp_destroy(mbs_mlp *m)
+{
+    if (!m)
+        return MBS_OK;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    for (auto &b : m->z)
+        b.release();
+    for (auto &b : m->a)
+        b.release();
+    DevBuf<float> *bufs[] = {&m->params, &m->grads, &m->adam_m, &m->adam_v, &m->y, &m->t,
+                             &m->dy, &m->da, &m->dz, &m->row_sq, &m->x_all, &m->t_all};
+    for (DevBuf<float> *b : bufs)
+        b->release();
+    m->idx.release();
+    delete m;
+    return MBS_OK;
+}
+
+int64_t mbs_mlp_param_count(const mbs_mlp *m) { return m ? m->n_params : -1; }
+
+int mbs_mlp_set_params(mbs_mlp *m, const float *params)
+{
+    if (!m || !params)
+        return fail(m ? m->ctx : nullptr, MBS_ERR_INVALID, "mbs_mlp_set_params: NULL argument");
+    mbs_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(m->params.p, params, sizeof(float) * m->n_params,
+                                hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBS_OK;
+}
+
+int mbs_mlp_get_params(mbs_mlp *m, float *params)
+{
+    if (!m || !params)
+        return fail(m ? m->ctx : nullptr, MBS_ERR_INVALID, "mbs_mlp_get_params: NULL argument");
+    mbs_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(params, m->params.p, sizeof(float) * m->n_params,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBS_OK;
+}
+
+int mbs_mlp_forward(mbs_mlp *m, const float *x, int32_t rows, float *y)
+{
+    if (!m || !x || !y || rows < 1)
+        return fail(m ? m->ctx : nullptr, MBS_ERR_INVALID, "mbs_mlp_forward: bad argument");
+    mbs_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = mlp_ensure_rows(m, rows);
+    if (rc)
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(m->a[0].p, x, sizeof(float) * rows * m->n_in,
+                                hipMemcpyHostToDevice, ctx->stream));
+    rc = mlp_forward_dev(m, rows);
+    if (rc)
+        return rc;
+    HIP_TRY(ctx, hipMemcpyAsync(y, m->y.p, sizeof(float) * rows * m->n_out,
+                                hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBS_OK;
+}
+
+int mbs_mlp_gradient(mbs_mlp *m, const float *x, const float *t, int32_t rows, double *loss,
+                     float *grad)
+{
+    if (!m || !x || !t || rows < 1)
+        return fail(m ? m->ctx : nullptr, MBS_ERR_INVALID, "mbs_mlp_gradient: bad argument");
+    mbs_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = mlp_ensure_rows(m, rows);
+    if (rc)
+        return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(m->a[0].p, x, sizeof(float) * rows * m->n_in,
+                                hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(m->t.p, t, sizeof(float) * rows * m->n_out,
+                                hipMemcpyHostToDevice, s));
+    rc = mlp_forward_dev(m, rows);
+    if (!rc)
+        rc = mlp_backward_dev(m, rows);
+    if (rc)
+        return rc;
+    std::vector<float> sq(rows);
+    HIP_TRY(ctx, hipMemcpyAsync(sq.data(), m->row_sq.p, sizeof(float) * rows,
+                                hipMemcpyDeviceToHost, s));
+    if (grad)
+        HIP_TRY(ctx, hipMemcpyAsync(grad, m->grads.p, sizeof(float) * m->n_params,
+                                    hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (loss) {
+        double acc = 0.0;
+        for (float v : sq)
+            acc += v;
+        *loss = acc / ((double)rows * m->n_out);
+    }
+    return MBS_OK;
+}
+
+int mbs_mlp_train(mbs_mlp *m, const float *x, const float *t, int32_t n, int32_t batch,
+                  int32_t epochs, float lr, uint64_t seed, double *final_loss)
+{
+    if (!m || !x || !t || n < 1 || batch < 1 || batch > n || epochs < 0 || !(lr > 0.0f))
+        return fail(m ? m->ctx : nullptr, MBS_ERR_INVALID, "mbs_mlp_train: bad argument");
+    mbs_ctx *ctx = m->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = mlp_ensure_rows(m, batch);
+    if (rc)
+        return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, m->x_all.ensure((size_t)n * m->n_in));
+    HIP_TRY(ctx, m->t_all.ensure((size_t)n * m->n_out));
+    HIP_TRY(ctx, m->idx.ensure(n));
+    HIP_TRY(ctx, hipMemcpyAsync(m->x_all.p, x, sizeof(float) * n * m->n_in,
+                                hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(m->t_all.p, t, sizeof(float) * n * m->n_out,
+                                hipMemcpyHostToDevice, s));
+    std::vector<int32_t> perm(n);
+    for (int i = 0; i < n; ++i)
+        perm[i] = i;
+    uint64_t st = seed;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    double last = 0.0;
+    std::vector<float> sq(batch);
+    for (int ep = 0; ep < epochs; ++ep) {
+        for (int i = n - 1; i > 0; --i)  // Fisher-Yates, deterministic in seed
+            std::swap(perm[i], perm[(int)(mlp_splitmix(st) % (uint64_t)(i + 1))]);
+        HIP_TRY(ctx, hipMemcpyAsync(m->idx.p, perm.data(), sizeof(int32_t) * n,
+                                    hipMemcpyHostToDevice, s));
+        double ep_sum = 0.0;
+        int ep_rows = 0;
+        for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+            const size_t nx = (size_t)batch * m->n_in, nt = (size_t)batch * m->n_out;
+            mlp_gather_kernel<<<(unsigned)((nx + 255) / 256), 256, 0, s>>>(
+                batch, m->n_in, m->idx.p + b0, m->x_all.p, m->a[0].p);
+            mlp_gather_kernel<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(
+                batch, m->n_out, m->idx.p + b0, m->t_all.p, m->t.p);
+            rc = mlp_forward_dev(m, batch);
+            if (!rc)
+                rc = mlp_backward_dev(m, batch);
+            if (rc)
+                return rc;
+            ++m->adam_t;
+            const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
+            const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
+            mlp_adam_kernel<<<(unsigned)((m->n_params + 255) / 256), 256, 0, s>>>(
+                (size_t)m->n_params, m->params.p, m->grads.p, m->adam_m.p, m->adam_v.p, lr, b1,
+                b2, eps, c1, c2);
+            HIP_TRY(ctx, hipGetLastError());
+            if (ep == epochs - 1) {
+                HIP_TRY(ctx, hipMemcpyAsync(sq.data(), m->row_sq.p, sizeof(float) * batch,
+                                            hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                for (float v : sq)
+                    ep_sum += v;
+                ep_rows += batch;
+            }
+        }
+        if (ep == epochs - 1 && ep_rows)
+            last = ep_sum / ((double)ep_rows * m->n_out);
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (final_loss)
+        *final_loss = last;
     return MBS_OK;
 }
 

@@ -77,6 +77,25 @@ def lib():
         L.mbs_autotune.restype = C.c_int
         L.mbs_set_graph.argtypes = [vp, C.c_int]
         L.mbs_set_graph.restype = C.c_int
+        fp = C.POINTER(C.c_float)
+        L.mbs_mlp_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_uint64, C.POINTER(vp)]
+        L.mbs_mlp_create.restype = C.c_int
+        L.mbs_mlp_destroy.argtypes = [vp]
+        L.mbs_mlp_destroy.restype = C.c_int
+        L.mbs_mlp_param_count.argtypes = [vp]
+        L.mbs_mlp_param_count.restype = C.c_int64
+        L.mbs_mlp_set_params.argtypes = [vp, fp]
+        L.mbs_mlp_set_params.restype = C.c_int
+        L.mbs_mlp_get_params.argtypes = [vp, fp]
+        L.mbs_mlp_get_params.restype = C.c_int
+        L.mbs_mlp_forward.argtypes = [vp, fp, C.c_int32, fp]
+        L.mbs_mlp_forward.restype = C.c_int
+        L.mbs_mlp_gradient.argtypes = [vp, fp, fp, C.c_int32, dp, fp]
+        L.mbs_mlp_gradient.restype = C.c_int
+        L.mbs_mlp_train.argtypes = [vp, fp, fp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
+                                    C.c_uint64, dp]
+        L.mbs_mlp_train.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -265,3 +284,65 @@ class Context:
         self._check(self._L.mbs_get_timing(self._h, C.byref(t)))
         return {"upload_ms": t.upload_ms, "kernel_ms": t.kernel_ms, "reduce_ms": t.reduce_ms,
                 "n_items": t.n_items, "n_blocks": t.n_blocks}
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """Device MLP of the learned control variate (mbs_mlp_*): n_in -> depth x
+    width GELU -> n_out, trained with the library's BF16-MFMA kernels."""
+
+    def __init__(self, ctx, n_in, width, depth, n_out, seed=0):
+        self.ctx = ctx
+        self.shape = (n_in, width, depth, n_out)
+        h = C.c_void_p()
+        ctx._check(ctx._L.mbs_mlp_create(ctx._h, n_in, width, depth, n_out, seed, C.byref(h)))
+        self._h = h
+        self.n_params = ctx._L.mbs_mlp_param_count(h)
+
+    def close(self):
+        if self._h:
+            self.ctx._L.mbs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self.ctx._check(self.ctx._L.mbs_mlp_get_params(self._h, _f(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.ctx._check(self.ctx._L.mbs_mlp_set_params(self._h, _f(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.shape[3]), dtype=np.float32)
+        self.ctx._check(self.ctx._L.mbs_mlp_forward(self._h, _f(x), x.shape[0], _f(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        self.ctx._check(self.ctx._L.mbs_mlp_gradient(self._h, _f(x), _f(t), x.shape[0],
+                                                     C.byref(loss), _f(g)))
+        return loss.value, g
+
+    def train(self, x, t, batch=256, epochs=10, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        loss = C.c_double()
+        self.ctx._check(self.ctx._L.mbs_mlp_train(self._h, _f(x), _f(t), x.shape[0],
+                                                  int(batch), int(epochs), float(lr), int(seed),
+                                                  C.byref(loss)))
+        return loss.value

@@ -361,3 +361,35 @@ def test_aperture_fft_oracle_against_closed_forms(pkg, oracle):
     want = np.array([oracle.aperture(px, py, syx, sxy, fl, A, B, consts, parea)
                      for A, B in ab]) / (-1j / 0.532)
     assert np.max(np.abs(got - want)) / parea < 2e-3
+
+
+# ---- control-variate MLP (oracle side) -------------------------------------------
+
+def test_mlp_oracle_gradient_against_finite_differences():
+    """The FP64 MLP restatement's analytic gradient equals central finite
+    differences of its loss (every parameter of a small network)."""
+    from oracle import mlp_oracle as mo
+    shapes = mo.layer_shapes(5, 6, 2, 4)
+    npar = sum(i * o + o for i, o in shapes)
+    rng = np.random.default_rng(4)
+    p = rng.normal(scale=0.5, size=npar)
+    x = rng.normal(size=(7, 5))
+    t = rng.normal(size=(7, 4))
+    loss, g = mo.loss_and_grad(p, shapes, x, t)
+    h = 1e-6
+    fd = np.empty(npar)
+    for i in range(npar):
+        e = np.zeros(npar)
+        e[i] = h
+        fd[i] = (mo.loss_and_grad(p + e, shapes, x, t)[0]
+                 - mo.loss_and_grad(p - e, shapes, x, t)[0]) / (2 * h)
+    assert np.max(np.abs(g - fd)) < 1e-8 * max(1.0, np.max(np.abs(fd)))
+    # GELU: Phi(z) + z phi(z) against a finite difference; GELU(0) = 0
+    z = np.linspace(-4, 4, 41)
+    np.testing.assert_allclose(mo.gelu_grad(z), (mo.gelu(z + 1e-6) - mo.gelu(z - 1e-6)) / 2e-6,
+                               atol=1e-8)
+    assert mo.gelu(np.array([0.0]))[0] == 0.0
+    # BF16 rounding: exact values survive, ties go to even
+    np.testing.assert_array_equal(mo.bf16(np.array([1.0, -3.0, 0.5])), [1.0, -3.0, 0.5])
+    assert mo.bf16(np.array([1.0 + 2 ** -8]))[0] == 1.0          # tie -> even (down)
+    assert mo.bf16(np.array([1.0 + 3 * 2 ** -8]))[0] == 1.0 + 2 ** -6  # tie -> even (up)
