@@ -193,6 +193,10 @@ int mbs_diffract_batch(mbs_ctx *ctx, const mbs_grid *grid,
  * 2*pi*dcos, M11..M44 (row-major 4x4).  noshadow != 0 averages the no-shadow
  * grid (requires MBS_NOSHADOW on the last mbs_diffract).  Overwrites rows. */
 int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows);
+/* The same for every orientation of the last mbs_diffract with
+ * MBS_PER_ORIENT: rows [n_orient][n_zen+1][18] (each grid carries its
+ * orientation's weight, as mbs_fetch_per_orient returns it). */
+int mbs_azimuth_average_per_orient(mbs_ctx *ctx, double *rows);
 
 /* Orientation sets of the orientation average, generated on the device
  * (one row of MBS_ORIENT_STRIDE doubles per orientation: beta, gamma, weight,

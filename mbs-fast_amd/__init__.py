@@ -9,11 +9,11 @@ module name ``mbs_fast_amd``.
 """
 from . import abi  # noqa: F401
 
-__all__ = ["abi", "engine", "handler", "synthetic"]
+__all__ = ["abi", "engine", "handler", "synthetic", "cv"]
 
 
 def __getattr__(name):
-    if name in ("engine", "handler", "build", "synthetic"):
+    if name in ("engine", "handler", "build", "synthetic", "cv"):
         import importlib
         return importlib.import_module(__name__ + "." + name)
     raise AttributeError(name)

@@ -34,6 +34,7 @@ NO_MFMA = 16
 
 ORIENT_SOBOL, ORIENT_HAMMERSLEY, ORIENT_SHOEMAKE = 0, 1, 2
 ORIENT_STRIDE = 16
+ORIENT_BETA, ORIENT_GAMMA, ORIENT_W, ORIENT_Q, ORIENT_R = 0, 1, 2, 3, 7
 
 _dp = C.POINTER(C.c_double)
 _i32p = C.POINTER(C.c_int32)

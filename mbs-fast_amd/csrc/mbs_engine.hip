@@ -865,6 +865,22 @@ int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows)
     return fetch(ctx, ctx->avg_rows.p, n, rows, false);
 }
 
+int mbs_azimuth_average_per_orient(mbs_ctx *ctx, double *rows)
+{
+    if (!ctx || !rows)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_azimuth_average_per_orient: NULL argument");
+    if (!ctx->have_result || !ctx->have_per)
+        return fail(ctx, MBS_ERR_INVALID, "per-orientation result was not requested");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->n_orient * ctx->n_zen1 * 18;
+    HIP_TRY(ctx, ctx->avg_rows.ensure(n));
+    azimuth_average_kernel<<<dim3(ctx->n_zen1, ctx->n_orient), 64, 0, ctx->stream>>>(
+        ctx->per_orient.p, ctx->theta.p, ctx->n_az, ctx->n_zen1 - 1, ctx->az_step,
+        ctx->avg_rows.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return fetch(ctx, ctx->avg_rows.p, n, rows, false);
+}
+
 int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double beta_sym,
                         double gamma_sym, double *out)
 {

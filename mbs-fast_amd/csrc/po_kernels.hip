@@ -860,12 +860,16 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
 // (src/handler/Handler.h:70-125) in the stable sin*sin form.  One wave per
 // row; the 64 lanes stride over azimuths and combine by a fixed butterfly, so
 // the result is deterministic.
+// blockIdx.y selects one of several grids stored back to back (the
+// per-orientation grids of an MBS_PER_ORIENT run).
 __global__ void __launch_bounds__(64) azimuth_average_kernel(
     const double *__restrict__ m, const double *__restrict__ theta, int n_az, int n_zen,
     double az_step, double *__restrict__ rows)
 {
     const int t = blockIdx.x;
     const int nZ = n_zen + 1;
+    m += (size_t)blockIdx.y * n_az * nZ * 16;
+    rows += (size_t)blockIdx.y * nZ * 18;
     const double th = theta[t];
     const double tol = 256.0 * 2.220446049250313e-16;
     const bool fwd = __builtin_fabs(th) <= tol;

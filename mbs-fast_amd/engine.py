@@ -96,6 +96,8 @@ def lib():
         L.mbs_mlp_train.argtypes = [vp, fp, fp, C.c_int32, C.c_int32, C.c_int32, C.c_float,
                                     C.c_uint64, dp]
         L.mbs_mlp_train.restype = C.c_int
+        L.mbs_azimuth_average_per_orient.argtypes = [vp, dp]
+        L.mbs_azimuth_average_per_orient.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -237,6 +239,12 @@ class Context:
         r, c, s = C.c_float(), C.c_float(), C.c_float()
         self._check(self._L.mbs_aperture_fft_timing(self._h, C.byref(r), C.byref(c), C.byref(s)))
         return {"rows_ms": r.value, "cols_ms": c.value, "sample_ms": s.value}
+
+    def azimuth_average_per_orient(self):
+        """Rows [n_orient, n_zen+1, 18] of the last MBS_PER_ORIENT result."""
+        out = np.zeros((self.batch.n_orient, self.grid.n_zen + 1, 18))
+        self._check(self._L.mbs_azimuth_average_per_orient(self._h, _d(out)))
+        return out
 
     def per_orient(self):
         g = self.grid
