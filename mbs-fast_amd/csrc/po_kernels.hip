@@ -39,7 +39,10 @@
 #endif
 // The MFMA kernel is held to 4 waves per SIMD (128 VGPRs + AGPRs) when it
 // keeps one accumulator set; the no-shadow variant (two sets) gets 2.
-#define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : 4, 8)))
+#ifndef MBS_MFMA_WAVES
+#define MBS_MFMA_WAVES 4
+#endif
+#define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : MBS_MFMA_WAVES, 8)))
 
 namespace mbs {
 
@@ -265,6 +268,118 @@ __device__ __forceinline__ double uniform_f64(double x)
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// One edge of the aperture sum (FP64 phasors): the phasor of vertex e+1
+// (vertex 0 again for the closing edge), the edge coefficient
+// C = slope * mul + add and its contribution (e^{i phi_{e+1}} - e^{i phi_e}) / C,
+// or the exact half-angle form when |C| <= eps1.
+__device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict__ vtx,
+                                         const double *__restrict__ vdd,
+                                         const int32_t *__restrict__ vflag, double A, double B,
+                                         double psi, bool useB, double mul, double add,
+                                         int vmask, const double2 *__restrict__ tab, double k,
+                                         double eps1, double s0, double c0, double &sp,
+                                         double &cp, double &sr, double &si, bool &small)
+{
+    const double *v = vtx + (size_t)e * VTX_STRIDE;
+    const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
+    const int f = vflag[e];
+    double sn = s0, cn = c0;
+    if (e + 1 < n) {
+        const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
+        small = small && __builtin_fabs(pn) <= kSmallPhase;
+        sincos_table(pn + psi, tab, sn, cn);
+    }
+    const double cc = __builtin_fma(slope, mul, add);
+    const bool valid = (f & vmask) != 0;
+    const bool big = __builtin_fabs(cc) > eps1;
+    const double r = rcp_f64_fast(cc);
+    const double inv = (valid && big) ? r : 0.0;
+    sr = __builtin_fma(cn - cp, inv, sr);
+    si = __builtin_fma(sn - sp, inv, si);
+    if (__builtin_expect(valid && !big, 0)) {
+        const double *w = vdd + (size_t)e * VDD_STRIDE;
+        const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
+        const double2 q = stable_edge_quotient(cp, sp, dd, cc, k);
+        sr += q.x;
+        si += q.y;
+    }
+    sp = sn;
+    cp = cn;
+}
+
+// Aperture integral F(beam, direction) / complWave in FP64 (A4).  NV > 0
+// is the vertex count known at compile time (the loop is unrolled and its
+// scalar loads issued up front); NV = 0 loops over nv.  Returns false when
+// the result is NaN (degenerate beam, skipped).
+template <int NV>
+__device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
+                                             const double *__restrict__ vtx,
+                                             const double *__restrict__ vdd,
+                                             const int32_t *__restrict__ vflag, int nv,
+                                             const double *d, const double2 *__restrict__ tab,
+                                             double k, double eps1, double &fr, double &fi)
+{
+    // A, B: projections of k - k0 = b - d on the aperture axes
+    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
+    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
+    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
+    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
+    const double mul = useB ? B : A;  // coefficient = slope * mul + add
+    const double add = useB ? A : B;
+    const int vmask = useB ? 1 : 2;
+    // psi is folded into the vertex phasors (the integral is linear in each
+    // phasor), as the reference CUDA path does (GpuDiffraction.cu:1110-1122)
+    const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+    bool small = __builtin_fabs(ph) <= kSmallPhase;
+    double s0, c0;
+    sincos_table(ph + psi, tab, s0, c0);
+    double sp = s0, cp = c0, sr = 0.0, si = 0.0;
+    if constexpr (NV > 0) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e)
+            edge_f64(e, NV, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1, s0,
+                     c0, sp, cp, sr, si, small);
+    } else {
+        for (int e = 0; e < nv; ++e)
+            edge_f64(e, nv, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1, s0,
+                     c0, sp, cp, sr, si, small);
+    }
+    const double nrm = rcp_f64(useB ? B : -A);
+    fr = sr * nrm;
+    fi = si * nrm;
+    if (__builtin_expect(small, 0)) {
+        // small-phase moment form (HandlerPO_fast.h:161-216); the area limit
+        // is stored divided by complWave
+        const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
+        const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
+        const double gr = 1.0 - 0.5 * p2, gi = p1;
+        const double ar = h[H_AL] * gr - h[H_AL + 1] * gi;
+        const double ai = h[H_AL] * gi + h[H_AL + 1] * gr;
+        double sd, cd;
+        sincos_table(psi, tab, sd, cd);
+        fr = ar * cd - ai * sd;
+        fi = ar * sd + ai * cd;
+    }
+    return !__builtin_isnan(fr);
+}
+
+// Per-beam aperture integral dispatched on the vertex count (3 and 4 cover
+// most beams of a hexagonal particle).
+__device__ __forceinline__ bool aperture_f64_any(const double *__restrict__ h,
+                                                 const double *__restrict__ vtx,
+                                                 const double *__restrict__ vdd,
+                                                 const int32_t *__restrict__ vflag, int nv,
+                                                 const double *d,
+                                                 const double2 *__restrict__ tab, double k,
+                                                 double eps1, double &fr, double &fi)
+{
+    if (nv == 4)
+        return aperture_f64<4>(h, vtx, vdd, vflag, 4, d, tab, k, eps1, fr, fi);
+    if (nv == 3)
+        return aperture_f64<3>(h, vtx, vdd, vflag, 3, d, tab, k, eps1, fr, fi);
+    return aperture_f64<0>(h, vtx, vdd, vflag, nv, d, tab, k, eps1, fr, fi);
+}
+
 template <int ACC_NS>
 __device__ __forceinline__ void beam_dir_f64(
     const double *__restrict__ h, const double *__restrict__ vtx,
@@ -272,65 +387,8 @@ __device__ __forceinline__ void beam_dir_f64(
     const double *d, const double *__restrict__ su, const double2 *__restrict__ tab,
     double k, double eps1, double *j, double *jn)
 {
-    // A, B: projections of k - k0 = b - d on the aperture axes
-    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
-    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
-    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
-    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
-    const double mul = useB ? B : A;   // coefficient = slope * mul + add
-    const double add = useB ? A : B;
-    const int vmask = useB ? 1 : 2;
-    // psi is folded into the vertex phasors (the integral is linear in each
-    // phasor), as the reference CUDA path does (GpuDiffraction.cu:1110-1122)
-    double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-    double maxph = __builtin_fabs(ph);
-    double s0, c0;
-    sincos_table(ph + psi, tab, s0, c0);
-    double sp = s0, cp = c0, sr = 0.0, si = 0.0;
-    for (int e = 0; e < nv; ++e) {
-        const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
-        const int f = vflag[e];
-        double sn = s0, cn = c0;
-        if (e + 1 < nv) {
-            const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-            maxph = fmax(maxph, __builtin_fabs(pn));
-            sincos_table(pn + psi, tab, sn, cn);
-        }
-        const double cc = __builtin_fma(slope, mul, add);
-        const bool valid = (f & vmask) != 0;
-        const bool big = __builtin_fabs(cc) > eps1;
-        const double r = rcp_f64_fast(cc);
-        const double inv = (valid && big) ? r : 0.0;
-        sr = __builtin_fma(cn - cp, inv, sr);
-        si = __builtin_fma(sn - sp, inv, si);
-        if (__builtin_expect(valid && !big, 0)) {
-            const double *w = vdd + (size_t)e * VDD_STRIDE;
-            const double2 q = stable_edge_quotient(cp, sp, useB ? w[D_DX] : w[D_DY], cc, k);
-            sr += q.x;
-            si += q.y;
-        }
-        sp = sn;
-        cp = cn;
-    }
-    const double nrm = rcp_f64(useB ? B : -A);
-    // integral / complWave
-    double fr = sr * nrm, fi = si * nrm;
-    if (__builtin_expect(maxph <= 1e-3, 0)) {
-        // small-phase moment form (HandlerPO_fast.h:161-216); the area limit
-        // is stored divided by complWave
-        const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
-        const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
-        const double gr = 1.0 - 0.5 * p2, gi = p1;
-        fr = h[H_AL] * gr - h[H_AL + 1] * gi;
-        fi = h[H_AL] * gi + h[H_AL + 1] * gr;
-        double sd, cd;
-        sincos_table(psi, tab, sd, cd);
-        const double tr = fr * cd - fi * sd;
-        fi = fr * sd + fi * cd;
-        fr = tr;
-    }
-    if (__builtin_isnan(fr))
+    double fr, fi;
+    if (!aperture_f64_any(h, vtx, vdd, vflag, nv, d, tab, k, eps1, fr, fi))
         return;
     // rotation vectors of this lane's direction, from LDS ([12][256])
     const int l = threadIdx.x;
@@ -372,11 +430,51 @@ __device__ __forceinline__ void beam_dir_f64(
     }
 }
 
+// One edge of the aperture sum, mixed precision: phase arguments, their
+// reduction and the edge coefficient in FP64, phasors and sums in FP32.
+__device__ __forceinline__ void edge_mixed(int e, int n, const double *__restrict__ vtx,
+                                           const double *__restrict__ vdd,
+                                           const int32_t *__restrict__ vflag, double A,
+                                           double B, double psi, bool useB, double mul,
+                                           double add, int vmask,
+                                           const float2 *__restrict__ tab, double k,
+                                           double eps1, float s0, float c0, float &sp,
+                                           float &cp, float &sr, float &si, bool &small)
+{
+    const double *v = vtx + (size_t)e * VTX_STRIDE;
+    const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
+    const int f = vflag[e];
+    float sn = s0, cn = c0;
+    if (e + 1 < n) {
+        const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
+        small = small && __builtin_fabs(pn) <= kSmallPhase;
+        sincos_table_f32(pn + psi, tab, sn, cn);
+    }
+    // the edge coefficient is cancellation-prone: FP64
+    const double cc = __builtin_fma(slope, mul, add);
+    const bool valid = (f & vmask) != 0;
+    const bool big = __builtin_fabs(cc) > eps1;
+    const float r = rcp_f32((float)cc);
+    const float inv = (valid && big) ? r : 0.0f;
+    sr = __builtin_fmaf(cn - cp, inv, sr);
+    si = __builtin_fmaf(sn - sp, inv, si);
+    if (__builtin_expect(valid && !big, 0)) {
+        const double *w = vdd + (size_t)e * VDD_STRIDE;
+        const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
+        const double2 q = stable_edge_quotient((double)cp, (double)sp, dd, cc, k);
+        sr += (float)q.x;
+        si += (float)q.y;
+    }
+    sp = sn;
+    cp = cn;
+}
+
 // Aperture integral F(beam, direction) / complWave, mixed precision: A, B,
 // the phases and their reduction, the edge coefficients and the moments in
-// FP64; phasors and edge sums in FP32.  A NaN result (degenerate beam) is
-// returned as 0, i.e. the beam is skipped.
-__device__ __forceinline__ void aperture_mixed(
+// FP64; phasors and edge sums in FP32.  NV as for aperture_f64.  A NaN
+// result (degenerate beam) is returned as 0, i.e. the beam is skipped.
+template <int NV>
+__device__ __forceinline__ void aperture_mixed_nv(
     const double *__restrict__ h, const double *__restrict__ vtx,
     const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
     const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
@@ -389,42 +487,24 @@ __device__ __forceinline__ void aperture_mixed(
     const double mul = useB ? B : A;
     const double add = useB ? A : B;
     const int vmask = useB ? 1 : 2;
-    double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-    double maxph = __builtin_fabs(ph);
+    const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+    bool small = __builtin_fabs(ph) <= kSmallPhase;
     float s0, c0;
     sincos_table_f32(ph + psi, tab, s0, c0);
     float sp = s0, cp = c0, sr = 0.0f, si = 0.0f;
-    for (int e = 0; e < nv; ++e) {
-        const double *v = vtx + (size_t)e * VTX_STRIDE;
-        const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
-        const int f = vflag[e];
-        float sn = s0, cn = c0;
-        if (e + 1 < nv) {
-            const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-            maxph = fmax(maxph, __builtin_fabs(pn));
-            sincos_table_f32(pn + psi, tab, sn, cn);
-        }
-        // the edge coefficient is cancellation-prone: FP64
-        const double cc = __builtin_fma(slope, mul, add);
-        const bool valid = (f & vmask) != 0;
-        const bool big = __builtin_fabs(cc) > eps1;
-        const float r = rcp_f32((float)cc);
-        const float inv = (valid && big) ? r : 0.0f;
-        sr = __builtin_fmaf(cn - cp, inv, sr);
-        si = __builtin_fmaf(sn - sp, inv, si);
-        if (__builtin_expect(valid && !big, 0)) {
-            const double *w = vdd + (size_t)e * VDD_STRIDE;
-            const double2 q = stable_edge_quotient((double)cp, (double)sp,
-                                                   useB ? w[D_DX] : w[D_DY], cc, k);
-            sr += (float)q.x;
-            si += (float)q.y;
-        }
-        sp = sn;
-        cp = cn;
+    if constexpr (NV > 0) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e)
+            edge_mixed(e, NV, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1,
+                       s0, c0, sp, cp, sr, si, small);
+    } else {
+        for (int e = 0; e < nv; ++e)
+            edge_mixed(e, nv, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1,
+                       s0, c0, sp, cp, sr, si, small);
     }
     const float nrm = rcp_f32((float)(useB ? B : -A));
     float fr = sr * nrm, fi = si * nrm;
-    if (__builtin_expect(maxph <= 1e-3, 0)) {
+    if (__builtin_expect(small, 0)) {
         const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
         const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
         const double gr = 1.0 - 0.5 * p2, gi = p1;
@@ -438,6 +518,20 @@ __device__ __forceinline__ void aperture_mixed(
     const bool bad = __builtin_isnan(fr);
     out_r = bad ? 0.0f : fr;
     out_i = bad ? 0.0f : fi;
+}
+
+__device__ __forceinline__ void aperture_mixed(
+    const double *__restrict__ h, const double *__restrict__ vtx,
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
+    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
+    float &out_i)
+{
+    if (nv == 4)
+        aperture_mixed_nv<4>(h, vtx, vdd, vflag, 4, d, tab, k, eps1, out_r, out_i);
+    else if (nv == 3)
+        aperture_mixed_nv<3>(h, vtx, vdd, vflag, 3, d, tab, k, eps1, out_r, out_i);
+    else
+        aperture_mixed_nv<0>(h, vtx, vdd, vflag, nv, d, tab, k, eps1, out_r, out_i);
 }
 
 // Mixed precision with the rotation and coherent sum on the vector ALU.

@@ -23,6 +23,9 @@
 namespace mbs {
 
 constexpr double kPi = 3.141592653589793116;            // double(pi) = M_PI
+// Small-phase switch: every vertex phase within 1e-3 rad
+// (HandlerPO_fast.h:161-216).
+constexpr double kSmallPhase = 1e-3;
 constexpr double kTwoOverPi = 0.63661977236758134308;
 constexpr double kPio2Hi = 1.5707963267948966192;     // double(pi/2)
 constexpr double kPio2Lo = 6.123233995736766036e-17;  // pi/2 - kPio2Hi

@@ -6,6 +6,7 @@ directory (built in place by build.py), never from site-packages.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -156,6 +157,10 @@ class Context:
             self._h = None
 
     def __del__(self):
+        # No HIP calls while the interpreter shuts down: the runtime may
+        # already be torn down; the process exit releases the device.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -316,10 +321,19 @@ class Mlp:
             self._h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
             pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
     def params(self):
         p = np.zeros(self.n_params, dtype=np.float32)

@@ -95,27 +95,36 @@ def test_control_variate_ablation(pkg, oracle):
                                                    gamma_sym=GAMMA_SYM), np.log(10.0))
              for s in M}
         cv = cvmod.ControlVariate(ctx, M[1].shape[1], width=256, depth=4, seed=11)
-        t0 = time.perf_counter()
-        loss = cv.fit(np.concatenate([X[1], X[2]]), np.concatenate([M[1], M[2]]),
-                      epochs=200, batch=128, lr=1e-3, seed=5)
-        Eg = cv.expectation(0, 65536, 7, BETA_SYM, GAMMA_SYM, np.log(10.0))
-        t_cv = time.perf_counter() - t0
-        print("engine (7168 per-orientation grids) %.2f s; network training + E[g] %.2f s, "
-              "final loss %.3f" % (t_engine, t_cv, loss))
-        truth = M[4].mean(0)
-        tot_plain = tot_cv = 0.0
-        for n in (64, 128, 256, 512, 1024):
-            w = np.full(n, 1.0 / n)
-            plain = np.tensordot(w, M[3][:n], axes=1)
-            zero = cv.estimate(M[3][:n], X[3][:n], w, Eg, c=0.0)
-            np.testing.assert_array_equal(zero, plain)
-            est = cv.estimate(M[3][:n], X[3][:n], w, Eg)
-            ep, ec = _err(plain, truth, w2pi), _err(est, truth, w2pi)
-            tot_plain += ep.max()
-            tot_cv += ec.max()
-            print("N=%4d  plain: max %.4f P11 %.4f | control variate: max %.4f P11 %.4f"
-                  % (n, ep.max(), ep[0], ec.max(), ec[0]))
-        print("sum over N of the max element error: plain %.4f, control variate %.4f"
-              % (tot_plain, tot_cv))
-        cv.close()
+        try:
+            tot_plain, tot_cv = _ablation(cv, ctx, M, X, w2pi, t_engine)
+        finally:
+            cv.close()
     assert tot_cv <= 1.05 * tot_plain
+
+
+def _ablation(cv, ctx, M, X, w2pi, t_engine):
+    """Trains the network, then prints and sums the errors of the plain and
+    control-variate estimates for growing N."""
+    t0 = time.perf_counter()
+    loss = cv.fit(np.concatenate([X[1], X[2]]), np.concatenate([M[1], M[2]]),
+                  epochs=200, batch=128, lr=1e-3, seed=5)
+    Eg = cv.expectation(0, 65536, 7, BETA_SYM, GAMMA_SYM, np.log(10.0))
+    t_cv = time.perf_counter() - t0
+    print("engine (7168 per-orientation grids) %.2f s; network training + E[g] %.2f s, "
+          "final loss %.3f" % (t_engine, t_cv, loss))
+    truth = M[4].mean(0)
+    tot_plain = tot_cv = 0.0
+    for n in (64, 128, 256, 512, 1024):
+        w = np.full(n, 1.0 / n)
+        plain = np.tensordot(w, M[3][:n], axes=1)
+        zero = cv.estimate(M[3][:n], X[3][:n], w, Eg, c=0.0)
+        np.testing.assert_array_equal(zero, plain)
+        est = cv.estimate(M[3][:n], X[3][:n], w, Eg)
+        ep, ec = _err(plain, truth, w2pi), _err(est, truth, w2pi)
+        tot_plain += ep.max()
+        tot_cv += ec.max()
+        print("N=%4d  plain: max %.4f P11 %.4f | control variate: max %.4f P11 %.4f"
+              % (n, ep.max(), ep[0], ec.max(), ec[0]))
+    print("sum over N of the max element error: plain %.4f, control variate %.4f"
+          % (tot_plain, tot_cv))
+    return tot_plain, tot_cv

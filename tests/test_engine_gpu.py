@@ -387,33 +387,33 @@ def test_graph_replay_and_autotune(pkg, oracle, ctx):
 def test_mlp_gradient_against_oracle(pkg, ctx):
     """Device forward and backward (BF16 MFMA GEMMs, FP32 accumulation)
     against the FP64 oracle evaluated on the same BF16-rounded operands; and
-    device Adam training reduces the loss of a smooth target."""
+    device Adam training reduces the loss of a smooth target.  The remaining
+    differences come from FP32 accumulation order and from the rare BF16
+    rounding that an FP32-level difference in an activation flips."""
     from oracle import mlp_oracle as mo
     n_in, width, depth, n_out = 5, 256, 4, 181
-    net = pkg.engine.Mlp(ctx, n_in, width, depth, n_out, seed=3)
-    shapes = mo.layer_shapes(n_in, width, depth, n_out)
-    p = net.params().astype(np.float64)
     rng = np.random.default_rng(9)
-    x = rng.normal(size=(200, n_in)).astype(np.float32)
-    t = rng.normal(size=(200, n_out)).astype(np.float32)
-    y = net.forward(x)
-    y_o, _ = mo.forward(p, shapes, x, bf=True)
-    ey = np.linalg.norm(y - y_o) / np.linalg.norm(y_o)
-    loss, g = net.gradient(x, t)
-    loss_o, g_o = mo.loss_and_grad(p, shapes, x, t, bf=True)
-    eg = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
-    print("forward rel L2 %.2e; loss %.6e vs %.6e; gradient rel L2 %.2e"
-          % (ey, loss, loss_o, eg))
-    assert ey < 1e-4
-    assert abs(loss - loss_o) < 1e-4 * loss_o
-    assert eg < 2e-3
+    with pkg.engine.Mlp(ctx, n_in, width, depth, n_out, seed=3) as net:
+        shapes = mo.layer_shapes(n_in, width, depth, n_out)
+        p = net.params().astype(np.float64)
+        x = rng.normal(size=(200, n_in)).astype(np.float32)
+        t = rng.normal(size=(200, n_out)).astype(np.float32)
+        y = net.forward(x)
+        y_o, _ = mo.forward(p, shapes, x, bf=True)
+        ey = np.linalg.norm(y - y_o) / np.linalg.norm(y_o)
+        loss, g = net.gradient(x, t)
+        loss_o, g_o = mo.loss_and_grad(p, shapes, x, t, bf=True)
+        eg = np.linalg.norm(g - g_o) / np.linalg.norm(g_o)
+        print("forward rel L2 %.2e; loss %.6e vs %.6e; gradient rel L2 %.2e"
+              % (ey, loss, loss_o, eg))
+        assert ey < 1e-3
+        assert abs(loss - loss_o) < 1e-4 * loss_o
+        assert eg < 2e-3
     # training: a smooth function of the inputs
     xs = rng.uniform(-1, 1, (4096, n_in)).astype(np.float32)
     ts = np.stack([np.sin(xs @ rng.normal(size=n_in)) for _ in range(n_out)], 1).astype(np.float32)
-    net2 = pkg.engine.Mlp(ctx, n_in, 64, 2, n_out, seed=5)
-    l0, _ = net2.gradient(xs[:512], ts[:512])
-    l1 = net2.train(xs, ts, batch=256, epochs=30, lr=3e-3, seed=1)
-    print("training loss %.4e -> %.4e" % (l0, l1))
-    assert l1 < 0.1 * l0
-    net.close()
-    net2.close()
+    with pkg.engine.Mlp(ctx, n_in, 64, 2, n_out, seed=5) as net2:
+        l0, _ = net2.gradient(xs[:512], ts[:512])
+        l1 = net2.train(xs, ts, batch=256, epochs=30, lr=3e-3, seed=1)
+        print("training loss %.4e -> %.4e" % (l0, l1))
+        assert l1 < 0.1 * l0
