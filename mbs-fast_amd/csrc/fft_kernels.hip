@@ -29,7 +29,10 @@
 
 namespace mbs {
 
-enum { FFT_THREADS = 256 };
+// Workgroup sizes: the row kernel (2 workgroups of 74 KB LDS per CU) runs
+// 512 threads, the column kernel (1 workgroup of 139 KB) 1024, so that each
+// SIMD keeps 4 waves in flight either way.
+enum { FFT_ROW_THREADS = 512, FFT_COL_THREADS = 1024 };
 // LDS sequences are FFT_PAD elements longer than n, so that the 8 columns of
 // the column kernel start 16 banks apart.
 enum { FFT_PAD = 8 };
@@ -99,14 +102,22 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// twiddles tw[m] = exp(-2 pi i m / n), m < n/2, computed in FP64
-__device__ __forceinline__ void fft_twiddles(float2 *tw, int n)
+// twiddles tw[m] = exp(-2 pi i m / n), m < n/2, computed once per call in
+// FP64 and rounded; the FFT kernels copy them into LDS.
+__global__ void fft_twiddle_kernel(int n, float2 *__restrict__ tw)
 {
-    for (int m = threadIdx.x; m < n / 2; m += blockDim.x) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n / 2) {
         double s, c;
         sincospi(-2.0 * (double)m / (double)n, &s, &c);
         tw[m] = make_float2((float)c, (float)s);
     }
+}
+
+__device__ __forceinline__ void fft_twiddles(float2 *tw, const float2 *__restrict__ gtw, int n)
+{
+    for (int m = threadIdx.x; m < n / 2; m += blockDim.x)
+        tw[m] = gtw[m];
 }
 
 // In-place FFTs of `count` length-n sequences stored in LDS at a stride of
@@ -178,8 +189,9 @@ __device__ __forceinline__ int fft_logical_block()
 // ROWS * (n + FFT_PAD) float2 + n/2 float2 twiddles.  Thread j rasterises
 // pixel bitrev(j) into LDS slot j (linear, conflict-free stores).
 template <int ROWS>
-__global__ void __launch_bounds__(FFT_THREADS) fft_rows_kernel(
-    int n_beams, int n, int logn, const double *__restrict__ rec, float2 *__restrict__ grid)
+__global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
+    int n_beams, int n, int logn, const double *__restrict__ rec,
+    const float2 *__restrict__ gtw, float2 *__restrict__ grid)
 {
     extern __shared__ float2 lds[];
     const int units_per_beam = n / ROWS;
@@ -191,7 +203,7 @@ __global__ void __launch_bounds__(FFT_THREADS) fft_rows_kernel(
     const int ld = n + FFT_PAD;
     float2 *data = lds;
     float2 *tw = lds + ROWS * ld;
-    fft_twiddles(tw, n);
+    fft_twiddles(tw, gtw, n);
     const double *r = rec + (size_t)beam * FB_STRIDE;
     const int ne = (int)r[FB_NE];
     const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
@@ -215,8 +227,8 @@ __global__ void __launch_bounds__(FFT_THREADS) fft_rows_kernel(
 // COLS columns of 64/COLS rows (COLS*8-byte row segments); row bitrev(j)
 // goes to slot j.
 template <int COLS>
-__global__ void __launch_bounds__(FFT_THREADS) fft_cols_kernel(int n_beams, int n, int logn,
-                                                               float2 *__restrict__ grid)
+__global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
+    int n_beams, int n, int logn, const float2 *__restrict__ gtw, float2 *__restrict__ grid)
 {
     extern __shared__ float2 lds[];
     const int units_per_beam = n / COLS;
@@ -228,10 +240,10 @@ __global__ void __launch_bounds__(FFT_THREADS) fft_cols_kernel(int n_beams, int 
     const int ld = n + FFT_PAD;
     float2 *data = lds;
     float2 *tw = lds + COLS * ld;
-    fft_twiddles(tw, n);
+    fft_twiddles(tw, gtw, n);
     float2 *g = grid + (size_t)beam * n * n + col0;
     const int c = threadIdx.x & (COLS - 1);
-    const int r_step = FFT_THREADS / COLS;
+    const int r_step = blockDim.x / COLS;
     for (int j = threadIdx.x / COLS; j < n; j += r_step)
         data[c * ld + j] = g[(size_t)fft_bitrev(j, logn) * n + c];
     __syncthreads();

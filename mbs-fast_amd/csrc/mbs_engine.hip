@@ -86,7 +86,7 @@ struct mbs_ctx {
     DevBuf<WorkItem> items;
     DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows, orient;
     // aperture FFT
-    DevBuf<float2> fft_grid;
+    DevBuf<float2> fft_grid, fft_tw;
     DevBuf<double> fft_rec, fft_ab, fft_out;
     DevBuf<int64_t> fft_beams;
     int fft_n = 0, fft_batch = 0;
@@ -255,6 +255,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     if (ctx->graph_exec)
         (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->fft_grid.release();
+    ctx->fft_tw.release();
     ctx->fft_rec.release();
     ctx->fft_ab.release();
     ctx->fft_out.release();
@@ -761,8 +762,8 @@ static int launch_fft_rows(mbs_ctx *ctx, int nb, int n, int logn)
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int units = nb * (n / R);
     const int blocks = (units + 7) / 8 * 8;
-    fft_rows_kernel<R><<<blocks, FFT_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_rec.p,
-                                                                ctx->fft_grid.p);
+    fft_rows_kernel<R><<<blocks, FFT_ROW_THREADS, lds, ctx->stream>>>(
+        nb, n, logn, ctx->fft_rec.p, ctx->fft_tw.p, ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
     return MBS_OK;
 }
@@ -775,7 +776,8 @@ static int launch_fft_cols(mbs_ctx *ctx, int nb, int n, int logn)
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int units = nb * (n / C);
     const int blocks = (units + 7) / 8 * 8;
-    fft_cols_kernel<C><<<blocks, FFT_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_grid.p);
+    fft_cols_kernel<C><<<blocks, FFT_COL_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_tw.p,
+                                                                    ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
     return MBS_OK;
 }
@@ -941,6 +943,8 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->fft_beams.p, beams, sizeof(int64_t) * n_beams,
                                 hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, ctx->fft_tw.ensure(n / 2));
+    fft_twiddle_kernel<<<(n / 2 + 255) / 256, 256, 0, s>>>(n, ctx->fft_tw.p);
     fft_setup_kernel<<<(n_beams + 63) / 64, 64, 0, s>>>(n_beams, ctx->fft_beams.p, ctx->nv.p,
                                                        ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
                                                        n, pad, ctx->fft_rec.p);
