@@ -239,7 +239,7 @@ int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double
  * call).  Device workspace: n_beams * n * n * 8 bytes. */
 #define MBS_FFT_MAX_BATCH 64
 #define MBS_FFT_MIN_N 64
-#define MBS_FFT_MAX_N 4096
+#define MBS_FFT_MAX_N 2048
 int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_t n,
                      double pad, int32_t n_points, const double *ab, int32_t radius,
                      double *out);

@@ -19,9 +19,15 @@
 //                     their n-point FFTs in LDS, writes them (n^2 x 8 B)
 //   fft_cols_kernel   reads COLS adjacent columns (64-byte row segments),
 //                     does their FFTs in LDS, writes them back (2 n^2 x 8 B)
-// The 1-D FFT is radix-2 decimation in time, in place in LDS after a
-// bit-reversed store, with two stages fused per pass (4 points per thread
-// and pass, radix-4 style), twiddles from an LDS table made in FP64.
+// The 1-D FFT is the Stockham autosort formulation (Govindaraju et al.,
+// "High performance discrete Fourier transforms on graphics processors",
+// SC08): radix-8 stages and one radix-4 or radix-2 stage, each thread taking
+// whole butterflies into registers, with a barrier between the read and the
+// write phase so the stages run in place in one LDS buffer.  The first stage
+// reads the raster (row pass) or HBM (column pass) directly and the last
+// stage writes HBM directly, so each pass touches HBM once per element.
+// Twiddles come from a table of exp(-2 pi i m / n) made once per call in
+// FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,13 +35,10 @@
 
 namespace mbs {
 
-// Workgroup sizes: the row kernel (2 workgroups of 74 KB LDS per CU) runs
-// 512 threads, the column kernel (1 workgroup of 139 KB) 1024, so that each
-// SIMD keeps 4 waves in flight either way.
-enum { FFT_ROW_THREADS = 512, FFT_COL_THREADS = 1024 };
-// LDS sequences are FFT_PAD elements longer than n, so that the 8 columns of
-// the column kernel start 16 banks apart.
-enum { FFT_PAD = 8 };
+// Workgroups: the row kernel holds FFT_ROWS rows (2 workgroups of 68 KB LDS
+// per CU at n = 2048) with 512 threads, the column kernel FFT_COLS columns
+// (one workgroup of 135 KB) with 1024 threads: 4 waves per SIMD either way.
+enum { FFT_ROW_THREADS = 512, FFT_COL_THREADS = 1024, FFT_ROWS = 4, FFT_COLS = 8 };
 enum { FB_XC = 0, FB_YC = 1, FB_DX = 2, FB_DY = 3, FB_NE = 4, FB_EDGE = 5, FB_STRIDE = 5 + 3 * 32 };
 
 // Per-beam raster set-up: bounding box, pixel size and the edge lines
@@ -102,78 +105,16 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// twiddles tw[m] = exp(-2 pi i m / n), m < n/2, computed once per call in
-// FP64 and rounded; the FFT kernels copy them into LDS.
+// twiddles tw[m] = exp(-2 pi i m / n), m < n, computed once per call in
+// FP64 and rounded.
 __global__ void fft_twiddle_kernel(int n, float2 *__restrict__ tw)
 {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < n / 2) {
+    if (m < n) {
         double s, c;
         sincospi(-2.0 * (double)m / (double)n, &s, &c);
         tw[m] = make_float2((float)c, (float)s);
     }
-}
-
-__device__ __forceinline__ void fft_twiddles(float2 *tw, const float2 *__restrict__ gtw, int n)
-{
-    for (int m = threadIdx.x; m < n / 2; m += blockDim.x)
-        tw[m] = gtw[m];
-}
-
-// In-place FFTs of `count` length-n sequences stored in LDS at a stride of
-// n + FFT_PAD (already in bit-reversed order).  All threads of the block
-// take part.
-__device__ __forceinline__ void fft_lds(float2 *data, int count, int n, int logn,
-                                        const float2 *__restrict__ tw)
-{
-    const int quads = count * (n >> 2);
-    int s = 0;
-    for (; s + 1 < logn; s += 2) {
-        const int h = 1 << s;
-        const int sh1 = logn - 1 - s;  // k * n/(2h) = k << sh1
-        for (int q = threadIdx.x; q < quads; q += blockDim.x) {
-            const int seq = q >> (logn - 2);
-            const int g = q & ((n >> 2) - 1);
-            const int k = g & (h - 1);
-            const int i = ((g >> s) << (s + 2)) | k;
-            float2 *d = data + seq * (n + FFT_PAD);
-            const float2 w1 = tw[k << sh1];
-            const float2 w2 = tw[k << (sh1 - 1)];
-            const float2 x0 = d[i], x1 = d[i + h], x2 = d[i + 2 * h], x3 = d[i + 3 * h];
-            const float2 t1 = cmulf(w1, x1), t3 = cmulf(w1, x3);
-            const float2 y0 = make_float2(x0.x + t1.x, x0.y + t1.y);
-            const float2 y1 = make_float2(x0.x - t1.x, x0.y - t1.y);
-            const float2 y2 = make_float2(x2.x + t3.x, x2.y + t3.y);
-            const float2 y3 = make_float2(x2.x - t3.x, x2.y - t3.y);
-            const float2 u2 = cmulf(w2, y2);
-            const float2 w2y3 = cmulf(w2, y3);
-            const float2 u3 = make_float2(w2y3.y, -w2y3.x);  // (-i) w2 y3
-            d[i] = make_float2(y0.x + u2.x, y0.y + u2.y);
-            d[i + 2 * h] = make_float2(y0.x - u2.x, y0.y - u2.y);
-            d[i + h] = make_float2(y1.x + u3.x, y1.y + u3.y);
-            d[i + 3 * h] = make_float2(y1.x - u3.x, y1.y - u3.y);
-        }
-        __syncthreads();
-    }
-    if (s < logn) {  // last single stage: h = n/2, twiddle tw[i]
-        const int h = 1 << s;
-        const int pairs = count * h;
-        for (int p = threadIdx.x; p < pairs; p += blockDim.x) {
-            const int seq = p >> s;
-            const int i = p & (h - 1);
-            float2 *d = data + seq * (n + FFT_PAD);
-            const float2 t = cmulf(tw[i], d[i + h]);
-            const float2 a = d[i];
-            d[i] = make_float2(a.x + t.x, a.y + t.y);
-            d[i + h] = make_float2(a.x - t.x, a.y - t.y);
-        }
-        __syncthreads();
-    }
-}
-
-__device__ __forceinline__ int fft_bitrev(int x, int logn)
-{
-    return (int)(__builtin_bitreverse32((uint32_t)x) >> (32 - logn));
 }
 
 // Grid index of workgroup work: XCD-aware, so that consecutive work units
@@ -185,71 +126,168 @@ __device__ __forceinline__ int fft_logical_block()
     return (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
 }
 
-// Raster + row FFTs.  Work unit = (beam, ROWS rows); dynamic LDS:
-// ROWS * (n + FFT_PAD) float2 + n/2 float2 twiddles.  Thread j rasterises
-// pixel bitrev(j) into LDS slot j (linear, conflict-free stores).
-template <int ROWS>
-__global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
-    int n_beams, int n, int logn, const double *__restrict__ rec,
-    const float2 *__restrict__ gtw, float2 *__restrict__ grid)
+// LDS index of element i of a sequence: one pad slot per 32 elements
+// (breaks the power-of-two strides of the early Stockham stages).
+__device__ __forceinline__ int fft_pad(int i) { return i + (i >> 5); }
+
+// In-place forward DFT of R points (exp(-2 pi i j k / R)), R = 2, 4, 8.
+template <int R>
+__device__ __forceinline__ void fft_dft(float2 *v)
 {
-    extern __shared__ float2 lds[];
-    const int units_per_beam = n / ROWS;
-    const int unit = fft_logical_block();
-    if (unit >= n_beams * units_per_beam)
-        return;
-    const int beam = unit / units_per_beam;
-    const int row0 = (unit - beam * units_per_beam) * ROWS;
-    const int ld = n + FFT_PAD;
-    float2 *data = lds;
-    float2 *tw = lds + ROWS * ld;
-    fft_twiddles(tw, gtw, n);
-    const double *r = rec + (size_t)beam * FB_STRIDE;
-    const int ne = (int)r[FB_NE];
-    const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
-    for (int rr = 0; rr < ROWS; ++rr) {
-        const double py = fft_pixel(row0 + rr, n, dy, yc);
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
-            const bool in = fft_inside(r, ne, fft_pixel(fft_bitrev(j, logn), n, dx, xc), py);
-            data[rr * ld + j] = make_float2(in ? 1.0f : 0.0f, 0.0f);
+    if constexpr (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = make_float2(a.x + b.x, a.y + b.y);
+        v[1] = make_float2(a.x - b.x, a.y - b.y);
+    } else if constexpr (R == 4) {
+        const float2 a0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y);
+        const float2 a1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+        const float2 a2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y);
+        const float2 a3 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+        v[0] = make_float2(a0.x + a2.x, a0.y + a2.y);
+        v[2] = make_float2(a0.x - a2.x, a0.y - a2.y);
+        v[1] = make_float2(a1.x + a3.y, a1.y - a3.x);  // a1 - i a3
+        v[3] = make_float2(a1.x - a3.y, a1.y + a3.x);  // a1 + i a3
+    } else {
+        float2 e[4] = {v[0], v[2], v[4], v[6]};
+        float2 o[4] = {v[1], v[3], v[5], v[7]};
+        fft_dft<4>(e);
+        fft_dft<4>(o);
+        const float h = 0.70710678118654752f;
+        const float2 o1 = make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));   // w^1
+        const float2 o2 = make_float2(o[2].y, -o[2].x);                                // w^2 = -i
+        const float2 o3 = make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));  // w^3
+        v[0] = make_float2(e[0].x + o[0].x, e[0].y + o[0].y);
+        v[4] = make_float2(e[0].x - o[0].x, e[0].y - o[0].y);
+        v[1] = make_float2(e[1].x + o1.x, e[1].y + o1.y);
+        v[5] = make_float2(e[1].x - o1.x, e[1].y - o1.y);
+        v[2] = make_float2(e[2].x + o2.x, e[2].y + o2.y);
+        v[6] = make_float2(e[2].x - o2.x, e[2].y - o2.y);
+        v[3] = make_float2(e[3].x + o3.x, e[3].y + o3.y);
+        v[7] = make_float2(e[3].x - o3.x, e[3].y - o3.y);
+    }
+}
+
+// One Stockham stage of radix R over `cnt` length-n sequences; ns = product
+// of the radices of the earlier stages.  Butterfly idx covers sequence seq,
+// position j (seq fastest when COLFAST, else j fastest); it reads elements
+// j + r n/R (twiddled by exp(-2 pi i k r / (ns R)), k = j mod ns) and writes
+// (j / ns) ns R + k + r ns.  load(seq, i) / store(seq, i, v) give the
+// element access; all reads of the stage precede all its writes.
+template <int R, int PB, bool COLFAST, typename Load, typename Store>
+__device__ __forceinline__ void fft_stage(int n, int lg_ns, int cnt,
+                                          const float2 *__restrict__ tw, Load load,
+                                          Store store)
+{
+    const int nb = n / R;
+    const int total = cnt * nb;
+    const int ns = 1 << lg_ns;
+    const int tstep = n / (ns * R);
+    float2 v[PB][R];
+    for (int b = 0; b < PB; ++b) {
+        const int idx = threadIdx.x + b * blockDim.x;
+        if (idx >= total)
+            break;
+        const int seq = COLFAST ? idx % cnt : idx / nb;
+        const int j = COLFAST ? idx / cnt : idx - seq * nb;
+        const int k = j & (ns - 1);
+        for (int r = 0; r < R; ++r) {
+            float2 x = load(seq, j + r * nb);
+            if (r > 0 && k > 0)
+                x = cmulf(x, tw[k * r * tstep]);
+            v[b][r] = x;
         }
     }
     __syncthreads();
-    fft_lds(data, ROWS, n, logn, tw);
-    float2 *g = grid + ((size_t)beam * n + row0) * n;
-    for (int rr = 0; rr < ROWS; ++rr)
-        for (int x = threadIdx.x; x < n; x += blockDim.x)
-            g[(size_t)rr * n + x] = data[rr * ld + x];
+    for (int b = 0; b < PB; ++b) {
+        const int idx = threadIdx.x + b * blockDim.x;
+        if (idx >= total)
+            break;
+        const int seq = COLFAST ? idx % cnt : idx / nb;
+        const int j = COLFAST ? idx / cnt : idx - seq * nb;
+        const int k = j & (ns - 1);
+        fft_dft<R>(v[b]);
+        const int base = ((j >> lg_ns) << lg_ns) * R + k;
+        for (int r = 0; r < R; ++r)
+            store(seq, base + r * ns, v[b][r]);
+    }
+    __syncthreads();
 }
 
-// Column FFTs.  Work unit = (beam, COLS adjacent columns); dynamic LDS:
-// COLS * (n + FFT_PAD) float2 + n/2 float2 twiddles.  Lanes of a wave read
-// COLS columns of 64/COLS rows (COLS*8-byte row segments); row bitrev(j)
-// goes to slot j.
-template <int COLS>
-__global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
-    int n_beams, int n, int logn, const float2 *__restrict__ gtw, float2 *__restrict__ grid)
+// Stages S..end of a length 2^LOGN transform: radix 8 while at least 3 bits
+// remain, then one radix-4 or radix-2 stage.  Stage 0 reads through `first`,
+// the last stage writes through `last`, stages in between stay in LDS.
+template <int LOGN, int CNT, int T, bool COLFAST, int S, typename First, typename Last>
+__device__ __forceinline__ void fft_stages(float2 *buf, int ld, const float2 *__restrict__ tw,
+                                           First first, Last last)
 {
+    constexpr int N = 1 << LOGN;
+    constexpr int A = LOGN / 3, REM = LOGN % 3, NST = A + (REM ? 1 : 0);
+    if constexpr (S < NST) {
+        constexpr int R = S < A ? 8 : (1 << REM);
+        constexpr int PB = (CNT * (N / R) + T - 1) / T;
+        auto lds_load = [&](int seq, int i) { return buf[seq * ld + fft_pad(i)]; };
+        auto lds_store = [&](int seq, int i, float2 x) { buf[seq * ld + fft_pad(i)] = x; };
+        if constexpr (S == 0)
+            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, first, lds_store);
+        else if constexpr (S == NST - 1)
+            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, lds_load, last);
+        else
+            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, lds_load, lds_store);
+        fft_stages<LOGN, CNT, T, COLFAST, S + 1>(buf, ld, tw, first, last);
+    }
+}
+
+// LDS elements per sequence: padded length, plus 8 so that consecutive
+// sequences (the columns of the column pass) start 16 banks apart.
+__host__ __device__ constexpr int fft_ld(int n) { return n + (n >> 5) + 8; }
+
+// Raster + row FFTs.  Work unit = (beam, FFT_ROWS rows); LDS FFT_ROWS x
+// fft_ld(n) float2.  Row pixels are rasterised as stage 0 reads them.
+template <int LOGN>
+__global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
+    int n_beams, const double *__restrict__ rec, const float2 *__restrict__ tw,
+    float2 *__restrict__ grid)
+{
+    constexpr int N = 1 << LOGN;
     extern __shared__ float2 lds[];
-    const int units_per_beam = n / COLS;
+    const int units_per_beam = N / FFT_ROWS;
     const int unit = fft_logical_block();
     if (unit >= n_beams * units_per_beam)
         return;
     const int beam = unit / units_per_beam;
-    const int col0 = (unit - beam * units_per_beam) * COLS;
-    const int ld = n + FFT_PAD;
-    float2 *data = lds;
-    float2 *tw = lds + COLS * ld;
-    fft_twiddles(tw, gtw, n);
-    float2 *g = grid + (size_t)beam * n * n + col0;
-    const int c = threadIdx.x & (COLS - 1);
-    const int r_step = blockDim.x / COLS;
-    for (int j = threadIdx.x / COLS; j < n; j += r_step)
-        data[c * ld + j] = g[(size_t)fft_bitrev(j, logn) * n + c];
-    __syncthreads();
-    fft_lds(data, COLS, n, logn, tw);
-    for (int row = threadIdx.x / COLS; row < n; row += r_step)
-        g[(size_t)row * n + c] = data[c * ld + row];
+    const int row0 = (unit - beam * units_per_beam) * FFT_ROWS;
+    const double *r = rec + (size_t)beam * FB_STRIDE;
+    const int ne = (int)r[FB_NE];
+    const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
+    float2 *g = grid + ((size_t)beam * N + row0) * N;
+    auto raster = [&](int seq, int i) {
+        const bool in = fft_inside(r, ne, fft_pixel(i, N, dx, xc),
+                                   fft_pixel(row0 + seq, N, dy, yc));
+        return make_float2(in ? 1.0f : 0.0f, 0.0f);
+    };
+    auto out = [&](int seq, int i, float2 x) { g[(size_t)seq * N + i] = x; };
+    fft_stages<LOGN, FFT_ROWS, FFT_ROW_THREADS, false, 0>(lds, fft_ld(N), tw, raster, out);
+}
+
+// Column FFTs.  Work unit = (beam, FFT_COLS adjacent columns); LDS FFT_COLS x
+// fft_ld(n) float2.  Lanes take columns fastest, so a wave reads and writes
+// 64-byte row segments.
+template <int LOGN>
+__global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
+    int n_beams, const float2 *__restrict__ tw, float2 *__restrict__ grid)
+{
+    constexpr int N = 1 << LOGN;
+    extern __shared__ float2 lds[];
+    const int units_per_beam = N / FFT_COLS;
+    const int unit = fft_logical_block();
+    if (unit >= n_beams * units_per_beam)
+        return;
+    const int beam = unit / units_per_beam;
+    const int col0 = (unit - beam * units_per_beam) * FFT_COLS;
+    float2 *g = grid + (size_t)beam * N * N + col0;
+    auto in = [&](int seq, int i) { return g[(size_t)i * N + seq]; };
+    auto out = [&](int seq, int i, float2 x) { g[(size_t)i * N + seq] = x; };
+    fft_stages<LOGN, FFT_COLS, FFT_COL_THREADS, true, 0>(lds, fft_ld(N), tw, in, out);
 }
 
 __device__ __forceinline__ double lanczos(double x, int radius)

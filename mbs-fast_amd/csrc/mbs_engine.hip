@@ -747,38 +747,31 @@ int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out)
 
 }  // extern "C"
 
-static int fft_lds_limit(mbs_ctx *ctx, int *bytes)
+template <int LOGN>
+static int launch_fft(mbs_ctx *ctx, int nb)
 {
-    HIP_TRY(ctx, hipDeviceGetAttribute(bytes, hipDeviceAttributeMaxSharedMemoryPerBlock,
-                                       ctx->device));
-    return MBS_OK;
-}
-
-template <int R>
-static int launch_fft_rows(mbs_ctx *ctx, int nb, int n, int logn)
-{
-    const size_t lds = (size_t)(R * (n + FFT_PAD) + n / 2) * sizeof(float2);
-    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<R>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int units = nb * (n / R);
-    const int blocks = (units + 7) / 8 * 8;
-    fft_rows_kernel<R><<<blocks, FFT_ROW_THREADS, lds, ctx->stream>>>(
-        nb, n, logn, ctx->fft_rec.p, ctx->fft_tw.p, ctx->fft_grid.p);
+    constexpr int N = 1 << LOGN;
+    const size_t lds_rows = (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2);
+    const size_t lds_cols = (size_t)FFT_COLS * fft_ld(N) * sizeof(float2);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<LOGN>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<LOGN>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
+    hipStream_t s = ctx->stream;
+    const int rb = (nb * (N / FFT_ROWS) + 7) / 8 * 8;
+    const int cb = (nb * (N / FFT_COLS) + 7) / 8 * 8;
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[0], s));
+    fft_rows_kernel<LOGN><<<rb, FFT_ROW_THREADS, lds_rows, s>>>(nb, ctx->fft_rec.p, ctx->fft_tw.p,
+                                                               ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
-    return MBS_OK;
-}
-
-template <int C>
-static int launch_fft_cols(mbs_ctx *ctx, int nb, int n, int logn)
-{
-    const size_t lds = (size_t)(C * (n + FFT_PAD) + n / 2) * sizeof(float2);
-    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<C>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int units = nb * (n / C);
-    const int blocks = (units + 7) / 8 * 8;
-    fft_cols_kernel<C><<<blocks, FFT_COL_THREADS, lds, ctx->stream>>>(nb, n, logn, ctx->fft_tw.p,
-                                                                    ctx->fft_grid.p);
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
+    fft_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(nb, ctx->fft_tw.p,
+                                                               ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
+    if (ctx->timing)
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[2], s));
     return MBS_OK;
 }
 
@@ -930,10 +923,11 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
                         (long long)beams[i]);
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int limit = 0;
-    int rc = fft_lds_limit(ctx, &limit);
-    if (rc)
-        return rc;
-    auto lds_for = [n](int r) { return (size_t)(r * (n + FFT_PAD) + n / 2) * sizeof(float2); };
+    HIP_TRY(ctx, hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                       ctx->device));
+    if ((size_t)limit < (size_t)FFT_COLS * fft_ld(n) * sizeof(float2))
+        return fail(ctx, MBS_ERR_CAPACITY, "FFT size %d needs %zu bytes of LDS per workgroup "
+                    "(device limit %d)", n, (size_t)FFT_COLS * fft_ld(n) * sizeof(float2), limit);
     int logn = 0;
     while ((1 << logn) < n)
         ++logn;
@@ -943,8 +937,8 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(ctx->fft_beams.p, beams, sizeof(int64_t) * n_beams,
                                 hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, ctx->fft_tw.ensure(n / 2));
-    fft_twiddle_kernel<<<(n / 2 + 255) / 256, 256, 0, s>>>(n, ctx->fft_tw.p);
+    HIP_TRY(ctx, ctx->fft_tw.ensure(n));
+    fft_twiddle_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, ctx->fft_tw.p);
     fft_setup_kernel<<<(n_beams + 63) / 64, 64, 0, s>>>(n_beams, ctx->fft_beams.p, ctx->nv.p,
                                                        ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
                                                        n, pad, ctx->fft_rec.p);
@@ -953,28 +947,18 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
         for (hipEvent_t &e : ctx->fft_ev)
             if (!e)
                 HIP_TRY(ctx, hipEventCreate(&e));
-    if (ctx->timing)
-        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[0], s));
-    if ((size_t)limit >= lds_for(4))
-        rc = launch_fft_rows<4>(ctx, n_beams, n, logn);
-    else if ((size_t)limit >= lds_for(2))
-        rc = launch_fft_rows<2>(ctx, n_beams, n, logn);
-    else
-        rc = launch_fft_rows<1>(ctx, n_beams, n, logn);
+    int rc = MBS_OK;
+    switch (logn) {
+    case 6: rc = launch_fft<6>(ctx, n_beams); break;
+    case 7: rc = launch_fft<7>(ctx, n_beams); break;
+    case 8: rc = launch_fft<8>(ctx, n_beams); break;
+    case 9: rc = launch_fft<9>(ctx, n_beams); break;
+    case 10: rc = launch_fft<10>(ctx, n_beams); break;
+    case 11: rc = launch_fft<11>(ctx, n_beams); break;
+    default: return fail(ctx, MBS_ERR_INVALID, "FFT size %d", n);
+    }
     if (rc)
         return rc;
-    if (ctx->timing)
-        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
-    if ((size_t)limit >= lds_for(8))
-        rc = launch_fft_cols<8>(ctx, n_beams, n, logn);
-    else if ((size_t)limit >= lds_for(4))
-        rc = launch_fft_cols<4>(ctx, n_beams, n, logn);
-    else
-        rc = launch_fft_cols<2>(ctx, n_beams, n, logn);
-    if (rc)
-        return rc;
-    if (ctx->timing)
-        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[2], s));
     ctx->fft_n = n;
     ctx->fft_batch = n_beams;
     if (n_points > 0) {

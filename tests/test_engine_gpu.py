@@ -310,7 +310,7 @@ def test_orientation_sets_on_device(pkg, oracle, ctx, kind):
         ctx.orientation_set(7, 10)
 
 
-@pytest.mark.parametrize("n", [256, 2048])
+@pytest.mark.parametrize("n", [64, 256, 1024, 2048])
 def test_aperture_fft_on_device(pkg, oracle, ctx, n):
     """Device aperture FFT (FP32) of real beam apertures against the oracle's
     FP64 FFT of the identical raster: spectra to FP32 accuracy, sampled
@@ -346,6 +346,8 @@ def test_aperture_fft_on_device(pkg, oracle, ctx, n):
         assert err < 1e-5
     with pytest.raises(pkg.engine.EngineError):
         ctx.aperture_fft(beams[:1], 1000)
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.aperture_fft(beams[:1], 4096)
     with pytest.raises(pkg.engine.EngineError):
         ctx.aperture_fft(np.zeros(65, np.int64), 256)
     with pytest.raises(pkg.engine.EngineError):
