@@ -224,7 +224,8 @@ def bench_aperture_fft(ctx, batch, args):
             "cols_GBps": 2 * grid_bytes / (c * 1e-3) / 1e9,
             "roofline": {"bound": "hbm", "achieved": 3 * grid_bytes / ((r + c) * 1e-3) / 1e9,
                          "peak": 8000.0, "unit": "GB/s",
-                         "frac": 3 * grid_bytes / ((r + c) * 1e-3) / 8e12, "traffic": None}}
+                         "frac": 3 * grid_bytes / ((r + c) * 1e-3) / 8e12,
+                         "traffic": measured_traffic().get("fft")}}
 
 
 def run(args):

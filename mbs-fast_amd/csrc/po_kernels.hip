@@ -454,7 +454,9 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     const double cc = __builtin_fma(slope, mul, add);
     const bool valid = (f & vmask) != 0;
     const bool big = __builtin_fabs(cc) > eps1;
-    const float r = rcp_f32((float)cc);
+    // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
+    // carry more rounding than that
+    const float r = __builtin_amdgcn_rcpf((float)cc);
     const float inv = (valid && big) ? r : 0.0f;
     sr = __builtin_fmaf(cn - cp, inv, sr);
     si = __builtin_fmaf(sn - sp, inv, si);
