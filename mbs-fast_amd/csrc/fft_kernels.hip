@@ -36,9 +36,18 @@
 namespace mbs {
 
 // Workgroups: the row kernel holds FFT_ROWS rows (2 workgroups of 68 KB LDS
-// per CU at n = 2048) with 512 threads, the column kernel FFT_COLS columns
-// (one workgroup of 135 KB) with 1024 threads: 4 waves per SIMD either way.
-enum { FFT_ROW_THREADS = 512, FFT_COL_THREADS = 1024, FFT_ROWS = 4, FFT_COLS = 8 };
+// per CU at n = 2048) with 512 threads; the column kernel FFT_COLS columns
+// with FFT_COL_THREADS threads (8 columns: one 135 KB workgroup of 1024
+// threads per CU; 4 columns: two 68 KB workgroups of 512).
+#ifndef MBS_FFT_COLS
+#define MBS_FFT_COLS 8
+#endif
+enum {
+    FFT_ROW_THREADS = 512,
+    FFT_ROWS = 4,
+    FFT_COLS = MBS_FFT_COLS,
+    FFT_COL_THREADS = MBS_FFT_COLS * 128
+};
 enum { FB_XC = 0, FB_YC = 1, FB_DX = 2, FB_DY = 3, FB_NE = 4, FB_EDGE = 5, FB_STRIDE = 5 + 3 * 32 };
 
 // Per-beam raster set-up: bounding box, pixel size and the edge lines
@@ -87,17 +96,6 @@ __global__ void fft_setup_kernel(int n_beams, const int64_t *__restrict__ beams,
 __device__ __forceinline__ double fft_pixel(int j, int n, double d, double c)
 {
     return __builtin_fma((double)(2 * j + 1 - n), 0.5 * d, c);
-}
-
-__device__ __forceinline__ bool fft_inside(const double *__restrict__ r, int ne, double px,
-                                           double py)
-{
-    bool in = true;
-    for (int e = 0; e < ne; ++e) {
-        const double *l = r + FB_EDGE + 3 * e;
-        in = in && (__builtin_fma(l[0], px, __builtin_fma(l[1], py, l[2])) > 0.0);
-    }
-    return in;
 }
 
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b)
@@ -242,7 +240,8 @@ __device__ __forceinline__ void fft_stages(float2 *buf, int ld, const float2 *__
 __host__ __device__ constexpr int fft_ld(int n) { return n + (n >> 5) + 8; }
 
 // Raster + row FFTs.  Work unit = (beam, FFT_ROWS rows); LDS FFT_ROWS x
-// fft_ld(n) float2.  Row pixels are rasterised as stage 0 reads them.
+// fft_ld(n) float2 plus FFT_ROWS x 32 doubles of per-row edge offsets.  Row
+// pixels are rasterised as stage 0 reads them.
 template <int LOGN>
 __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
     int n_beams, const double *__restrict__ rec, const float2 *__restrict__ tw,
@@ -260,9 +259,21 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
     const int ne = (int)r[FB_NE];
     const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
     float2 *g = grid + ((size_t)beam * N + row0) * N;
+    // per row and edge, the y part b*py + c of the edge line (the same FMA
+    // the pixel test would do per pixel), after the sequences in LDS
+    double *off = reinterpret_cast<double *>(lds + FFT_ROWS * fft_ld(N));
+    for (int q = threadIdx.x; q < FFT_ROWS * ne; q += blockDim.x) {
+        const int rr = q / ne, e = q - rr * ne;
+        const double *l = r + FB_EDGE + 3 * e;
+        off[q] = __builtin_fma(l[1], fft_pixel(row0 + rr, N, dy, yc), l[2]);
+    }
+    __syncthreads();
     auto raster = [&](int seq, int i) {
-        const bool in = fft_inside(r, ne, fft_pixel(i, N, dx, xc),
-                                   fft_pixel(row0 + seq, N, dy, yc));
+        const double px = fft_pixel(i, N, dx, xc);
+        const double *o = off + seq * ne;
+        bool in = true;
+        for (int e = 0; e < ne; ++e)
+            in = in && (__builtin_fma(r[FB_EDGE + 3 * e], px, o[e]) > 0.0);
         return make_float2(in ? 1.0f : 0.0f, 0.0f);
     };
     auto out = [&](int seq, int i, float2 x) { g[(size_t)seq * N + i] = x; };

@@ -751,7 +751,8 @@ template <int LOGN>
 static int launch_fft(mbs_ctx *ctx, int nb)
 {
     constexpr int N = 1 << LOGN;
-    const size_t lds_rows = (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2);
+    const size_t lds_rows = (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2) +
+                            (size_t)FFT_ROWS * MBS_MAX_VERTICES * sizeof(double);
     const size_t lds_cols = (size_t)FFT_COLS * fft_ld(N) * sizeof(float2);
     HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<LOGN>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
