@@ -645,8 +645,6 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     // switch to the exact half-angle edge form happens at |C| <= 3e-5, the
     // floor the reference applies to its FP32 build (GpuDiffraction.cu:498-512).
     a.eps1 = prec ? std::max(ctx->consts.eps1, 3e-5) : ctx->consts.eps1;
-    a.cw_re = ctx->consts.compl_wave[0];
-    a.cw_im = ctx->consts.compl_wave[1];
     a.out = ctx->part.p;
     a.out_ns = ns ? ctx->part_ns.p : nullptr;
     a.per_orient = (per && !jones) ? ctx->per_orient.p : nullptr;

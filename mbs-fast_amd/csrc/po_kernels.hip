@@ -217,18 +217,13 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
         w[n] = (float)re;
         w[32 + n] = (float)im;
     }
-    // FP32 copy of what the mixed-precision kernel uses in FP32
+    // FP32 copy of the rotation data the vector-ALU mixed kernel uses
     for (int q = 0; q < 6; ++q) {
         h32[H32_T0 + q] = (float)h[H_T0 + q];
         h32[H32_T1 + q] = (float)h[H_T1 + q];
     }
     for (int q = 0; q < 8; ++q)
         h32[H32_JP + q] = (float)h[H_JP + q];
-    h32[H32_AL + 0] = (float)h[H_AL + 0];
-    h32[H32_AL + 1] = (float)h[H_AL + 1];
-    for (int q = 0; q < 5; ++q)
-        h32[H32_MOM + q] = (float)h[H_MOM + q];
-    h32[H32_PAD] = 0.0f;
     for (int i = 0; i < nv; ++i) {
         double *v = vtx + (size_t)(v0 + i) * VTX_STRIDE;
         const uint8_t f = ef[v0 + i];
@@ -620,7 +615,7 @@ struct KernelArgs {
     const int32_t *ext;      // [n_beams]
     const int64_t *ooff;     // [n_orient+1]
     const double *weights;   // [n_orient]
-    double k, eps1, cw_re, cw_im;
+    double k, eps1;
     double *out;             // Mueller mode: [n_items][cells][16]; Jones: [n_items][cells][8]
     double *out_ns;          // same shape, no-shadow (NS only)
     double *per_orient;      // [n_orient][cells][16] (PER only, Mueller mode)

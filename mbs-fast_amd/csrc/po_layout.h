@@ -22,16 +22,9 @@ enum {
     HDR_STRIDE = 40
 };
 
-// FP32 copy of the header fields used in FP32 by the mixed kernel.
-enum {
-    H32_T0 = 0,
-    H32_T1 = 6,
-    H32_JP = 12,
-    H32_AL = 20,
-    H32_MOM = 22,
-    H32_PAD = 27,
-    HDR32_STRIDE = 28
-};
+// FP32 copy of the rotation data (T0, T1, complWave * J_phased) used by the
+// vector-ALU form of the mixed kernel.
+enum { H32_T0 = 0, H32_T1 = 6, H32_JP = 12, HDR32_STRIDE = 20 };
 
 // MFMA B operand of one beam (mixed kernel), WB_STRIDE floats in lane order:
 // lane l holds B[l/32][l%32] of the complex contraction

@@ -1,20 +1,12 @@
 // po_math.h -- device math for the PO diffraction kernels (gfx950).
 //
-// sincos is evaluated here rather than through the device library so that the
-// hot loop costs ~30 VALU instructions per phasor and its accuracy is known:
-//   * reduction x = n*(pi/2) + r with n = rint(x*2/pi) and a two-constant
-//     FMA Cody-Waite step.  For |n| < 2^20 the first step fma(-n, P1, x) is
-//     exact (both x and n*P1 are multiples of 2^-52 or finer and the
-//     difference is below 1), the second adds one rounding;
-//   * Taylor polynomials of sin (to r^15) and cos (to r^16) on
-//     |r| <= pi/4 (+rounding slack), truncation below 7e-17 -- coefficients
-//     are exactly (-1)^j/(2j+1)! and (-1)^j/(2j)!, nothing else;
-//   * quadrant from n mod 4.
-// Phases reach ~1e4 rad for the largest particles, far inside |n| < 2^20.
-//
+// Phasors e^{i x} are evaluated here rather than through the device library
+// so that the hot loop's cost and accuracy are known: a table-driven
+// reduction against a 256-entry LDS table of e^{2 pi i j / 256} and short
+// Taylor polynomials (see sincos_table below).  Phases reach ~1e4 rad for the
+// largest particles; the reduction is exact to an ulp for |x| < 2^20.
 // The FP32 variant used by the mixed-precision kernels reduces in FP64 the
-// same way and evaluates shorter Taylor polynomials (sin to r^9, cos to r^10)
-// in FP32 -- error ~1 ulp of FP32.
+// same way and evaluates the polynomials in FP32 (error ~1 ulp of FP32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,38 +18,6 @@ constexpr double kPi = 3.141592653589793116;            // double(pi) = M_PI
 // Small-phase switch: every vertex phase within 1e-3 rad
 // (HandlerPO_fast.h:161-216).
 constexpr double kSmallPhase = 1e-3;
-constexpr double kTwoOverPi = 0.63661977236758134308;
-constexpr double kPio2Hi = 1.5707963267948966192;     // double(pi/2)
-constexpr double kPio2Lo = 6.123233995736766036e-17;  // pi/2 - kPio2Hi
-
-// Taylor coefficients (-1)^j / (2j+1)!  and  (-1)^j / (2j)!
-constexpr double kS1 = -1.0 / 6.0;
-constexpr double kS2 = 1.0 / 120.0;
-constexpr double kS3 = -1.0 / 5040.0;
-constexpr double kS4 = 1.0 / 362880.0;
-constexpr double kS5 = -1.0 / 39916800.0;
-constexpr double kS6 = 1.0 / 6227020800.0;
-constexpr double kS7 = -1.0 / 1307674368000.0;
-constexpr double kC1 = -1.0 / 2.0;
-constexpr double kC2 = 1.0 / 24.0;
-constexpr double kC3 = -1.0 / 720.0;
-constexpr double kC4 = 1.0 / 40320.0;
-constexpr double kC5 = -1.0 / 3628800.0;
-constexpr double kC6 = 1.0 / 479001600.0;
-constexpr double kC7 = -1.0 / 87178291200.0;
-constexpr double kC8 = 1.0 / 20922789888000.0;
-
-__device__ __forceinline__ void quadrant_fix(int q, double ps, double pc,
-                                             double &s, double &c)
-{
-    // sin(r + q*pi/2), cos(r + q*pi/2) from sin r, cos r
-    const bool swap = q & 1;
-    double ss = swap ? pc : ps;
-    double cc = swap ? ps : pc;
-    s = (q & 2) ? -ss : ss;
-    c = ((q + 1) & 2) ? -cc : cc;
-}
-
 // One Horner step p*z + c as a single three-operand v_fma_f64 with the
 // coefficient in an SGPR pair.  Written as inline assembly because the
 // compiler otherwise keeps loop-invariant FP64 coefficients in VGPRs and
@@ -67,30 +27,6 @@ __device__ __forceinline__ double horner(double p, double z, double c)
     double r;
     asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(p), "v"(z), "s"(c));
     return r;
-}
-
-__device__ __forceinline__ void sincos_fast(double x, double &s, double &c)
-{
-    const double n = __builtin_rint(x * kTwoOverPi);
-    double r = __builtin_fma(-n, kPio2Hi, x);
-    r = __builtin_fma(-n, kPio2Lo, r);
-    const double z = r * r;
-    double ps = __builtin_fma(z, kS7, kS6);
-    ps = horner(ps, z, kS5);
-    ps = horner(ps, z, kS4);
-    ps = horner(ps, z, kS3);
-    ps = horner(ps, z, kS2);
-    ps = horner(ps, z, kS1);
-    ps = __builtin_fma(r * z, ps, r);
-    double pc = __builtin_fma(z, kC8, kC7);
-    pc = horner(pc, z, kC6);
-    pc = horner(pc, z, kC5);
-    pc = horner(pc, z, kC4);
-    pc = horner(pc, z, kC3);
-    pc = horner(pc, z, kC2);
-    pc = __builtin_fma(pc, z, kC1);
-    pc = __builtin_fma(pc, z, 1.0);
-    quadrant_fix((int)n, ps, pc, s, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -139,31 +75,6 @@ __device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restr
     const float cd = __builtin_fmaf(z, __builtin_fmaf(z, 1.0f / 24.0f, -0.5f), 1.0f);
     s = __builtin_fmaf(e.y, cd, e.x * sd);
     c = __builtin_fmaf(e.x, cd, -(e.y * sd));
-}
-
-// FP64 argument, FP32 result: FP64 reduction, FP32 Taylor polynomials.
-__device__ __forceinline__ void sincos_mixed(double x, float &s, float &c)
-{
-    const double n = __builtin_rint(x * kTwoOverPi);
-    double rd = __builtin_fma(-n, kPio2Hi, x);
-    rd = __builtin_fma(-n, kPio2Lo, rd);
-    const float r = (float)rd;
-    const float z = r * r;
-    float ps = __builtin_fmaf(z, (float)(1.0 / 362880.0), (float)(-1.0 / 5040.0));
-    ps = __builtin_fmaf(z, ps, (float)(1.0 / 120.0));
-    ps = __builtin_fmaf(z, ps, (float)(-1.0 / 6.0));
-    ps = __builtin_fmaf(r * z, ps, r);
-    float pc = __builtin_fmaf(z, (float)(-1.0 / 3628800.0), (float)(1.0 / 40320.0));
-    pc = __builtin_fmaf(z, pc, (float)(-1.0 / 720.0));
-    pc = __builtin_fmaf(z, pc, (float)(1.0 / 24.0));
-    pc = __builtin_fmaf(z, pc, -0.5f);
-    pc = __builtin_fmaf(z, pc, 1.0f);
-    const int q = (int)n;
-    const bool swap = q & 1;
-    const float ss = swap ? pc : ps;
-    const float cc = swap ? ps : pc;
-    s = (q & 2) ? -ss : ss;
-    c = ((q + 1) & 2) ? -cc : cc;
 }
 
 // 1/x to full FP64 precision: hardware estimate + two Newton steps.
