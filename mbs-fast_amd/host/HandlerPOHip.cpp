@@ -31,14 +31,17 @@
 // FP32-storage switch, src/cuda/GpuDiffraction.cu:15-41) it runs the mixed
 // kernel (FP32 storage, FP64 phases and moments).  MBS_HIP_PRECISION=fp64 or
 // =fp32 overrides the build default at run time.
+// Multi-size (rescaled beam sets): the scale / waveIndex arguments and the
+// fused multi-scale member, with the reference's scale-on-pack rule.
 // Not provided (return false, as in a reference build without these paths):
-// the phi-FFT, multi-size and theta-probe variants.
+// the phi-FFT and theta-probe variants.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
 #include "mbs_hip.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -130,48 +133,142 @@ struct PackedBatch
     }
 };
 
+// A rescaled beam set (the multi-size path): the particle, and with it every
+// beam, is scaled by `scale` about the origin.  As the reference's pack does
+// (src/cuda/GpuDiffraction.cu:273-358, applied when |scale - 1| > 1e-15,
+// :4324), aperture vertices and the beam centre are multiplied by the scale,
+// the area by its square, and J_phased is rebuilt from the beam's raw Jones
+// matrix: J_phased = sign * a(scale) * J * exp(i waveIndex * path * scale),
+// path = the optical path (external beams) or the projected length
+// (internal beams), sign = -1 for external beams and for internal beams with
+// an odd number of interactions, a(scale) the mean over the beam's
+// absorption paths p of exp(cAbs * p * scale) (1 for p <= DBL_EPSILON).
+// Slopes, axes and polarisation data do not change with size.
+struct PackScale
+{
+    bool on = false;
+    double scale = 1.0, waveIndex = 0.0, cAbs = 0.0;
+
+    PackScale() = default;
+    PackScale(double s, double k, double c)
+        : on(std::fabs(s - 1.0) > 1e-15), scale(s), waveIndex(k), cAbs(c) {}
+};
+
+double AbsorptionFactor(const PreparedBeam &pb, double scale, double cAbs)
+{
+    if (pb.absorptionPaths.empty())
+        return 1.0;
+    double sum = 0.0;
+    for (double path : pb.absorptionPaths)
+        sum += path > DBL_EPSILON ? std::exp(cAbs * path * scale) : 1.0;
+    return sum / (double)pb.absorptionPaths.size();
+}
+
+void ScaledJones(const PreparedBeam &pb, const PackScale &sc, double jp[8])
+{
+    const double path = (pb.isExternal ? pb.origBeam.opticalPath : pb.info.projLenght)
+                        * sc.scale;
+    double sign = pb.isExternal ? -1.0 : 1.0;
+    if (!pb.isExternal && (pb.origBeam.nActs & 1))
+        sign = -sign;
+    const double amp = sign * AbsorptionFactor(pb, sc.scale, sc.cAbs);
+    const double sn = std::sin(sc.waveIndex * path), cs = std::cos(sc.waveIndex * path);
+    const ::complex raw[4] = {pb.origBeam.J.m11, pb.origBeam.J.m12, pb.origBeam.J.m21,
+                              pb.origBeam.J.m22};
+    for (int k = 0; k < 4; ++k)
+    {
+        const double jr = real(raw[k]) * amp, ji = imag(raw[k]) * amp;
+        jp[2 * k] = jr * cs - ji * sn;
+        jp[2 * k + 1] = jr * sn + ji * cs;
+    }
+}
+
 // The job of pack_prepared_gpu_beam (src/cuda/GpuDiffraction.cu:273-358):
 // PreparedBeam (src/handler/HandlerPO.h:14-46) -> structure of arrays.
+// Two passes over the orientations, both OpenMP-parallel as the reference's
+// packing is (:4407-4428): beam and vertex counts (which also validate every
+// beam), then, after a prefix sum, every orientation fills its own slice.
 // Returns false for a beam the device path cannot take.
 bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
-          PackedBatch &pk)
+          const PackScale &sc, PackedBatch &pk)
 {
-    pk.orientOffsets.assign(1, 0);
-    pk.vertOffsets.assign(1, 0);
-    for (int o = start; o < start + nOrient; ++o)
+    std::vector<int64_t> nb(nOrient), nvert(nOrient);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
+    for (int i = 0; i < nOrient; ++i)
     {
-        const PreparedOrientation &po = prepared[o];
-        pk.weights.push_back(po.sinZenith);
-        for (const PreparedBeam &pb : po.beams)
+        int64_t v = 0;
+        for (const PreparedBeam &pb : prepared[start + i].beams)
         {
             const BeamEdgeData &ed = pb.edgeData;
             if (!ed.valid || ed.nVertices < MBS_MIN_VERTICES
                 || ed.nVertices > MBS_MAX_VERTICES)
-                return false;
+                bad = 1;
+            v += ed.nVertices;
+        }
+        nb[i] = (int64_t)prepared[start + i].beams.size();
+        nvert[i] = v;
+    }
+    if (bad)
+        return false;
+    pk.orientOffsets.assign(nOrient + 1, 0);
+    std::vector<int64_t> vbase(nOrient + 1, 0);
+    for (int i = 0; i < nOrient; ++i)
+    {
+        pk.orientOffsets[i + 1] = pk.orientOffsets[i] + nb[i];
+        vbase[i + 1] = vbase[i] + nvert[i];
+    }
+    const size_t nBeams = (size_t)pk.orientOffsets[nOrient];
+    const size_t nVerts = (size_t)vbase[nOrient];
+    pk.weights.resize(nOrient);
+    pk.scalars.resize(nBeams * MBS_BEAM_SCALARS);
+    pk.nv.resize(nBeams);
+    pk.external.resize(nBeams);
+    pk.vertOffsets.resize(nBeams + 1);
+    pk.vx.resize(nVerts);
+    pk.vy.resize(nVerts);
+    pk.syx.resize(nVerts);
+    pk.sxy.resize(nVerts);
+    pk.flags.resize(nVerts);
+    pk.vertOffsets[nBeams] = (int64_t)nVerts;
+    const double g = sc.on ? sc.scale : 1.0, g2 = g * g;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < nOrient; ++i)
+    {
+        const PreparedOrientation &po = prepared[start + i];
+        pk.weights[i] = po.sinZenith;
+        size_t bi = (size_t)pk.orientOffsets[i];
+        size_t vi = (size_t)vbase[i];
+        for (const PreparedBeam &pb : po.beams)
+        {
+            const BeamEdgeData &ed = pb.edgeData;
+            double jp[8] = {pb.jp00r, pb.jp00i, pb.jp01r, pb.jp01i,
+                            pb.jp10r, pb.jp10i, pb.jp11r, pb.jp11i};
+            if (sc.on)
+                ScaledJones(pb, sc, jp);
             const double s[MBS_BEAM_SCALARS] = {
                 pb.horAx, pb.horAy, pb.horAz, pb.verAx, pb.verAy, pb.verAz,
-                pb.bdx, pb.bdy, pb.bdz, pb.cenx, pb.ceny, pb.cenz,
-                pb.beam_area,
+                pb.bdx, pb.bdy, pb.bdz, pb.cenx * g, pb.ceny * g, pb.cenz * g,
+                pb.beam_area * g2,
                 pb.pNTx, pb.pNTy, pb.pNTz, pb.pNPx, pb.pNPy, pb.pNPz,
                 pb.pnxDTx, pb.pnxDTy, pb.pnxDTz, pb.pnxDPx, pb.pnxDPy, pb.pnxDPz,
-                pb.jp00r, pb.jp00i, pb.jp01r, pb.jp01i,
-                pb.jp10r, pb.jp10i, pb.jp11r, pb.jp11i,
+                jp[0], jp[1], jp[2], jp[3], jp[4], jp[5], jp[6], jp[7],
                 pb.normx, pb.normy, pb.normz};
-            pk.scalars.insert(pk.scalars.end(), s, s + MBS_BEAM_SCALARS);
-            pk.nv.push_back(ed.nVertices);
-            pk.external.push_back(pb.isExternal ? 1 : 0);
-            for (int v = 0; v < ed.nVertices; ++v)
+            std::memcpy(&pk.scalars[bi * MBS_BEAM_SCALARS], s, sizeof s);
+            pk.nv[bi] = ed.nVertices;
+            pk.external[bi] = pb.isExternal ? 1 : 0;
+            pk.vertOffsets[bi] = (int64_t)vi;
+            for (int v = 0; v < ed.nVertices; ++v, ++vi)
             {
-                pk.vx.push_back(ed.x[v]);
-                pk.vy.push_back(ed.y[v]);
-                pk.syx.push_back(ed.edge_valid_x[v] ? ed.slope_yx[v] : 0.0);
-                pk.sxy.push_back(ed.edge_valid_y[v] ? ed.slope_xy[v] : 0.0);
-                pk.flags.push_back((uint8_t)((ed.edge_valid_x[v] ? MBS_EDGE_VALID_X : 0)
-                                             | (ed.edge_valid_y[v] ? MBS_EDGE_VALID_Y : 0)));
+                pk.vx[vi] = ed.x[v] * g;
+                pk.vy[vi] = ed.y[v] * g;
+                pk.syx[vi] = ed.edge_valid_x[v] ? ed.slope_yx[v] : 0.0;
+                pk.sxy[vi] = ed.edge_valid_y[v] ? ed.slope_xy[v] : 0.0;
+                pk.flags[vi] = (uint8_t)((ed.edge_valid_x[v] ? MBS_EDGE_VALID_X : 0)
+                                         | (ed.edge_valid_y[v] ? MBS_EDGE_VALID_Y : 0));
             }
-            pk.vertOffsets.push_back((int64_t)pk.vx.size());
+            ++bi;
         }
-        pk.orientOffsets.push_back((int64_t)pk.nv.size());
     }
     return true;
 }
@@ -238,92 +335,91 @@ std::string FormatGpuInfo(const GpuDeviceInfo &info)
 }
 
 // ---- HandlerPO GPU members (src/handler/HandlerPO.h:98-146) -----------------
-bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
-    const std::vector<PreparedOrientation> &prepared, int start, int count, int device,
-    Arr2D &localM, Arr2D &localM_noshadow, double scale, double waveIndex,
-    unsigned long long /*cacheToken*/)
+namespace
 {
-    if (!isCoh)
-        return false;
-    // Rescaled beam sets belong to the multi-size path, not provided here.
-    if (scale != 1.0 || (waveIndex != 0.0 && waveIndex != m_waveIndex))
-        return false;
-    if (start < 0 || count < 0 || start > (int)prepared.size())
-        return false;
-    const int nOrient = std::min(count, (int)prepared.size() - start);
-    const int nAz = m_sphere.nAzimuth;
-    const int nZen = m_sphere.nZenith;
-    if (nOrient == 0 || nAz * (nZen + 1) == 0)
-        return true;
+// Grid of ScatteringRange (src/handler/Handler.h:15-189): zenith values,
+// azimuth step and the transverse basis vf.
+struct GridSpec
+{
+    int nAz = 0, nZen = 0;
+    double azStep = 0.0;
+    std::vector<double> theta, vf;
+};
 
-    PackedBatch pk;
-    if (!Pack(prepared, start, nOrient, pk))
-        return false;
-    if (pk.nv.empty())
-        return true;
+GridSpec MakeGrid(const ScatteringRange &sphere)
+{
+    GridSpec g;
+    g.nAz = sphere.nAzimuth;
+    g.nZen = sphere.nZenith;
+    g.azStep = sphere.azinuthStep;
+    g.theta.resize(g.nZen + 1);
+    g.vf.resize((size_t)g.nAz * (g.nZen + 1) * 3);
+    for (int t = 0; t <= g.nZen; ++t)
+        g.theta[t] = sphere.GetZenith(t);
+    for (int p = 0; p < g.nAz; ++p)
+        for (int t = 0; t <= g.nZen; ++t)
+        {
+            double *w = &g.vf[((size_t)p * (g.nZen + 1) + t) * 3];
+            w[0] = sphere.vf[p][t].x;
+            w[1] = sphere.vf[p][t].y;
+            w[2] = sphere.vf[p][t].z;
+        }
+    return g;
+}
 
+// This thread's context on `device`, created on first use, with the grid
+// uploaded when it changed.  Returns null after a failure (the caller
+// returns `ok`).
+ThreadContexts::Entry *Context(int device, GridSpec &g, bool &ok)
+{
     ThreadContexts::Entry &e = t_contexts.byDevice[device];
     if (!e.ctx && mbs_ctx_create(device, &e.ctx) != MBS_OK)
     {
         e.ctx = nullptr;
-        return DeviceFailure("mbs_ctx_create", mbs_last_error(nullptr));
+        ok = DeviceFailure("mbs_ctx_create", mbs_last_error(nullptr));
+        return nullptr;
     }
-    // Grid: zenith values, azimuth step and the transverse basis vf of
-    // ScatteringRange (src/handler/Handler.h:15-189); uploaded when changed.
-    std::vector<double> theta(nZen + 1), vf((size_t)nAz * (nZen + 1) * 3);
-    for (int t = 0; t <= nZen; ++t)
-        theta[t] = m_sphere.GetZenith(t);
-    for (int p = 0; p < nAz; ++p)
-        for (int t = 0; t <= nZen; ++t)
-        {
-            double *w = &vf[((size_t)p * (nZen + 1) + t) * 3];
-            w[0] = m_sphere.vf[p][t].x;
-            w[1] = m_sphere.vf[p][t].y;
-            w[2] = m_sphere.vf[p][t].z;
-        }
-    if (e.nAz != nAz || e.nZen != nZen || e.azStep != m_sphere.azinuthStep
-        || e.theta != theta || e.vf != vf)
+    if (e.nAz != g.nAz || e.nZen != g.nZen || e.azStep != g.azStep || e.theta != g.theta
+        || e.vf != g.vf)
     {
-        mbs_grid g;
-        g.n_az = nAz;
-        g.n_zen = nZen;
-        g.theta = theta.data();
-        g.az_step = m_sphere.azinuthStep;
-        g.vf = vf.data();
-        const int rc = mbs_set_grid(e.ctx, &g);
-        if (rc == MBS_ERR_DEVICE)
-            return DeviceFailure("mbs_set_grid", mbs_last_error(e.ctx));
+        mbs_grid mg;
+        mg.n_az = g.nAz;
+        mg.n_zen = g.nZen;
+        mg.theta = g.theta.data();
+        mg.az_step = g.azStep;
+        mg.vf = g.vf.data();
+        const int rc = mbs_set_grid(e.ctx, &mg);
         if (rc != MBS_OK)
-            return false;
-        e.nAz = nAz;
-        e.nZen = nZen;
-        e.azStep = m_sphere.azinuthStep;
-        e.theta.swap(theta);
-        e.vf.swap(vf);
+        {
+            ok = rc == MBS_ERR_DEVICE ? DeviceFailure("mbs_set_grid", mbs_last_error(e.ctx))
+                                      : false;
+            return nullptr;
+        }
+        e.nAz = g.nAz;
+        e.nZen = g.nZen;
+        e.azStep = g.azStep;
+        e.theta.swap(g.theta);
+        e.vf.swap(g.vf);
     }
+    return &e;
+}
 
-    // Handler constants (src/handler/Handler.cpp:102-126).
-    mbs_consts c;
-    std::memset(&c, 0, sizeof c);
-    c.wave_index = m_waveIndex;
-    c.eps1 = m_eps1;
-    c.compl_wave[0] = real(m_complWave);
-    c.compl_wave[1] = imag(m_complWave);
-    c.inv_compl_wave[0] = real(m_invComplWave);
-    c.inv_compl_wave[1] = imag(m_invComplWave);
-    c.legacy_sign = m_legacySign ? 1 : 0;
-
-    const bool noShadow = ComputeNoShadow();
+// Diffracts one packed batch and adds the result into localM (and into
+// *localMns when it is given).
+bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
+              Arr2D &localM, Arr2D *localMns)
+{
+    const int nAz = e.nAz, nZen = e.nZen;
     const bool timing = EnvIs("MBS_GPU_TIMING", "1");
     mbs_set_timing(e.ctx, timing ? 1 : 0);
     const mbs_beam_batch view = pk.View();
     int rc = mbs_upload_beams(e.ctx, &view, &c);
     if (rc == MBS_OK)
-        rc = mbs_diffract(e.ctx, PrecisionFlag() | (noShadow ? MBS_NOSHADOW : 0u));
+        rc = mbs_diffract(e.ctx, PrecisionFlag() | (localMns ? MBS_NOSHADOW : 0u));
     std::vector<double> mFull((size_t)nAz * (nZen + 1) * 16, 0.0);
-    std::vector<double> mNs(noShadow ? mFull.size() : 0, 0.0);
+    std::vector<double> mNs(localMns ? mFull.size() : 0, 0.0);
     if (rc == MBS_OK)
-        rc = mbs_accumulate_mueller(e.ctx, mFull.data(), noShadow ? mNs.data() : nullptr);
+        rc = mbs_accumulate_mueller(e.ctx, mFull.data(), localMns ? mNs.data() : nullptr);
     if (rc == MBS_ERR_DEVICE)
         return DeviceFailure("diffraction", mbs_last_error(e.ctx));
     if (rc != MBS_OK)
@@ -338,8 +434,8 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
             std::fprintf(stderr,
                          "GPU timing: %d orientations, %zu beams: upload %.3f ms, kernel %.3f "
                          "ms, reduce %.3f ms (%d work items, %d workgroups)\n",
-                         nOrient, pk.nv.size(), tm.upload_ms, tm.kernel_ms, tm.reduce_ms,
-                         tm.n_items, tm.n_blocks);
+                         (int)pk.weights.size(), pk.nv.size(), tm.upload_ms, tm.kernel_ms,
+                         tm.reduce_ms, tm.n_items, tm.n_blocks);
     }
     for (int p = 0; p < nAz; ++p)
         for (int t = 0; t <= nZen; ++t)
@@ -348,14 +444,63 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
             double *dst = localM.RawCell(p, t);
             for (int k = 0; k < 16; ++k)
                 dst[k] += mFull[cell + k];
-            if (noShadow)
+            if (localMns)
             {
-                double *dn = localM_noshadow.RawCell(p, t);
+                double *dn = localMns->RawCell(p, t);
                 for (int k = 0; k < 16; ++k)
                     dn[k] += mNs[cell + k];
             }
         }
     return true;
+}
+}  // namespace
+
+// Handler constants (src/handler/Handler.cpp:102-126).
+static mbs_consts HandlerConsts(double waveIndex, double eps1, const ::complex &complWave,
+                                const ::complex &invComplWave, bool legacySign)
+{
+    mbs_consts c;
+    std::memset(&c, 0, sizeof c);
+    c.wave_index = waveIndex;
+    c.eps1 = eps1;
+    c.compl_wave[0] = real(complWave);
+    c.compl_wave[1] = imag(complWave);
+    c.inv_compl_wave[0] = real(invComplWave);
+    c.inv_compl_wave[1] = imag(invComplWave);
+    c.legacy_sign = legacySign ? 1 : 0;
+    return c;
+}
+
+// scale != 1 is the multi-size path's rescaled beam set (see PackScale); the
+// aperture integrals keep the handler's own wave index, `waveIndex` only
+// re-phases J_phased, as in the reference.
+bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
+    const std::vector<PreparedOrientation> &prepared, int start, int count, int device,
+    Arr2D &localM, Arr2D &localM_noshadow, double scale, double waveIndex,
+    unsigned long long /*cacheToken*/)
+{
+    if (!isCoh)
+        return false;
+    if (start < 0 || count < 0 || start > (int)prepared.size())
+        return false;
+    const int nOrient = std::min(count, (int)prepared.size() - start);
+    if (nOrient == 0 || m_sphere.nAzimuth * (m_sphere.nZenith + 1) == 0)
+        return true;
+
+    PackedBatch pk;
+    if (!Pack(prepared, start, nOrient, PackScale(scale, waveIndex, AbsorptionCoefficient()),
+              pk))
+        return false;
+    if (pk.nv.empty())
+        return true;
+    GridSpec g = MakeGrid(m_sphere);
+    bool ok = false;
+    ThreadContexts::Entry *e = Context(device, g, ok);
+    if (!e)
+        return ok;
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    return RunBatch(*e, pk, c, localM, ComputeNoShadow() ? &localM_noshadow : nullptr);
 }
 
 bool HandlerPO::HandleOrientationsToLocalGpu(const std::vector<PreparedOrientation> &prepared,
@@ -388,11 +533,41 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     return false;
 }
 
-bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOrientation> &,
-                                                   int, int, const std::vector<double> &,
-                                                   double, std::vector<Arr2D> &)
+// The multi-size path's fused call (src/scattering/TracerPOTotal.cpp:4380-4432):
+// one traced beam set, diffracted for every scale s into localMs[s] (full
+// output only; the caller uses it without no-shadow output).  Each scale is
+// packed as in HandleOrientationsToLocalGpu(..., scale, waveIndex) and runs
+// on the same device context.
+bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOrientation> &prepared,
+                                                   int start, int count,
+                                                   const std::vector<double> &scales,
+                                                   double waveIndex, std::vector<Arr2D> &localMs)
 {
-    return false;
+    if (!isCoh)
+        return false;
+    if (start < 0 || count < 0 || start > (int)prepared.size() || scales.empty()
+        || localMs.size() < scales.size())
+        return false;
+    const int nOrient = std::min(count, (int)prepared.size() - start);
+    if (nOrient == 0 || m_sphere.nAzimuth * (m_sphere.nZenith + 1) == 0)
+        return true;
+    GridSpec g = MakeGrid(m_sphere);
+    bool ok = false;
+    ThreadContexts::Entry *e = Context(0, g, ok);
+    if (!e)
+        return ok;
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    PackedBatch pk;
+    for (size_t s = 0; s < scales.size(); ++s)
+    {
+        if (!Pack(prepared, start, nOrient,
+                  PackScale(scales[s], waveIndex, AbsorptionCoefficient()), pk))
+            return false;
+        if (!pk.nv.empty() && !RunBatch(*e, pk, c, localMs[s], nullptr))
+            return false;
+    }
+    return true;
 }
 
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(

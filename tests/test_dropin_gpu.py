@@ -50,6 +50,8 @@ def _run(binary, args, wd, out, backend, env=None):
                        text=True, timeout=600)
     assert p.returncode == 0, "%s failed (rc %d):\n%s\n%s" % (
         os.path.basename(binary), p.returncode, p.stdout[-2000:], p.stderr[-2000:])
+    with open(os.path.join(wd, out + ".log"), "w") as f:
+        f.write(p.stdout + p.stderr)
     m = re.search(r"Phase 2 \(diffraction[^)]*\): ([0-9.]+) s", p.stdout)
     return float(m.group(1)) if m else float("nan")
 
@@ -58,11 +60,14 @@ def _dat(wd, out, suffix=""):
     return np.loadtxt(os.path.join(wd, out, out + suffix + ".dat"), skiprows=1)
 
 
-def _case(L, D, n_im, refl, n_orient, nphi):
-    wd = tempfile.mkdtemp(prefix="mbs_dropin_")
+def _case(particle, n_im, refl, n_orient, nphi, wd=None):
+    """particle: the --particle numbers, or a particle file name in wd."""
+    wd = wd or tempfile.mkdtemp(prefix="mbs_dropin_")
     with open(os.path.join(wd, "theta.grid"), "w") as f:
         f.write("\n".join("%.10g" % t for t in THETA_CONFIG2) + "\n")
-    args = ["--method", "po", "--particle", "1", str(L), str(D),
+    source = (["--pf", particle] if isinstance(particle, str)
+              else ["--particle"] + [str(a) for a in particle])
+    args = ["--method", "po"] + source + [
             "--refractive-index", "1.3116", str(n_im), "--wavelength-um", "0.532",
             "--max-reflections", str(refl), "--sobol-seed", str(n_orient), "1",
             "--theta-grid-file", "theta.grid", "--phi-points", str(nphi),
@@ -71,13 +76,7 @@ def _case(L, D, n_im, refl, n_orient, nphi):
     return wd, args
 
 
-@pytest.mark.parametrize("shape", [(20, 10, 1e-4, 6, 8, 24), (100, 50, 1e-4, 10, 16, 96)],
-                         ids=["D10_8orient_24phi", "D50_16orient_96phi"])
-def test_reference_cli_gpu_backend_matches_cpu_backend(shape):
-    L, D, n_im, refl, n_orient, nphi = shape
-    print("case D=%g L=%g refl=%d orientations=%d phi=%d" % (D, L, refl, n_orient, nphi),
-          flush=True)
-    wd, args = _case(L, D, n_im, refl, n_orient, nphi)
+def _compare(wd, args):
     t_cpu = _run(CPU_BIN, args, wd, "cpu", "cpu")
     cpu, cpu_ns = _dat(wd, "cpu"), _dat(wd, "cpu", "_noshadow")
     t64 = _run(HIP_BIN, args, wd, "g64", "gpu", {"MBS_HIP_PRECISION": "fp64"})
@@ -92,3 +91,124 @@ def test_reference_cli_gpu_backend_matches_cpu_backend(shape):
           "(no-shadow %.2e), mixed P11 %.2e" % (t_cpu, t64, t32, e64, e64_ns, e32))
     assert e64 < 1e-6 and e64_ns < 1e-6
     assert e32 < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(20, 10, 1e-4, 6, 8, 24), (100, 50, 1e-4, 10, 16, 96)],
+                         ids=["D10_8orient_24phi", "D50_16orient_96phi"])
+def test_reference_cli_gpu_backend_matches_cpu_backend(shape):
+    L, D, n_im, refl, n_orient, nphi = shape
+    print("case D=%g L=%g refl=%d orientations=%d phi=%d" % (D, L, refl, n_orient, nphi),
+          flush=True)
+    _compare(*_case((1, L, D), n_im, refl, n_orient, nphi))
+
+
+# The reference's other built-in particles (src/main.cpp ParticleType):
+# bullet rosette (3: LENGTH DIAMETER, six bullets), droxtal (4: SIZE), concave
+# hexagonal column (10: LENGTH DIAMETER CAVITY_DEPTH, traced as non-convex) and
+# the two-column aggregate (12: LENGTH DIAMETER 2, non-convex).  Non-convex
+# tracing gives beams with more vertices and shadowed apertures; the slot and
+# the engine must handle them unchanged.
+@pytest.mark.parametrize("particle", [(3, 20, 10), (4, 10), (10, 20, 10, 0.3), (12, 20, 10, 2)],
+                         ids=["bullet_rosette", "droxtal", "concave_hexagonal",
+                              "two_column_aggregate"])
+def test_reference_cli_gpu_backend_other_particles(particle):
+    print("case particle %s refl=6 orientations=8 phi=24" % (particle,), flush=True)
+    _compare(*_case(particle, 1e-4, 6, 8, 24))
+
+
+def write_column_aggregate(path, n_columns=8, seed=7, d_range=(5.0, 30.0), ld_range=(1.0, 4.0)):
+    """Writes an aggregate of hexagonal columns in the reference's particle
+    file format (README.md "strict text format": concave flag, '1 8' =
+    aggregate of 8-facet components, symmetry 180 360, blank-line separated
+    facets wound with outward normals).  Column i: diameter D ~ U(d_range)
+    across corners, length D * U(ld_range), axis along x, y or z with a
+    random spin about it, centre placed at random with overlapping bounding
+    spheres rejected.  Axes stay on the coordinate axes because the loader
+    snaps coordinates to 2^-40 of the particle's extent
+    (src/particle/Particle.cpp:796-806) and then checks each facet's
+    planarity to 2^-40 of the facet's own size (:116-160): a hexagon at a
+    constant axis coordinate and a side rectangle parallel to the axis stay
+    exactly planar under that snapping, a tilted facet does not.  Returns
+    the bounding radius of every column."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    while len(cols) < n_columns:
+        d = rng.uniform(*d_range)
+        length = d * rng.uniform(*ld_range)
+        r_bound = np.hypot(0.5 * d, 0.5 * length)
+        axis = int(rng.integers(3))
+        spin = rng.uniform(0.0, np.pi / 3)
+        spread = 0.5 * (40.0 + 10.0 * len(cols))
+        for _ in range(1000):
+            c = rng.uniform(-spread, spread, 3)
+            if all(np.linalg.norm(c - c2) > r_bound + r2 + 0.5 for c2, r2, _ in cols):
+                break
+        else:
+            continue
+        ang = spin + np.arange(6) * np.pi / 3
+        ring = np.stack([0.5 * d * np.cos(ang), 0.5 * d * np.sin(ang)], 1)
+        top = np.c_[ring, np.full(6, 0.5 * length)]
+        bot = np.c_[ring, np.full(6, -0.5 * length)]
+        facets = [top, bot[::-1]] + [np.array([bot[k], bot[(k + 1) % 6], top[(k + 1) % 6], top[k]])
+                                     for k in range(6)]
+        perm = [(1, 2, 0), (2, 0, 1), (0, 1, 2)][axis]  # cyclic, so the winding stays outward
+        cols.append((c, r_bound, [f[:, perm] + c for f in facets]))
+    with open(path, "w") as f:
+        f.write("1\n1 8\n180 360\n")
+        for _, _, facets in cols:
+            for fc in facets:
+                f.write("\n" + "".join("%.17g %.17g %.17g\n" % tuple(v) for v in fc))
+    return [r for _, r, _ in cols]
+
+
+def test_reference_cli_gpu_backend_eight_column_aggregate():
+    """An 8-column aggregate read from a particle file (seed 7, D ~ U(5, 30)
+    um, L/D ~ U(1, 4)): non-convex tracing over 64 facets, ~4.4k beams per
+    orientation with up to 13 vertices at 6 reflections."""
+    wd = tempfile.mkdtemp(prefix="mbs_dropin_")
+    radii = write_column_aggregate(os.path.join(wd, "aggregate.particle"))
+    print("case 8-column aggregate (bounding radii %s um) refl=6 orientations=4 phi=24"
+          % ", ".join("%.1f" % r for r in radii), flush=True)
+    _compare(*_case("aggregate.particle", 1e-4, 6, 4, 24, wd))
+
+
+@pytest.mark.parametrize("fused", [False, True], ids=["per_size_scaled", "fused_multi_scale"])
+def test_reference_cli_multi_size(fused):
+    """The reference's shared-trace multi-size mode (--multigrid D_MIN D_MAX N,
+    src/scattering/TracerPOTotal.cpp:4070-4520): one trace at the largest size,
+    then every size diffracted from rescaled beams.  Per size it calls
+    HandleOrientationsToLocalGpu(..., scale, waveIndex) (scale < 1 for the
+    smaller sizes); with MBS_GPU_MULTI_K_FULL=1 it calls the fused
+    HandleOrientationsToLocalGpuMultiK for all sizes at once.  Absorbing
+    (m = 1.3116 + 1e-3 i), so the absorption factors are rescaled too."""
+    wd = tempfile.mkdtemp(prefix="mbs_multisize_")
+    args = ["--method", "po", "--particle", "1", "20", "10",
+            "--refractive-index", "1.3116", "1e-3", "--wavelength-um", "0.532",
+            "--max-reflections", "6", "--oldauto", "8", "--multigrid", "6", "10", "3",
+            "--scattering-grid", "0", "180", "24", "180", "--cutoff-profile", "off",
+            "--threads", str(_threads()), "--close"]
+    print("case multi-size D 6..10 (3 sizes), 150 orientations, fused=%s" % fused, flush=True)
+    t_cpu = _run(CPU_BIN, args, wd, "cpu", "cpu")
+    env = {"MBS_HIP_PRECISION": "fp64"}
+    gpu_args = list(args)
+    if fused:
+        # The reference CLI accepts this experimental switch only when told to.
+        env["MBS_GPU_MULTI_K_FULL"] = "1"
+        gpu_args.append("--allow-experimental-environment")
+    t_gpu = _run(HIP_BIN, gpu_args, wd, "gpu", "gpu", env)
+    log = open(os.path.join(wd, "gpu.log")).read()
+    if fused:
+        assert "fused multi-k kernel enabled" in log
+        assert "falling back to per-size GPU path" not in log
+    sizes = sorted(f[len("cpu_"):-len(".dat")] for f in os.listdir(os.path.join(wd, "cpu"))
+                   if f.endswith(".dat"))
+    assert len(sizes) == 3, sizes
+    for size in sizes:
+        cpu = _dat(wd, "cpu", "_" + size)
+        gpu = _dat(wd, "gpu", "_" + size)
+        assert gpu.shape == cpu.shape
+        np.testing.assert_array_equal(gpu[:, :2], cpu[:, :2])
+        e = M.gate_l2(gpu, cpu)
+        print("  size %s: gate_l2 fp64 %.2e" % (size, e))
+        assert e < 1e-6
+    print("multi-size phase 2: cpu %.2f s, gpu %.2f s" % (t_cpu, t_gpu))

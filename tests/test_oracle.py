@@ -240,6 +240,30 @@ def test_solid_angle_weights_sum_to_4pi(pkg, oracle):
         assert avg[:, 1].sum() == pytest.approx(4 * np.pi, rel=1e-13)
 
 
+def test_convergence_error_measures():
+    """A uniform M11 scale of (1 + d) is an M11 error d and leaves every
+    ratio exact; adding e * M11 to one element is an error e in that element
+    alone, in both the weighted and the plot form."""
+    import metrics
+    rng = np.random.default_rng(3)
+    theta = np.radians(np.r_[np.arange(0, 10.001, 0.05), np.arange(10.5, 180.001, 0.5)])
+    ref = np.zeros((len(theta), 18))
+    ref[:, 0], ref[:, 1] = np.degrees(theta), metrics.two_pi_dcos(theta)
+    ref[:, 2] = rng.uniform(1, 1e4, len(theta))
+    ref[:, 3:] = rng.uniform(-0.5, 0.5, (len(theta), 15)) * ref[:, 2:3]
+    for f in (metrics.convergence_errors, metrics.plot_element_errors):
+        np.testing.assert_array_equal(f(ref, ref), np.zeros(16))
+        a = ref.copy()
+        a[:, 2:] *= 1.003
+        e = f(a, ref)
+        assert e[0] == pytest.approx(0.003, rel=1e-9) and np.all(e[1:] < 1e-15)
+        a = ref.copy()
+        a[:, 2 + 6] += 0.02 * a[:, 2]
+        e = f(a, ref)
+        assert e[6] == pytest.approx(0.02, rel=1e-9)
+        assert np.all(np.delete(e, 6) < 1e-15)
+
+
 def test_azimuth_average_undoes_frame_rotation(pkg, oracle):
     """If M(phi_p) = M0 . L(-phi_p)^-1, the lab-frame average returns M0."""
     theta = np.linspace(0.1, 3.0, 5)
