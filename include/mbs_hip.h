@@ -298,7 +298,7 @@ int mbs_set_timing(mbs_ctx *ctx, int enable);
 int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out);
 
 /* Tuning: approximate number of workgroups one mbs_diffract launch aims
- * for (default 4096).  Work items are sized from it. */
+ * for (default 65536).  Work items are sized from it. */
 int mbs_set_target_blocks(mbs_ctx *ctx, int blocks);
 
 /* Work decomposition of mbs_diffract: 0 = automatic (beam segments of equal

@@ -96,7 +96,9 @@ struct mbs_ctx {
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
     size_t pinned_cap = 0;
-    int target_blocks = 4096;
+    // ~65 workgroups per slot (256 CUs x 4): items small enough that the
+    // static per-XCD share of workgroups carries the same work on every XCD
+    int target_blocks = 65536;
     int work_mode = 0;  // 0 auto, 1 whole orientations, 2 beam segments
     // launch shape chosen by mbs_autotune, per precision (0 FP64, 1 mixed)
     struct Tuned {
@@ -1089,7 +1091,7 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
     int rc = mbs_set_timing(ctx, 1);
     if (rc)
         return rc;
-    static const int kBlocks[] = {1024, 2048, 4096, 8192, 16384, 32768};
+    static const int kBlocks[] = {4096, 8192, 16384, 32768, 65536, 131072};
     const int n_tiles = (ctx->cells + 255) / 256;
     std::vector<mbs_tune_entry> tried;
     mbs_ctx::Tuned best;

@@ -30,13 +30,14 @@
 #include "po_math.h"
 #include "po_layout.h"
 
-// Occupancy hint of the main kernel (waves per SIMD); the default lets the
-// compiler choose.  Builds may pass -DMBS_MIN_WAVES=N for tuning.
-#ifdef MBS_MIN_WAVES
-#define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
-#else
-#define MBS_OCCUPANCY
+// Occupancy of the vector-ALU kernels: at least 4 waves per SIMD (128
+// VGPRs).  The FP64 kernel, whose batched vertex phasors would otherwise take
+// ~140 VGPRs and 3 waves, spills only a few scalars per beam to keep it.
+// Builds may pass -DMBS_MIN_WAVES=N for tuning.
+#ifndef MBS_MIN_WAVES
+#define MBS_MIN_WAVES 4
 #endif
+#define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
 // The MFMA kernel is held to 4 waves per SIMD (128 VGPRs + AGPRs) when it
 // keeps one accumulator set; the no-shadow variant (two sets) gets 2.
 #ifndef MBS_MFMA_WAVES
@@ -263,35 +264,52 @@ __device__ __forceinline__ double uniform_f64(double x)
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Lanes for which edge e enters the sum, as a wave mask: an edge with a
+// valid dx in the lanes of mB (|B| > |A|), with a valid dy in the others.
+// The flags are wave-uniform, so this is scalar mask arithmetic and the
+// per-lane choice costs no vector instruction.  Used by the FP64 sums; in
+// the mixed ones the extra scalar work measured slower than the two vector
+// instructions it saves.
+__device__ __forceinline__ uint64_t edge_valid_mask(int flags, uint64_t mB)
+{
+    const uint64_t mx = (flags & EDGE_VX) ? ~0ull : 0ull;
+    const uint64_t my = (flags & EDGE_VY) ? ~0ull : 0ull;
+    return (mB & mx) | (~mB & my);
+}
+
 // One edge of the aperture sum (FP64 phasors): the phasor of vertex e+1
 // (vertex 0 again for the closing edge), the edge coefficient
-// C = slope * mul + add and its contribution (e^{i phi_{e+1}} - e^{i phi_e}) / C,
-// or the exact half-angle form when |C| <= eps1.
+// C = A + (dy/dx) B (|B| > |A|) or A (dx/dy) + B (otherwise) and its
+// contribution (e^{i phi_{e+1}} - e^{i phi_e}) / C, or the exact half-angle
+// form when |C| <= eps1.
 __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict__ vtx,
                                          const double *__restrict__ vdd,
                                          const int32_t *__restrict__ vflag, double A, double B,
-                                         double psi, bool useB, double mul, double add,
-                                         int vmask, const double2 *__restrict__ tab, double k,
+                                         double psi, bool useB, uint64_t mB,
+                                         const double2 *__restrict__ tab, double k,
                                          double eps1, double s0, double c0, double &sp,
                                          double &cp, double &sr, double &si, bool &small)
 {
     const double *v = vtx + (size_t)e * VTX_STRIDE;
-    const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
-    const int f = vflag[e];
+    // both slopes as scalars and both coefficients, then a per-lane select
+    // (a select between the two loads becomes a divergent branch around each
+    // scalar load)
+    const double syx = uniform_f64(v[V_SYX]), sxy = uniform_f64(v[V_SXY]);
+    const int f = __builtin_amdgcn_readfirstlane(vflag[e]);
     double sn = s0, cn = c0;
     if (e + 1 < n) {
         const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
         small = small && __builtin_fabs(pn) <= kSmallPhase;
         sincos_table(pn + psi, tab, sn, cn);
     }
-    const double cc = __builtin_fma(slope, mul, add);
-    const bool valid = (f & vmask) != 0;
-    const bool big = __builtin_fabs(cc) > eps1;
+    const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
+    const uint64_t mvalid = edge_valid_mask(f, mB);
+    const uint64_t mbig = __builtin_amdgcn_ballot_w64(__builtin_fabs(cc) > eps1);
     const double r = rcp_f64_fast(cc);
-    const double inv = (valid && big) ? r : 0.0;
+    const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
     sr = __builtin_fma(cn - cp, inv, sr);
     si = __builtin_fma(sn - sp, inv, si);
-    if (__builtin_expect(valid && !big, 0)) {
+    if (__builtin_expect(__builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig), 0)) {
         const double *w = vdd + (size_t)e * VDD_STRIDE;
         const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
         const double2 q = stable_edge_quotient(cp, sp, dd, cc, k);
@@ -303,8 +321,9 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
 }
 
 // Aperture integral F(beam, direction) / complWave in FP64 (A4).  NV > 0
-// is the vertex count known at compile time (the loop is unrolled and its
-// scalar loads issued up front); NV = 0 loops over nv.  Returns false when
+// is the vertex count known at compile time (the loops are unrolled, the
+// scalar loads issued up front and every vertex's table read issued before
+// the first is used); NV = 0 loops over nv.  Returns false when
 // the result is NaN (degenerate beam, skipped).
 template <int NV>
 __device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
@@ -319,24 +338,70 @@ __device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
     const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
     const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
     const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
-    const double mul = useB ? B : A;  // coefficient = slope * mul + add
-    const double add = useB ? A : B;
-    const int vmask = useB ? 1 : 2;
+    const uint64_t mB = __builtin_amdgcn_ballot_w64(useB);  // lanes summing the dx-valid edges
     // psi is folded into the vertex phasors (the integral is linear in each
     // phasor), as the reference CUDA path does (GpuDiffraction.cu:1110-1122)
-    const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-    bool small = __builtin_fabs(ph) <= kSmallPhase;
-    double s0, c0;
-    sincos_table(ph + psi, tab, s0, c0);
-    double sp = s0, cp = c0, sr = 0.0, si = 0.0;
+    double sr = 0.0, si = 0.0;
+    bool small;
     if constexpr (NV > 0) {
+        // all vertex phasors first (scalar loads, then every table read), so
+        // the LDS latency of one vertex overlaps the others; then the edges
+        double kx[NV], ky[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            kx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KX]);
+            ky[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KY]);
+        }
+        double dv[NV];
+        double2 ev[NV];
+        small = true;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const double p = __builtin_fma(A, kx[e], B * ky[e]);
+            small = small && __builtin_fabs(p) <= kSmallPhase;
+            sincos_reduce(p + psi, tab, dv[e], ev[e]);
+        }
+        double sv[NV], cv[NV];
 #pragma unroll
         for (int e = 0; e < NV; ++e)
-            edge_f64(e, NV, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1, s0,
-                     c0, sp, cp, sr, si, small);
+            sincos_finish(dv[e], ev[e], sv[e], cv[e]);
+        double syx[NV], sxy[NV];
+        int fl[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            syx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SYX]);
+            sxy[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SXY]);
+            fl[e] = __builtin_amdgcn_readfirstlane(vflag[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int f = e + 1 < NV ? e + 1 : 0;
+            // both coefficients from scalar slopes, then a per-lane select
+            const double cx = __builtin_fma(syx[e], B, A);
+            const double cy = __builtin_fma(sxy[e], A, B);
+            const double cc = useB ? cx : cy;
+            const uint64_t mvalid = edge_valid_mask(fl[e], mB);
+            const uint64_t mbig = __builtin_amdgcn_ballot_w64(__builtin_fabs(cc) > eps1);
+            const double r = rcp_f64_fast(cc);
+            const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
+            sr = __builtin_fma(cv[f] - cv[e], inv, sr);
+            si = __builtin_fma(sv[f] - sv[e], inv, si);
+            if (__builtin_expect(__builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig), 0)) {
+                const double *w = vdd + (size_t)e * VDD_STRIDE;
+                const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
+                const double2 q = stable_edge_quotient(cv[e], sv[e], useB ? dx : dy, cc, k);
+                sr += q.x;
+                si += q.y;
+            }
+        }
     } else {
+        const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+        small = __builtin_fabs(ph) <= kSmallPhase;
+        double s0, c0;
+        sincos_table(ph + psi, tab, s0, c0);
+        double sp = s0, cp = c0;
         for (int e = 0; e < nv; ++e)
-            edge_f64(e, nv, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1, s0,
+            edge_f64(e, nv, vtx, vdd, vflag, A, B, psi, useB, mB, tab, k, eps1, s0,
                      c0, sp, cp, sr, si, small);
     }
     const double nrm = rcp_f64(useB ? B : -A);
@@ -430,15 +495,14 @@ __device__ __forceinline__ void beam_dir_f64(
 __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restrict__ vtx,
                                            const double *__restrict__ vdd,
                                            const int32_t *__restrict__ vflag, double A,
-                                           double B, double psi, bool useB, double mul,
-                                           double add, int vmask,
+                                           double B, double psi, bool useB, int vmask,
                                            const float2 *__restrict__ tab, double k,
                                            double eps1, float s0, float c0, float &sp,
                                            float &cp, float &sr, float &si, bool &small)
 {
     const double *v = vtx + (size_t)e * VTX_STRIDE;
-    const double slope = useB ? uniform_f64(v[V_SYX]) : uniform_f64(v[V_SXY]);
-    const int f = vflag[e];
+    const double syx = uniform_f64(v[V_SYX]), sxy = uniform_f64(v[V_SXY]);
+    const int f = __builtin_amdgcn_readfirstlane(vflag[e]);
     float sn = s0, cn = c0;
     if (e + 1 < n) {
         const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
@@ -446,7 +510,7 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
         sincos_table_f32(pn + psi, tab, sn, cn);
     }
     // the edge coefficient is cancellation-prone: FP64
-    const double cc = __builtin_fma(slope, mul, add);
+    const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
     const bool valid = (f & vmask) != 0;
     const bool big = __builtin_fabs(cc) > eps1;
     // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
@@ -466,39 +530,15 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     cp = cn;
 }
 
-// Aperture integral F(beam, direction) / complWave, mixed precision: A, B,
-// the phases and their reduction, the edge coefficients and the moments in
-// FP64; phasors and edge sums in FP32.  NV as for aperture_f64.  A NaN
-// result (degenerate beam) is returned as 0, i.e. the beam is skipped.
-template <int NV>
-__device__ __forceinline__ void aperture_mixed_nv(
-    const double *__restrict__ h, const double *__restrict__ vtx,
-    const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
-    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
-    float &out_i)
+// The end of the mixed aperture integral: the 1/B or -1/A normalisation,
+// the small-phase moment form when every vertex phase is within 1e-3 rad,
+// and a NaN result (degenerate beam) returned as 0, i.e. the beam skipped.
+__device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__ h, double A,
+                                                      double B, double psi, bool useB,
+                                                      bool small, float sr, float si,
+                                                      const float2 *__restrict__ tab,
+                                                      float &out_r, float &out_i)
 {
-    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
-    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
-    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
-    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
-    const double mul = useB ? B : A;
-    const double add = useB ? A : B;
-    const int vmask = useB ? 1 : 2;
-    const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-    bool small = __builtin_fabs(ph) <= kSmallPhase;
-    float s0, c0;
-    sincos_table_f32(ph + psi, tab, s0, c0);
-    float sp = s0, cp = c0, sr = 0.0f, si = 0.0f;
-    if constexpr (NV > 0) {
-#pragma unroll
-        for (int e = 0; e < NV; ++e)
-            edge_mixed(e, NV, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1,
-                       s0, c0, sp, cp, sr, si, small);
-    } else {
-        for (int e = 0; e < nv; ++e)
-            edge_mixed(e, nv, vtx, vdd, vflag, A, B, psi, useB, mul, add, vmask, tab, k, eps1,
-                       s0, c0, sp, cp, sr, si, small);
-    }
     const float nrm = rcp_f32((float)(useB ? B : -A));
     float fr = sr * nrm, fi = si * nrm;
     if (__builtin_expect(small, 0)) {
@@ -515,6 +555,95 @@ __device__ __forceinline__ void aperture_mixed_nv(
     const bool bad = __builtin_isnan(fr);
     out_r = bad ? 0.0f : fr;
     out_i = bad ? 0.0f : fi;
+}
+
+// Aperture integral F(beam, direction) / complWave, mixed precision: A, B,
+// the phases and their reduction, the edge coefficients and the moments in
+// FP64; phasors and edge sums in FP32.
+//
+// NV > 0: the vertex count is known at compile time.  The phases of all
+// vertices are reduced and their table reads issued together, so the LDS
+// latency of one vertex overlaps the arithmetic of the others; the edges
+// follow.
+template <int NV>
+__device__ __forceinline__ void aperture_mixed_nv(
+    const double *__restrict__ h, const double *__restrict__ vtx,
+    const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
+    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
+    float &out_i)
+{
+    const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
+    const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
+    const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
+    const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
+    const int vmask = useB ? EDGE_VX : EDGE_VY;
+    float sr = 0.0f, si = 0.0f;
+    bool small;
+    if constexpr (NV > 0) {
+        // scalar loads first: LDS reads and scalar loads share one wait
+        // counter, so a scalar load between two table reads would drain both
+        double kx[NV], ky[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            kx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KX]);
+            ky[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KY]);
+        }
+        float dv[NV];
+        float2 ev[NV];
+        small = true;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const double p = __builtin_fma(A, kx[e], B * ky[e]);
+            small = small && __builtin_fabs(p) <= kSmallPhase;
+            sincos_reduce_f32(p + psi, tab, dv[e], ev[e]);
+        }
+        float sv[NV], cv[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e)
+            sincos_finish_f32(dv[e], ev[e], sv[e], cv[e]);
+        double syx[NV], sxy[NV];
+        int fl[NV];
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            syx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SYX]);
+            sxy[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SXY]);
+            fl[e] = __builtin_amdgcn_readfirstlane(vflag[e]);
+        }
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int f = e + 1 < NV ? e + 1 : 0;
+            // both coefficients from scalar slopes, then a per-lane select of
+            // the result (selecting the slope first costs two moves and a
+            // select per half of the FP64 operand)
+            const double cx = __builtin_fma(syx[e], B, A);
+            const double cy = __builtin_fma(sxy[e], A, B);
+            const double cc = useB ? cx : cy;
+            const bool valid = (fl[e] & vmask) != 0;
+            const bool big = __builtin_fabs(cc) > eps1;
+            const float r = __builtin_amdgcn_rcpf((float)cc);
+            const float inv = (valid && big) ? r : 0.0f;
+            sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
+            si = __builtin_fmaf(sv[f] - sv[e], inv, si);
+            if (__builtin_expect(valid && !big, 0)) {
+                const double *w = vdd + (size_t)e * VDD_STRIDE;
+                const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
+                const double2 q = stable_edge_quotient((double)cv[e], (double)sv[e],
+                                                       useB ? dx : dy, cc, k);
+                sr += (float)q.x;
+                si += (float)q.y;
+            }
+        }
+    } else {
+        const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+        small = __builtin_fabs(ph) <= kSmallPhase;
+        float s0, c0;
+        sincos_table_f32(ph + psi, tab, s0, c0);
+        float sp = s0, cp = c0;
+        for (int e = 0; e < nv; ++e)
+            edge_mixed(e, nv, vtx, vdd, vflag, A, B, psi, useB, vmask, tab, k, eps1,
+                       s0, c0, sp, cp, sr, si, small);
+    }
+    aperture_mixed_finish(h, A, B, psi, useB, small, sr, si, tab, out_r, out_i);
 }
 
 __device__ __forceinline__ void aperture_mixed(
@@ -696,10 +825,18 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
         }
         U j[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         U jn[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        // the next beam's vertex offset and count are loaded one beam
+        // ahead, so a beam's header and vertex loads issue together
+        int64_t v0_next = b0 < b1 ? voff[b0] : 0;
+        int nv_next = b0 < b1 ? nv[b0] : 0;
         for (int64_t b = b0; b < b1; ++b) {
             const double *h = hdr + (size_t)b * HDR_STRIDE;
-            const int64_t v0 = voff[b];
-            const int nvb = nv[b];
+            const int64_t v0 = v0_next;
+            const int nvb = nv_next;
+            if (b + 1 < b1) {
+                v0_next = voff[b + 1];
+                nv_next = nv[b + 1];
+            }
             const double *bv = vtx + (size_t)v0 * VTX_STRIDE;
             const double *bd = vdd + (size_t)v0 * VDD_STRIDE;
             if constexpr (PREC == 0) {
@@ -830,13 +967,22 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
             b1 = it.b_end;
         }
         f32x16 g0 = {}, g1 = {}, n0 = {}, n1 = {};
+        // the next beam's vertex offset and count are loaded one beam
+        // ahead, so a beam's header and vertex loads issue together
+        int64_t v0_next = b0 < b1 ? voff[b0] : 0;
+        int nv_next = b0 < b1 ? nv[b0] : 0;
         for (int64_t b = b0; b < b1; ++b) {
             const float bop = wb[(size_t)b * WB_STRIDE + lane];
-            const int64_t v0 = voff[b];
+            const int64_t v0 = v0_next;
+            const int nvb = nv_next;
+            if (b + 1 < b1) {
+                v0_next = voff[b + 1];
+                nv_next = nv[b + 1];
+            }
             float fr, fi;
             aperture_mixed(hdr + (size_t)b * HDR_STRIDE, vtx + (size_t)v0 * VTX_STRIDE,
-                           vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nv[b], d, s_tab, k,
-                           eps1, fr, fi);
+                           vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nvb, d, s_tab, k, eps1,
+                           fr, fi);
             // A operands: lanes 0-31 (Re F), lanes 32-63 (Im F) of the
             // wave's directions 0-31, then the same for directions 32-63
             const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(fr),

@@ -40,6 +40,9 @@ enum { WB_COLS = 24, WB_STRIDE = 64 };
 // only by the rare near-singular branch, live in a separate array.
 enum { V_KX = 0, V_KY = 1, V_SYX = 2, V_SXY = 3, VTX_STRIDE = 4 };
 enum { D_DX = 0, D_DY = 1, VDD_STRIDE = 2 };
+// Per-vertex edge flags (as the C ABI's MBS_EDGE_VALID_X / _Y): the edge
+// from this vertex has a usable dx (enters the |B| > |A| sum) / dy.
+enum { EDGE_VX = 1, EDGE_VY = 2 };
 
 // Grid tables, SoA [DT_COUNT][cells].
 enum { DT_DIR = 0, DT_U0 = 3, DT_U1 = 9, DT_COUNT = 15 };

@@ -6,7 +6,7 @@
 // Taylor polynomials (see sincos_table below).  Phases reach ~1e4 rad for the
 // largest particles; the reduction is exact to an ulp for |x| < 2^20.
 // The FP32 variant used by the mixed-precision kernels reduces in FP64 the
-// same way and evaluates the polynomials in FP32 (error ~1 ulp of FP32).
+// same way and evaluates shorter polynomials in FP32 (error ~1 ulp of FP32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,13 +43,19 @@ constexpr double kTabInvStep = 40.743665431525205956;  // 256 / (2 pi)
 constexpr double kTabStep1 = 3.141592653589793116 / 128.0;
 constexpr double kTabStep2 = 1.2246467991473532e-16 / 128.0;
 
-__device__ __forceinline__ void sincos_table(double x, const double2 *__restrict__ tab,
-                                             double &s, double &c)
+// In two halves, as for the FP32 variant below: reduction and table read,
+// then polynomials and product.
+__device__ __forceinline__ void sincos_reduce(double x, const double2 *__restrict__ tab,
+                                              double &d, double2 &e)
 {
     const double j = __builtin_rint(x * kTabInvStep);
-    double d = __builtin_fma(-j, kTabStep1, x);
+    d = __builtin_fma(-j, kTabStep1, x);
     d = __builtin_fma(-j, kTabStep2, d);
-    const double2 e = tab[(int)j & (kTabN - 1)];
+    e = tab[(int)j & (kTabN - 1)];
+}
+
+__device__ __forceinline__ void sincos_finish(double d, double2 e, double &s, double &c)
+{
     const double z = d * d;
     double ps = __builtin_fma(z, -1.0 / 5040.0, 1.0 / 120.0);
     ps = horner(ps, z, -1.0 / 6.0);
@@ -61,20 +67,48 @@ __device__ __forceinline__ void sincos_table(double x, const double2 *__restrict
     c = __builtin_fma(e.x, cd, -(e.y * sd));
 }
 
-// FP32 result from an FP64 argument, same table (converted to FP32 values).
-__device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restrict__ tab,
-                                                 float &s, float &c)
+__device__ __forceinline__ void sincos_table(double x, const double2 *__restrict__ tab,
+                                             double &s, double &c)
+{
+    double d;
+    double2 e;
+    sincos_reduce(x, tab, d, e);
+    sincos_finish(d, e, s, c);
+}
+
+// FP32 result from an FP64 argument, same table (converted to FP32 values),
+// in two halves so that callers can issue the table reads of several
+// arguments before using any of them: the FP64 reduction and table read,
+// then the FP32 polynomials and the product with the table entry.
+__device__ __forceinline__ void sincos_reduce_f32(double x, const float2 *__restrict__ tab,
+                                                  float &d, float2 &e)
 {
     const double j = __builtin_rint(x * kTabInvStep);
     double dd = __builtin_fma(-j, kTabStep1, x);
     dd = __builtin_fma(-j, kTabStep2, dd);
-    const float d = (float)dd;
-    const float2 e = tab[(int)j & (kTabN - 1)];
+    d = (float)dd;
+    e = tab[(int)j & (kTabN - 1)];
+}
+
+// |d| <= pi/256 = 0.0123, so sin d = d - d^3/6 and cos d = 1 - d^2/2 are
+// exact to FP32: the next terms, d^5/120 < 2.4e-12 and d^4/24 < 9.4e-10,
+// lie far below half an ulp of the results (3e-8 near 1).
+__device__ __forceinline__ void sincos_finish_f32(float d, float2 e, float &s, float &c)
+{
     const float z = d * d;
-    const float sd = __builtin_fmaf(d * z, __builtin_fmaf(z, 1.0f / 120.0f, -1.0f / 6.0f), d);
-    const float cd = __builtin_fmaf(z, __builtin_fmaf(z, 1.0f / 24.0f, -0.5f), 1.0f);
+    const float sd = __builtin_fmaf(d * z, -1.0f / 6.0f, d);
+    const float cd = __builtin_fmaf(z, -0.5f, 1.0f);
     s = __builtin_fmaf(e.y, cd, e.x * sd);
     c = __builtin_fmaf(e.x, cd, -(e.y * sd));
+}
+
+__device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restrict__ tab,
+                                                 float &s, float &c)
+{
+    float d;
+    float2 e;
+    sincos_reduce_f32(x, tab, d, e);
+    sincos_finish_f32(d, e, s, c);
 }
 
 // 1/x to full FP64 precision: hardware estimate + two Newton steps.

@@ -10,8 +10,9 @@ error is at most 1 % on every element P11 ... P44.
 
 Workload: D = 50 um, L = 100 um, m = 1.3116 + 1e-4 i, lambda = 0.532 um, 10
 reflections, theta rows 0:0.05:10, 10.5:0.5:180 (541) x 96 phi, Owen-Sobol
-orientations (--sobol-seed N 1).  Converged reference: FP64 engine, an
-independent scramble (seed 7) with N_REF orientations.
+orientations (--sobol-seed N 1).  Converged reference: FP64 engine, the mean
+of REF_RUNS independent scrambles (seeds 7, 8, ...) of REF_N orientations
+each (4 x 32768 = 131072 by default; every run reports when it ends).
 
 Sides:
   gpu fp64   oracle/_ref/mbs_po_hip --backend gpu, MBS_HIP_PRECISION=fp64
@@ -51,7 +52,8 @@ pytestmark = [pytest.mark.gpu,
                                  reason="reference CLI builds (oracle/_ref) are not present")]
 
 THETA = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
-N_REF = int(os.environ.get("MBS_CONVERGENCE_NREF", "131072"))
+REF_N = int(os.environ.get("MBS_CONVERGENCE_REF_N", "32768"))
+REF_RUNS = int(os.environ.get("MBS_CONVERGENCE_REF_RUNS", "4"))
 N_MAX = int(os.environ.get("MBS_CONVERGENCE_NMAX", "32768"))
 NPHI = 96
 BAR = 0.01
@@ -114,10 +116,21 @@ def test_time_to_one_percent():
     wd = tempfile.mkdtemp(prefix="mbs_conv_")
     with open(os.path.join(wd, "theta.grid"), "w") as f:
         f.write("\n".join("%.10g" % t for t in THETA) + "\n")
-    print("converged reference: fp64 engine, %d orientations (seed 7)" % N_REF, flush=True)
-    ref, wall, ph = _run(wd, HIP_BIN, "gpu", N_REF, 7, "ref", "fp64")
-    print("reference run: wall %.1f s (trace %.1f s, diffraction %.1f s)" % (wall, *ph),
-          flush=True)
+    print("converged reference: fp64 engine, %d x %d orientations (seeds 7..%d)"
+          % (REF_RUNS, REF_N, 6 + REF_RUNS), flush=True)
+    parts = []
+    for r in range(REF_RUNS):
+        rows, wall, ph = _run(wd, HIP_BIN, "gpu", REF_N, 7 + r, "ref%d" % r, "fp64")
+        parts.append(rows)
+        print("reference run %d: wall %.1f s (trace %.1f s, diffraction %.1f s)"
+              % (r, wall, *ph), flush=True)
+    ref = np.mean(parts, axis=0)
+    if REF_RUNS > 1:
+        # spread of the independent scrambles: the reference's own error is
+        # about this divided by sqrt(REF_RUNS)
+        spread = max(M.convergence_errors(p, ref).max() for p in parts)
+        print("largest element error of one reference run against their mean: %.4f" % spread,
+              flush=True)
     sweeps = {name: _sweep(wd, name, prec, ref) for name, prec in (("gpu fp64", "fp64"),
                                                                    ("gpu mixed", "fp32"))}
     n_cpu = 64

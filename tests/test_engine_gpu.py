@@ -43,7 +43,7 @@ def _setup(pkg, fx):
 def _run(ctx, grid, batch, consts, flags=0, noshadow=False, target=None):
     ctx.set_grid(grid)
     ctx.upload(batch, consts)
-    ctx.set_target_blocks(target or 4096)
+    ctx.set_target_blocks(target or 65536)
     return ctx.mueller(flags, noshadow)
 
 
@@ -383,7 +383,7 @@ def test_graph_replay_and_autotune(pkg, oracle, ctx):
         assert len(table) >= 6
         m, _ = ctx.mueller(flags)
         assert M.grid_l2(m, ref, grid.theta) < tol
-    ctx.set_target_blocks(4096)  # back to the default shape for later tests
+    ctx.set_target_blocks(65536)  # back to the default shape for later tests
 
 
 def test_mlp_gradient_against_oracle(pkg, ctx):
