@@ -503,12 +503,13 @@ static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
     if constexpr (PREC == 2)
         diffract_mfma_kernel<NS, JONES, PER><<<blocks, 256, 0, s>>>(
-            a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.wb, a.vtx, a.vdd, a.vflag,
+            a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
+            a.vflag,
             a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
             a.per_orient);
     else
         diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
-            a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd,
+            a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd,
             a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
             a.per_orient);
 }
@@ -548,21 +549,23 @@ static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, int per, bool jones
     const int rt = 256;
     if (jones) {
         jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-            ctx->part.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res.p,
+            ctx->part.p, ctx->items.p, n_items, cells, ctx->n_az, ctx->weights.p, ctx->res.p,
             per ? ctx->per_orient.p : nullptr);
         HIP_TRY(ctx, hipGetLastError());
         if (ns) {
             jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->weights.p, ctx->res_ns.p,
+                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->n_az, ctx->weights.p,
+                ctx->res_ns.p,
                 nullptr);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {
         const int rb = (int)((mcount + rt - 1) / rt);
-        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->res.p);
+        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->n_az,
+                                                ctx->res.p);
         HIP_TRY(ctx, hipGetLastError());
         if (ns) {
-            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells,
+            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells, ctx->n_az,
                                                     ctx->res_ns.p);
             HIP_TRY(ctx, hipGetLastError());
         }
@@ -630,6 +633,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     a.n_items = n_items;
     a.n_tiles = n_tiles;
     a.cells = cells;
+    a.n_az = ctx->n_az;
     a.tab = ctx->tab.p;
     a.hdr = ctx->hdr.p;
     a.hdr32 = ctx->hdr32.p;

@@ -1,7 +1,8 @@
 // po_kernels.hip -- gfx950 kernels of the PO diffraction engine.
 //
-// Work mapping.  One lane owns one scattering direction (grid cell); a
-// 256-lane workgroup owns 256 consecutive cells of one "work item".  A work
+// Work mapping.  One lane owns one scattering direction (grid cell, in the
+// device's theta-major numbering, see abi_cell); a 256-lane workgroup owns 256
+// consecutive cells of one "work item".  A work
 // item is either a run of whole orientations (Mueller mode: the lane sums the
 // Jones matrices of every beam of an orientation in registers, converts to
 // Mueller, and accumulates weight*Mueller over the run) or one segment of one
@@ -38,14 +39,31 @@
 #define MBS_MIN_WAVES 4
 #endif
 #define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
-// The MFMA kernel is held to 4 waves per SIMD (128 VGPRs + AGPRs) when it
-// keeps one accumulator set; the no-shadow variant (two sets) gets 2.
+// The MFMA kernel is held to 5 waves per SIMD (~100 VGPRs + AGPRs) when it
+// keeps one accumulator set -- its 26.8 KB of LDS per workgroup also lets
+// five workgroups share a CU; the no-shadow variant (two sets) gets 2.
 #ifndef MBS_MFMA_WAVES
-#define MBS_MFMA_WAVES 4
+#define MBS_MFMA_WAVES 5
 #endif
 #define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : MBS_MFMA_WAVES, 8)))
 
 namespace mbs {
+
+// ---------------------------------------------------------------------------
+// Direction cells are numbered theta-major on the device (cell = t n_az + p:
+// the azimuths of one theta row are consecutive lanes), while the C ABI's
+// grids are azimuth-major (p (n_zen + 1) + t).  Every azimuth cell of a pole
+// row has the same direction, and a beam leaving along it meets singular edge
+// coefficients there in every such cell; theta-major order keeps those cells
+// in two waves instead of one wave per azimuth.  abi_cell maps a device cell
+// to its place in the grids the kernels hand back.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ size_t abi_cell(int cell, int n_az, int cells)
+{
+    const int t = cell / n_az;
+    const int p = cell - t * n_az;
+    return (size_t)p * (cells / n_az) + t;
+}
 
 // ---------------------------------------------------------------------------
 // Grid tables: per cell dir(3), u0(6), u1(6) as SoA [DT_COUNT][cells].
@@ -63,13 +81,15 @@ __global__ void grid_tables_kernel(const double *__restrict__ theta,
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= cells)
         return;
-    const int p = cell / n_zen1;
-    const int t = cell - p * n_zen1;
+    // device cell order is theta-major (see abi_cell); vf is azimuth-major
+    const int t = cell / n_az;
+    const int p = cell - t * n_az;
     double st, ct, sp, cp;
     sincos(theta[t], &st, &ct);
     sincos(p * az_step, &sp, &cp);
     const double d[3] = {st * cp, st * sp, -ct};
-    const double f[3] = {vf[3 * cell], vf[3 * cell + 1], vf[3 * cell + 2]};
+    const double *vfc = vf + 3 * ((size_t)p * n_zen1 + t);
+    const double f[3] = {vfc[0], vfc[1], vfc[2]};
     double vt[3] = {f[1] * d[2] - f[2] * d[1], f[2] * d[0] - f[0] * d[2],
                     f[0] * d[1] - f[1] * d[0]};
     const double l2 = vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2];
@@ -264,6 +284,22 @@ __device__ __forceinline__ double uniform_f64(double x)
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
+// Every vertex phase within 1e-3 rad: the cell takes the small-phase moment
+// form and its edge sum is not used, so the near-singular edges need not be
+// evaluated.  That matters: a beam leaving along the incident direction
+// (through parallel facets) meets it with A = B = 0 on the theta = 0 row,
+// whose azimuth cells lie in as many different waves.
+__device__ __forceinline__ bool all_phases_small(const double *__restrict__ vtx, int nv, double A,
+                                                 double B)
+{
+    bool small = true;
+    for (int e = 0; e < nv; ++e) {
+        const double *v = vtx + (size_t)e * VTX_STRIDE;
+        small = small && __builtin_fabs(__builtin_fma(A, v[V_KX], B * v[V_KY])) <= kSmallPhase;
+    }
+    return small;
+}
+
 // Lanes for which edge e enters the sum, as a wave mask: an edge with a
 // valid dx in the lanes of mB (|B| > |A|), with a valid dy in the others.
 // The flags are wave-uniform, so this is scalar mask arithmetic and the
@@ -288,7 +324,7 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
                                          double psi, bool useB, uint64_t mB,
                                          const double2 *__restrict__ tab, double k,
                                          double eps1, double s0, double c0, double &sp,
-                                         double &cp, double &sr, double &si, bool &small)
+                                         double &cp, double &sr, double &si, bool small)
 {
     const double *v = vtx + (size_t)e * VTX_STRIDE;
     // both slopes as scalars and both coefficients, then a per-lane select
@@ -299,7 +335,6 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
     double sn = s0, cn = c0;
     if (e + 1 < n) {
         const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-        small = small && __builtin_fabs(pn) <= kSmallPhase;
         sincos_table(pn + psi, tab, sn, cn);
     }
     const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
@@ -309,10 +344,10 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
     const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
     sr = __builtin_fma(cn - cp, inv, sr);
     si = __builtin_fma(sn - sp, inv, si);
-    if (__builtin_expect(__builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig), 0)) {
+    if (__builtin_expect(__builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig) && !small, 0)) {
         const double *w = vdd + (size_t)e * VDD_STRIDE;
         const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
-        const double2 q = stable_edge_quotient(cp, sp, dd, cc, k);
+        const double2 q = stable_edge_quotient(cp, sp, cn, sn, dd, cc, k);
         sr += q.x;
         si += q.y;
     }
@@ -386,17 +421,19 @@ __device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
             const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
             sr = __builtin_fma(cv[f] - cv[e], inv, sr);
             si = __builtin_fma(sv[f] - sv[e], inv, si);
-            if (__builtin_expect(__builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig), 0)) {
+            const bool rare = __builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig) && !small;
+            if (__builtin_expect(rare, 0)) {
                 const double *w = vdd + (size_t)e * VDD_STRIDE;
                 const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
-                const double2 q = stable_edge_quotient(cv[e], sv[e], useB ? dx : dy, cc, k);
+                const double2 q =
+                    stable_edge_quotient(cv[e], sv[e], cv[f], sv[f], useB ? dx : dy, cc, k);
                 sr += q.x;
                 si += q.y;
             }
         }
     } else {
+        small = all_phases_small(vtx, nv, A, B);
         const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-        small = __builtin_fabs(ph) <= kSmallPhase;
         double s0, c0;
         sincos_table(ph + psi, tab, s0, c0);
         double sp = s0, cp = c0;
@@ -498,7 +535,7 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
                                            double B, double psi, bool useB, int vmask,
                                            const float2 *__restrict__ tab, double k,
                                            double eps1, float s0, float c0, float &sp,
-                                           float &cp, float &sr, float &si, bool &small)
+                                           float &cp, float &sr, float &si, bool small)
 {
     const double *v = vtx + (size_t)e * VTX_STRIDE;
     const double syx = uniform_f64(v[V_SYX]), sxy = uniform_f64(v[V_SXY]);
@@ -506,7 +543,6 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     float sn = s0, cn = c0;
     if (e + 1 < n) {
         const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-        small = small && __builtin_fabs(pn) <= kSmallPhase;
         sincos_table_f32(pn + psi, tab, sn, cn);
     }
     // the edge coefficient is cancellation-prone: FP64
@@ -519,10 +555,11 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     const float inv = (valid && big) ? r : 0.0f;
     sr = __builtin_fmaf(cn - cp, inv, sr);
     si = __builtin_fmaf(sn - sp, inv, si);
-    if (__builtin_expect(valid && !big, 0)) {
+    if (__builtin_expect(valid && !big && !small, 0)) {
         const double *w = vdd + (size_t)e * VDD_STRIDE;
         const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
-        const double2 q = stable_edge_quotient((double)cp, (double)sp, dd, cc, k);
+        const double2 q =
+            stable_edge_quotient((double)cp, (double)sp, (double)cn, (double)sn, dd, cc, k);
         sr += (float)q.x;
         si += (float)q.y;
     }
@@ -624,18 +661,18 @@ __device__ __forceinline__ void aperture_mixed_nv(
             const float inv = (valid && big) ? r : 0.0f;
             sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
             si = __builtin_fmaf(sv[f] - sv[e], inv, si);
-            if (__builtin_expect(valid && !big, 0)) {
+            if (__builtin_expect(valid && !big && !small, 0)) {
                 const double *w = vdd + (size_t)e * VDD_STRIDE;
                 const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
-                const double2 q = stable_edge_quotient((double)cv[e], (double)sv[e],
-                                                       useB ? dx : dy, cc, k);
+                const double2 q = stable_edge_quotient((double)cv[e], (double)sv[e], (double)cv[f],
+                                                       (double)sv[f], useB ? dx : dy, cc, k);
                 sr += (float)q.x;
                 si += (float)q.y;
             }
         }
     } else {
+        small = all_phases_small(vtx, nv, A, B);
         const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
-        small = __builtin_fabs(ph) <= kSmallPhase;
         float s0, c0;
         sincos_table_f32(ph + psi, tab, s0, c0);
         float sp = s0, cp = c0;
@@ -731,7 +768,7 @@ __device__ __forceinline__ bool map_block(int n_items, int n_tiles, int &item,
 
 struct KernelArgs {
     const WorkItem *items;
-    int n_items, n_tiles, cells;
+    int n_items, n_tiles, cells, n_az;
     const double *tab;       // grid tables [DT_COUNT][cells]
     const double *hdr;       // [n_beams][HDR_STRIDE]
     const float *hdr32;      // [n_beams][HDR32_STRIDE]
@@ -779,7 +816,7 @@ __device__ __forceinline__ void flush_mueller(const T *j, double w, double *dst,
 // compiler to keep reading those through the scalar cache.
 template <int PREC, int NS, int JONES, int PER>
 __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
-    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
+    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,
     const float *__restrict__ hdr32, const double *__restrict__ vtx,
     const double *__restrict__ vdd, const int32_t *__restrict__ vflag,
@@ -869,7 +906,8 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
         } else {
             const double w = weights[o];
             const bool first = (o == it.o_begin);
-            double *per = PER ? per_orient + ((size_t)o * cells + cell) * 16 : nullptr;
+            double *per = PER ? per_orient + ((size_t)o * cells + abi_cell(cell, n_az, cells)) * 16
+                              : nullptr;
             flush_mueller(j, w, out + ((size_t)item * cells + cell) * 16, first, per);
             if constexpr (NS)
                 flush_mueller(jn, w, out_ns + ((size_t)item * cells + cell) * 16, first,
@@ -896,39 +934,45 @@ enum { G_LD = 25 };  // LDS row stride (floats) of the per-wave G transpose
 
 // G (this wave's accumulators) -> this lane's Jones sum.  Accumulator v of
 // lane l holds G[dir][n] with n = l % 32, dir = 8 (v/4) + 4 (l/32) + v % 4
-// (+32 for the second block).  Called by every thread of the block.
+// (+32 for the second block).  The two 32-direction blocks pass through one
+// 32-row LDS tile in turn (half the LDS of staging both at once, which lets
+// five workgroups share a CU).  Called by every thread of the block.
 __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1,
                                               float *__restrict__ sg,
                                               const float *__restrict__ su, float *j)
 {
     const int lane = threadIdx.x & 63;
     const int n = lane & 31;
-    if (n < WB_COLS) {
-        for (int v = 0; v < 16; ++v) {
-            const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            sg[i * G_LD + n] = g0[v];
-            sg[(32 + i) * G_LD + n] = g1[v];
+    for (int blk = 0; blk < 2; ++blk) {
+        const f32x16 &g = blk ? g1 : g0;
+        if (n < WB_COLS) {
+            for (int v = 0; v < 16; ++v) {
+                const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                sg[i * G_LD + n] = g[v];
+            }
         }
-    }
-    __syncthreads();
-    const float *g = sg + lane * G_LD;
-    for (int t = 0; t < 4; ++t) {  // t = 2 i + c
-        const int i = t >> 1, c = t & 1;
-        float re = 0.0f, im = 0.0f;
-        for (int m = 0; m < 6; ++m) {
-            const float um = su[(6 * i + m) * 256 + threadIdx.x];
-            re = __builtin_fmaf(um, g[4 * m + 2 * c], re);
-            im = __builtin_fmaf(um, g[4 * m + 2 * c + 1], im);
+        __syncthreads();
+        if ((lane >> 5) == blk) {
+            const float *gr = sg + (lane & 31) * G_LD;
+            for (int t = 0; t < 4; ++t) {  // t = 2 i + c
+                const int i = t >> 1, c = t & 1;
+                float re = 0.0f, im = 0.0f;
+                for (int m = 0; m < 6; ++m) {
+                    const float um = su[(6 * i + m) * 256 + threadIdx.x];
+                    re = __builtin_fmaf(um, gr[4 * m + 2 * c], re);
+                    im = __builtin_fmaf(um, gr[4 * m + 2 * c + 1], im);
+                }
+                j[2 * t] = re;
+                j[2 * t + 1] = im;
+            }
         }
-        j[2 * t] = re;
-        j[2 * t + 1] = im;
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 template <int NS, int JONES, int PER>
 __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
-    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
+    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,
     const float *__restrict__ wb, const double *__restrict__ vtx,
     const double *__restrict__ vdd, const int32_t *__restrict__ vflag,
@@ -947,7 +991,7 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
     __shared__ float2 s_tab[kTabN];
-    __shared__ float s_g[4 * 64 * G_LD];
+    __shared__ float s_g[4 * 32 * G_LD];
     __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
     for (int q = 0; q < 12; ++q)
         s_u[q * 256 + threadIdx.x] = (float)tab[(size_t)(DT_U0 + q) * cells + c];
@@ -957,7 +1001,7 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
         s_tab[threadIdx.x] = make_float2((float)tc, (float)ts);
     }
     __syncthreads();
-    float *sg = s_g + (threadIdx.x >> 6) * 64 * G_LD;
+    float *sg = s_g + (threadIdx.x >> 6) * 32 * G_LD;
     const int lane = threadIdx.x & 63;
     const WorkItem it = items[item];
     for (int o = it.o_begin; o < it.o_end; ++o) {
@@ -1013,7 +1057,8 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
         } else {
             const double w = weights[o];
             const bool first = (o == it.o_begin);
-            double *per = PER ? per_orient + ((size_t)o * cells + cell) * 16 : nullptr;
+            double *per = PER ? per_orient + ((size_t)o * cells + abi_cell(cell, n_az, cells)) * 16
+                              : nullptr;
             flush_mueller(j, w, out + ((size_t)item * cells + cell) * 16, first, per);
             if constexpr (NS)
                 flush_mueller(jn, w, out_ns + ((size_t)item * cells + cell) * 16, first,
@@ -1033,7 +1078,7 @@ __global__ void zero_kernel(double *__restrict__ p, size_t n)
 
 // Sum of per-item Mueller partials in item order: res[c] = sum_i part[i][c].
 __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_items,
-                                      int cells, double *__restrict__ res)
+                                      int cells, int n_az, double *__restrict__ res)
 {
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)cells * 16;
@@ -1042,7 +1087,7 @@ __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_ite
     double s = 0.0;
     for (int i = 0; i < n_items; ++i)
         s += part[(size_t)i * n + idx];
-    res[idx] = s;
+    res[abi_cell((int)(idx / 16), n_az, cells) * 16 + idx % 16] = s;
 }
 
 // Jones segments -> Mueller: items are ordered by orientation; for each
@@ -1050,7 +1095,7 @@ __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_ite
 // and accumulated with the orientation weight (TracerPOTotal.cpp:5985-5994).
 __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
                                         const WorkItem *__restrict__ items,
-                                        int n_items, int cells,
+                                        int n_items, int cells, int n_az,
                                         const double *__restrict__ weights,
                                         double *__restrict__ res,
                                         double *__restrict__ per_orient)
@@ -1058,6 +1103,7 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= cells)
         return;
+    const size_t out_cell = abi_cell(cell, n_az, cells);
     double m[16];
     for (int q = 0; q < 16; ++q)
         m[q] = 0.0;
@@ -1077,12 +1123,12 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
             for (int q = 0; q < 16; ++q)
                 po[q] = 0.0;
             add_mueller(po, j, weights[o]);
-            double *dst = per_orient + ((size_t)o * cells + cell) * 16;
+            double *dst = per_orient + ((size_t)o * cells + out_cell) * 16;
             for (int q = 0; q < 16; ++q)
                 dst[q] = po[q];
         }
     }
-    double *dst = res + (size_t)cell * 16;
+    double *dst = res + out_cell * 16;
     for (int q = 0; q < 16; ++q)
         dst[q] = m[q];
 }

@@ -44,7 +44,7 @@ enum { D_DX = 0, D_DY = 1, VDD_STRIDE = 2 };
 // from this vertex has a usable dx (enters the |B| > |A| sum) / dy.
 enum { EDGE_VX = 1, EDGE_VY = 2 };
 
-// Grid tables, SoA [DT_COUNT][cells].
+// Grid tables, SoA [DT_COUNT][cells], cells in the device's theta-major order.
 enum { DT_DIR = 0, DT_U0 = 3, DT_U1 = 9, DT_COUNT = 15 };
 
 // A unit of work: orientations [o_begin, o_end) whole (Mueller mode), or the

@@ -42,6 +42,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -407,10 +408,11 @@ ThreadContexts::Entry *Context(int device, GridSpec &g, bool &ok)
 // Diffracts one packed batch and adds the result into localM (and into
 // *localMns when it is given).
 bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
-              Arr2D &localM, Arr2D *localMns)
+              Arr2D &localM, Arr2D *localMns, double packMs)
 {
     const int nAz = e.nAz, nZen = e.nZen;
     const bool timing = EnvIs("MBS_GPU_TIMING", "1");
+    const auto t0 = std::chrono::steady_clock::now();
     mbs_set_timing(e.ctx, timing ? 1 : 0);
     const mbs_beam_batch view = pk.View();
     int rc = mbs_upload_beams(e.ctx, &view, &c);
@@ -427,15 +429,18 @@ bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts 
         std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(e.ctx));
         return false;
     }
+    const double callMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (timing)
     {
         mbs_timing tm;
         if (mbs_get_timing(e.ctx, &tm) == MBS_OK)
             std::fprintf(stderr,
-                         "GPU timing: %d orientations, %zu beams: upload %.3f ms, kernel %.3f "
-                         "ms, reduce %.3f ms (%d work items, %d workgroups)\n",
-                         (int)pk.weights.size(), pk.nv.size(), tm.upload_ms, tm.kernel_ms,
-                         tm.reduce_ms, tm.n_items, tm.n_blocks);
+                         "GPU timing: %d orientations, %zu beams: pack %.3f ms, upload %.3f ms, "
+                         "kernel %.3f ms, reduce %.3f ms, engine calls %.3f ms (%d work items, "
+                         "%d workgroups)\n",
+                         (int)pk.weights.size(), pk.nv.size(), packMs, tm.upload_ms,
+                         tm.kernel_ms, tm.reduce_ms, callMs, tm.n_items, tm.n_blocks);
     }
     for (int p = 0; p < nAz; ++p)
         for (int t = 0; t <= nZen; ++t)
@@ -488,9 +493,12 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
         return true;
 
     PackedBatch pk;
+    const auto tp = std::chrono::steady_clock::now();
     if (!Pack(prepared, start, nOrient, PackScale(scale, waveIndex, AbsorptionCoefficient()),
               pk))
         return false;
+    const double packMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
     if (pk.nv.empty())
         return true;
     GridSpec g = MakeGrid(m_sphere);
@@ -500,7 +508,7 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
         return ok;
     const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
                                        m_legacySign);
-    return RunBatch(*e, pk, c, localM, ComputeNoShadow() ? &localM_noshadow : nullptr);
+    return RunBatch(*e, pk, c, localM, ComputeNoShadow() ? &localM_noshadow : nullptr, packMs);
 }
 
 bool HandlerPO::HandleOrientationsToLocalGpu(const std::vector<PreparedOrientation> &prepared,
@@ -561,10 +569,14 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
     PackedBatch pk;
     for (size_t s = 0; s < scales.size(); ++s)
     {
+        const auto tp = std::chrono::steady_clock::now();
         if (!Pack(prepared, start, nOrient,
                   PackScale(scales[s], waveIndex, AbsorptionCoefficient()), pk))
             return false;
-        if (!pk.nv.empty() && !RunBatch(*e, pk, c, localMs[s], nullptr))
+        const double packMs =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp)
+                .count();
+        if (!pk.nv.empty() && !RunBatch(*e, pk, c, localMs[s], nullptr, packMs))
             return false;
     }
     return true;
