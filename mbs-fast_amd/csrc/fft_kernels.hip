@@ -35,16 +35,21 @@
 
 namespace mbs {
 
-// Workgroups: the row kernel holds FFT_ROWS rows (2 workgroups of 68 KB LDS
-// per CU at n = 2048) with 512 threads; the column kernel FFT_COLS columns
-// with FFT_COL_THREADS threads (8 columns: one 135 KB workgroup of 1024
-// threads per CU; 4 columns: two 68 KB workgroups of 512).
+// Workgroups: 128 threads per row or column.  The row kernel holds FFT_ROWS
+// rows: one row (17 KB of LDS at n = 2048, nine workgroups per CU) ran the
+// row pass 16 % faster than four (68 KB, two per CU), whose barriers stall
+// more of the CU at once.  The column kernel holds FFT_COLS columns (8: one
+// 135 KB workgroup of 1024 threads per CU; 4 columns, two 68 KB workgroups,
+// measured the same).
 #ifndef MBS_FFT_COLS
 #define MBS_FFT_COLS 8
 #endif
+#ifndef MBS_FFT_ROWS
+#define MBS_FFT_ROWS 1
+#endif
 enum {
-    FFT_ROW_THREADS = 512,
-    FFT_ROWS = 4,
+    FFT_ROWS = MBS_FFT_ROWS,
+    FFT_ROW_THREADS = MBS_FFT_ROWS * 128,
     FFT_COLS = MBS_FFT_COLS,
     FFT_COL_THREADS = MBS_FFT_COLS * 128
 };
