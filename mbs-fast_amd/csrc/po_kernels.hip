@@ -40,7 +40,7 @@
 #endif
 #define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
 // The MFMA kernel is held to 5 waves per SIMD (~100 VGPRs + AGPRs) when it
-// keeps one accumulator set -- its 26.8 KB of LDS per workgroup also lets
+// keeps one accumulator set -- its 28.8 KB of LDS per workgroup also lets
 // five workgroups share a CU; the no-shadow variant (two sets) gets 2.
 #ifndef MBS_MFMA_WAVES
 #define MBS_MFMA_WAVES 5
@@ -209,7 +209,35 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     h[H_MOM + 2] = k * k * mXX;
     h[H_MOM + 3] = 2.0 * k * k * mXY;
     h[H_MOM + 4] = k * k * mYY;
-    h[H_MOM + 5] = 0.0;
+    // Small-phase bound.  The vertex phases k (A x + B y) spread over at
+    // least k |(A, B)| w, w the least width of the vertex set, so some phase
+    // exceeds 1e-3 rad once max(|A|, |B|) > 2e-3 / (k w); the kernels test
+    // the vertices one by one only below that (2 % margin for rounding).  w
+    // is the least, over all vertex pairs, of the other vertices' spread
+    // across the line through the pair: every edge of the convex hull is such
+    // a pair, and the hull's least width lies across one of its edges.
+    double wmin = __builtin_inf();
+    if (nv <= 32) {
+        for (int i = 0; i < nv; ++i) {
+            const double xi = vx[v0 + i], yi = vy[v0 + i];
+            for (int j = i + 1; j < nv; ++j) {
+                const double ex = vx[v0 + j] - xi, ey = vy[v0 + j] - yi;
+                const double len = sqrt(ex * ex + ey * ey);
+                if (!(len > 0.0))
+                    continue;
+                double lo = 0.0, hi = 0.0;
+                for (int l = 0; l < nv; ++l) {
+                    const double c = ex * (vy[v0 + l] - yi) - ey * (vx[v0 + l] - xi);
+                    lo = fmin(lo, c);
+                    hi = fmax(hi, c);
+                }
+                wmin = fmin(wmin, (hi - lo) / len);
+            }
+        }
+    }
+    // a degenerate or very large polygon keeps the per-vertex test everywhere
+    h[H_PHB] = (wmin > 0.0 && wmin < __builtin_inf()) ? 2.04e-3 / (k * wmin)
+                                                      : __builtin_inf();
     // rotation vectors, pre-scaled by 1/2 (exact)
     for (int q = 0; q < 3; ++q) {
         h[H_T0 + q] = 0.5 * s[13 + q];      // NT
@@ -300,6 +328,17 @@ __device__ __forceinline__ bool all_phases_small(const double *__restrict__ vtx,
     return small;
 }
 
+template <int NV>
+__device__ __forceinline__ bool all_phases_small_nv(const double *kx, const double *ky, double A,
+                                                    double B)
+{
+    bool small = true;
+#pragma unroll
+    for (int e = 0; e < NV; ++e)
+        small = small && __builtin_fabs(__builtin_fma(A, kx[e], B * ky[e])) <= kSmallPhase;
+    return small;
+}
+
 // Lanes for which edge e enters the sum, as a wave mask: an edge with a
 // valid dx in the lanes of mB (|B| > |A|), with a valid dy in the others.
 // The flags are wave-uniform, so this is scalar mask arithmetic and the
@@ -334,8 +373,8 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
     const int f = __builtin_amdgcn_readfirstlane(vflag[e]);
     double sn = s0, cn = c0;
     if (e + 1 < n) {
-        const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-        sincos_table(pn + psi, tab, sn, cn);
+        const double xn = __builtin_fma(B, v[VTX_STRIDE + V_KY], psi);
+        sincos_table(__builtin_fma(A, v[VTX_STRIDE + V_KX], xn), tab, sn, cn);
     }
     const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
     const uint64_t mvalid = edge_valid_mask(f, mB);
@@ -353,6 +392,38 @@ __device__ __forceinline__ void edge_f64(int e, int n, const double *__restrict_
     }
     sp = sn;
     cp = cn;
+}
+
+// The edge sum of aperture_f64: both coefficients from scalar slopes, then
+// a per-lane select; the lanes an edge enters as scalar masks.
+template <int NV>
+__device__ __forceinline__ void edge_sum_f64_nv(
+    const double *syx, const double *sxy, const int *fl, const double *sv, const double *cv,
+    const double *__restrict__ vdd, double A, double B, bool useB, uint64_t mB, double k,
+    double eps1, bool small, double &sr, double &si)
+{
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+        const int f = e + 1 < NV ? e + 1 : 0;
+        const double cx = __builtin_fma(syx[e], B, A);
+        const double cy = __builtin_fma(sxy[e], A, B);
+        const double cc = useB ? cx : cy;
+        const uint64_t mvalid = edge_valid_mask(fl[e], mB);
+        const uint64_t mbig = __builtin_amdgcn_ballot_w64(__builtin_fabs(cc) > eps1);
+        const double r = rcp_f64_fast(cc);
+        const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
+        sr = __builtin_fma(cv[f] - cv[e], inv, sr);
+        si = __builtin_fma(sv[f] - sv[e], inv, si);
+        const bool rare = __builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig) && !small;
+        if (__builtin_expect(rare, 0)) {
+            const double *w = vdd + (size_t)e * VDD_STRIDE;
+            const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
+            const double2 q =
+                stable_edge_quotient(cv[e], sv[e], cv[f], sv[f], useB ? dx : dy, cc, k);
+            sr += q.x;
+            si += q.y;
+        }
+    }
 }
 
 // Aperture integral F(beam, direction) / complWave in FP64 (A4).  NV > 0
@@ -387,19 +458,22 @@ __device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
             kx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KX]);
             ky[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KY]);
         }
+        const double phb = uniform_f64(h[H_PHB]);
         double dv[NV];
         double2 ev[NV];
-        small = true;
 #pragma unroll
-        for (int e = 0; e < NV; ++e) {
-            const double p = __builtin_fma(A, kx[e], B * ky[e]);
-            small = small && __builtin_fabs(p) <= kSmallPhase;
-            sincos_reduce(p + psi, tab, dv[e], ev[e]);
-        }
+        for (int e = 0; e < NV; ++e)
+            sincos_reduce(__builtin_fma(A, kx[e], __builtin_fma(B, ky[e], psi)), tab, dv[e],
+                          ev[e]);
         double sv[NV], cv[NV];
 #pragma unroll
         for (int e = 0; e < NV; ++e)
             sincos_finish(dv[e], ev[e], sv[e], cv[e]);
+        // the vertex phases one by one only where the beam's bound leaves
+        // the small-phase form possible, after the table reads are consumed
+        small = false;
+        if (__builtin_expect(__builtin_fabs(useB ? B : -A) <= phb, 0))
+            small = all_phases_small_nv<NV>(kx, ky, A, B);
         double syx[NV], sxy[NV];
         int fl[NV];
 #pragma unroll
@@ -408,34 +482,13 @@ __device__ __forceinline__ bool aperture_f64(const double *__restrict__ h,
             sxy[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SXY]);
             fl[e] = __builtin_amdgcn_readfirstlane(vflag[e]);
         }
-#pragma unroll
-        for (int e = 0; e < NV; ++e) {
-            const int f = e + 1 < NV ? e + 1 : 0;
-            // both coefficients from scalar slopes, then a per-lane select
-            const double cx = __builtin_fma(syx[e], B, A);
-            const double cy = __builtin_fma(sxy[e], A, B);
-            const double cc = useB ? cx : cy;
-            const uint64_t mvalid = edge_valid_mask(fl[e], mB);
-            const uint64_t mbig = __builtin_amdgcn_ballot_w64(__builtin_fabs(cc) > eps1);
-            const double r = rcp_f64_fast(cc);
-            const double inv = __builtin_amdgcn_inverse_ballot_w64(mvalid & mbig) ? r : 0.0;
-            sr = __builtin_fma(cv[f] - cv[e], inv, sr);
-            si = __builtin_fma(sv[f] - sv[e], inv, si);
-            const bool rare = __builtin_amdgcn_inverse_ballot_w64(mvalid & ~mbig) && !small;
-            if (__builtin_expect(rare, 0)) {
-                const double *w = vdd + (size_t)e * VDD_STRIDE;
-                const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
-                const double2 q =
-                    stable_edge_quotient(cv[e], sv[e], cv[f], sv[f], useB ? dx : dy, cc, k);
-                sr += q.x;
-                si += q.y;
-            }
-        }
+        edge_sum_f64_nv<NV>(syx, sxy, fl, sv, cv, vdd, A, B, useB, mB, k, eps1, small, sr, si);
     } else {
-        small = all_phases_small(vtx, nv, A, B);
-        const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+        small = false;
+        if (__builtin_expect(__builtin_fabs(useB ? B : -A) <= h[H_PHB], 0))
+            small = all_phases_small(vtx, nv, A, B);
         double s0, c0;
-        sincos_table(ph + psi, tab, s0, c0);
+        sincos_table(__builtin_fma(A, vtx[V_KX], __builtin_fma(B, vtx[V_KY], psi)), tab, s0, c0);
         double sp = s0, cp = c0;
         for (int e = 0; e < nv; ++e)
             edge_f64(e, nv, vtx, vdd, vflag, A, B, psi, useB, mB, tab, k, eps1, s0,
@@ -542,8 +595,8 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     const int f = __builtin_amdgcn_readfirstlane(vflag[e]);
     float sn = s0, cn = c0;
     if (e + 1 < n) {
-        const double pn = __builtin_fma(A, v[VTX_STRIDE + V_KX], B * v[VTX_STRIDE + V_KY]);
-        sincos_table_f32(pn + psi, tab, sn, cn);
+        const double xn = __builtin_fma(B, v[VTX_STRIDE + V_KY], psi);
+        sincos_table_f32(__builtin_fma(A, v[VTX_STRIDE + V_KX], xn), tab, sn, cn);
     }
     // the edge coefficient is cancellation-prone: FP64
     const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
@@ -594,6 +647,61 @@ __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__
     out_i = bad ? 0.0f : fi;
 }
 
+// The edge sum of aperture_mixed_nv.  Both coefficients come from scalar
+// slopes, then a per-lane select of the result (selecting the slope first
+// costs two moves and a select per half of the FP64 operand).  A separate
+// path for waves whose lanes all take one branch of the reference's sum --
+// one coefficient per edge, invalid edges skipped by a scalar branch --
+// measured slower here and no faster in the FP64 sum (the code triples).  Near-singular edges (|C| <= eps1,
+// their term below zero) are evaluated in one pass after the sum, taken
+// only when some lane has one.
+template <int NV>
+__device__ __forceinline__ void edge_sum_mixed_nv(
+    const double *syx, const double *sxy, const int *fl, const float *sv, const float *cv,
+    const double *__restrict__ vdd, double A, double B, bool useB, int vmask, double k,
+    double eps1, bool small, float &sr, float &si)
+{
+    bool rare[NV];
+    bool any_rare = false;
+#pragma unroll
+    for (int e = 0; e < NV; ++e) {
+        const int f = e + 1 < NV ? e + 1 : 0;
+        const double cx = __builtin_fma(syx[e], B, A);
+        const double cy = __builtin_fma(sxy[e], A, B);
+        const double cc = useB ? cx : cy;
+        const bool valid = (fl[e] & vmask) != 0;
+        const bool big = __builtin_fabs(cc) > eps1;
+        // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
+        // carry more rounding than that
+        const float r = __builtin_amdgcn_rcpf((float)cc);
+        const float inv = (valid && big) ? r : 0.0f;
+        sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
+        si = __builtin_fmaf(sv[f] - sv[e], inv, si);
+        // near-singular lanes as scalar mask arithmetic (a second vector
+        // compare for !big otherwise)
+        rare[e] = __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(valid) &
+                                                      ~__builtin_amdgcn_ballot_w64(big));
+        any_rare = any_rare || rare[e];
+    }
+    if (__builtin_expect(any_rare && !small, 0)) {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            if (rare[e]) {
+                const int f = e + 1 < NV ? e + 1 : 0;
+                const double cc =
+                    useB ? __builtin_fma(syx[e], B, A) : __builtin_fma(sxy[e], A, B);
+                const double *w = vdd + (size_t)e * VDD_STRIDE;
+                const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
+                const double2 q = stable_edge_quotient((double)cv[e], (double)sv[e],
+                                                       (double)cv[f], (double)sv[f],
+                                                       useB ? dx : dy, cc, k);
+                sr += (float)q.x;
+                si += (float)q.y;
+            }
+        }
+    }
+}
+
 // Aperture integral F(beam, direction) / complWave, mixed precision: A, B,
 // the phases and their reduction, the edge coefficients and the moments in
 // FP64; phasors and edge sums in FP32.
@@ -627,6 +735,8 @@ __device__ __forceinline__ void aperture_mixed_nv(
         }
         float dv[NV];
         float2 ev[NV];
+        // every vertex tested here: the per-beam bound that spares the FP64
+        // kernel this test measured slower in this one
         small = true;
 #pragma unroll
         for (int e = 0; e < NV; ++e) {
@@ -646,35 +756,15 @@ __device__ __forceinline__ void aperture_mixed_nv(
             sxy[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_SXY]);
             fl[e] = __builtin_amdgcn_readfirstlane(vflag[e]);
         }
-#pragma unroll
-        for (int e = 0; e < NV; ++e) {
-            const int f = e + 1 < NV ? e + 1 : 0;
-            // both coefficients from scalar slopes, then a per-lane select of
-            // the result (selecting the slope first costs two moves and a
-            // select per half of the FP64 operand)
-            const double cx = __builtin_fma(syx[e], B, A);
-            const double cy = __builtin_fma(sxy[e], A, B);
-            const double cc = useB ? cx : cy;
-            const bool valid = (fl[e] & vmask) != 0;
-            const bool big = __builtin_fabs(cc) > eps1;
-            const float r = __builtin_amdgcn_rcpf((float)cc);
-            const float inv = (valid && big) ? r : 0.0f;
-            sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
-            si = __builtin_fmaf(sv[f] - sv[e], inv, si);
-            if (__builtin_expect(valid && !big && !small, 0)) {
-                const double *w = vdd + (size_t)e * VDD_STRIDE;
-                const double dx = uniform_f64(w[D_DX]), dy = uniform_f64(w[D_DY]);
-                const double2 q = stable_edge_quotient((double)cv[e], (double)sv[e], (double)cv[f],
-                                                       (double)sv[f], useB ? dx : dy, cc, k);
-                sr += (float)q.x;
-                si += (float)q.y;
-            }
-        }
+        edge_sum_mixed_nv<NV>(syx, sxy, fl, sv, cv, vdd, A, B, useB, vmask, k, eps1, small, sr,
+                              si);
     } else {
-        small = all_phases_small(vtx, nv, A, B);
-        const double ph = __builtin_fma(A, vtx[V_KX], B * vtx[V_KY]);
+        small = false;
+        if (__builtin_expect(__builtin_fabsf((float)(useB ? B : -A)) <= (float)h[H_PHB], 0))
+            small = all_phases_small(vtx, nv, A, B);
         float s0, c0;
-        sincos_table_f32(ph + psi, tab, s0, c0);
+        sincos_table_f32(__builtin_fma(A, vtx[V_KX], __builtin_fma(B, vtx[V_KY], psi)), tab, s0,
+                         c0);
         float sp = s0, cp = c0;
         for (int e = 0; e < nv; ++e)
             edge_mixed(e, nv, vtx, vdd, vflag, A, B, psi, useB, vmask, tab, k, eps1,
@@ -839,18 +929,19 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     using U2 = typename std::conditional<PREC == 1, float2, double2>::type;
     // LDS: this lane's rotation vectors u0, u1 ([12][256], read once per beam
     // after the vertex loop instead of occupying 12-24 VGPRs through it) and
-    // the 256-entry phasor table e^{i 2 pi j/256}
+    // the phasor table e^{i 2 pi j/TN} (512 FP32 entries, 256 FP64 ones)
+    constexpr int TN = PREC == 1 ? kTabN32 : kTabN;
     __shared__ U s_u[12 * 256];
-    __shared__ U2 s_tab[kTabN];
+    __shared__ U2 s_tab[TN];
     for (int q = 0; q < 12; ++q)
         s_u[q * 256 + threadIdx.x] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
-    {
+    for (int i = threadIdx.x; i < TN; i += blockDim.x) {
         double ts, tc;
-        sincospi((double)threadIdx.x / 128.0, &ts, &tc);
+        sincospi((double)i / (TN / 2), &ts, &tc);
         if constexpr (PREC == 1)
-            s_tab[threadIdx.x] = make_float2((float)tc, (float)ts);
+            s_tab[i] = make_float2((float)tc, (float)ts);
         else
-            s_tab[threadIdx.x] = make_double2(tc, ts);
+            s_tab[i] = make_double2(tc, ts);
     }
     __syncthreads();
     const WorkItem it = items[item];
@@ -990,15 +1081,15 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     double d[3];
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
-    __shared__ float2 s_tab[kTabN];
+    __shared__ float2 s_tab[kTabN32];
     __shared__ float s_g[4 * 32 * G_LD];
     __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
     for (int q = 0; q < 12; ++q)
         s_u[q * 256 + threadIdx.x] = (float)tab[(size_t)(DT_U0 + q) * cells + c];
-    {
+    for (int i = threadIdx.x; i < kTabN32; i += blockDim.x) {
         double ts, tc;
-        sincospi((double)threadIdx.x / 128.0, &ts, &tc);
-        s_tab[threadIdx.x] = make_float2((float)tc, (float)ts);
+        sincospi((double)i / (kTabN32 / 2), &ts, &tc);
+        s_tab[i] = make_float2((float)tc, (float)ts);
     }
     __syncthreads();
     float *sg = s_g + (threadIdx.x >> 6) * 32 * G_LD;

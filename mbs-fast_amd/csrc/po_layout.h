@@ -15,7 +15,8 @@ enum {
     H_KC = 8,    // -k * centre (3), zero for external beams
     H_EXT = 11,  // 1.0 for external beams
     H_AL = 12,   // small-phase area limit / complWave (complex)
-    H_MOM = 14,  // k mX, k mY, k^2 mXX, 2 k^2 mXY, k^2 mYY, pad (6)
+    H_MOM = 14,  // k mX, k mY, k^2 mXX, 2 k^2 mXY, k^2 mYY (5)
+    H_PHB = 19,  // max(|A|,|B|) above which some vertex phase exceeds 1e-3 rad
     H_T0 = 20,   // 1/2 (NT, nxDT)   (6)
     H_T1 = 26,   // 1/2 (NP, nxDP)   (6)
     H_JP = 32,   // complWave * J_phased (8)

@@ -6,7 +6,8 @@
 // Taylor polynomials (see sincos_table below).  Phases reach ~1e4 rad for the
 // largest particles; the reduction is exact to an ulp for |x| < 2^20.
 // The FP32 variant used by the mixed-precision kernels reduces in FP64 the
-// same way and evaluates shorter polynomials in FP32 (error ~1 ulp of FP32).
+// same way against a 512-entry table and needs only first- and second-order
+// corrections in FP32 (error ~1 ulp of FP32).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,15 +44,25 @@ constexpr double kTabInvStep = 40.743665431525205956;  // 256 / (2 pi)
 constexpr double kTabStep1 = 3.141592653589793116 / 128.0;
 constexpr double kTabStep2 = 1.2246467991473532e-16 / 128.0;
 
+// j = rint(x 256/2pi) by the addition of 1.5 2^52, whose sum carries j in its
+// low 32 bits (two's complement) and is j + 1.5 2^52 exactly for |j| < 2^51:
+// one FMA for the product and the rounding, one subtraction for j as a double.
+constexpr double kRoundMagic = 6755399441055744.0;  // 1.5 * 2^52
+__device__ __forceinline__ uint32_t low_bits(double t)
+{
+    return (uint32_t)__builtin_bit_cast(uint64_t, t);
+}
+
 // In two halves, as for the FP32 variant below: reduction and table read,
 // then polynomials and product.
 __device__ __forceinline__ void sincos_reduce(double x, const double2 *__restrict__ tab,
                                               double &d, double2 &e)
 {
-    const double j = __builtin_rint(x * kTabInvStep);
+    const double t = __builtin_fma(x, kTabInvStep, kRoundMagic);
+    const double j = t - kRoundMagic;
     d = __builtin_fma(-j, kTabStep1, x);
     d = __builtin_fma(-j, kTabStep2, d);
-    e = tab[(int)j & (kTabN - 1)];
+    e = tab[low_bits(t) & (kTabN - 1)];
 }
 
 __device__ __forceinline__ void sincos_finish(double d, double2 e, double &s, double &c)
@@ -76,30 +87,39 @@ __device__ __forceinline__ void sincos_table(double x, const double2 *__restrict
     sincos_finish(d, e, s, c);
 }
 
-// FP32 result from an FP64 argument, same table (converted to FP32 values),
-// in two halves so that callers can issue the table reads of several
-// arguments before using any of them: the FP64 reduction and table read,
-// then the FP32 polynomials and the product with the table entry.
+// FP32 result from an FP64 argument against a 512-entry FP32 table of
+// e^{2 pi i j / 512}, in two halves so that callers can issue the table
+// reads of several arguments before using any of them: the FP64 reduction
+// and table read, then the FP32 correction and the product with the table
+// entry.  The reduction is exact as above (kTab32Step1 = double(pi)/256 is
+// again an exact scaling, and |x - j kTab32Step1| < 2^-6).  Against the
+// 256-entry table the index costs one more instruction (a mask and a shift
+// instead of one shift of the low byte) and the correction two fewer.
+constexpr int kTabN32 = 512;
+constexpr double kTab32InvStep = 81.487330863050411912;  // 512 / (2 pi)
+constexpr double kTab32Step1 = 3.141592653589793116 / 256.0;
+constexpr double kTab32Step2 = 1.2246467991473532e-16 / 256.0;
+
 __device__ __forceinline__ void sincos_reduce_f32(double x, const float2 *__restrict__ tab,
                                                   float &d, float2 &e)
 {
-    const double j = __builtin_rint(x * kTabInvStep);
-    double dd = __builtin_fma(-j, kTabStep1, x);
-    dd = __builtin_fma(-j, kTabStep2, dd);
+    const double t = __builtin_fma(x, kTab32InvStep, kRoundMagic);
+    const double j = t - kRoundMagic;
+    double dd = __builtin_fma(-j, kTab32Step1, x);
+    dd = __builtin_fma(-j, kTab32Step2, dd);
     d = (float)dd;
-    e = tab[(int)j & (kTabN - 1)];
+    e = tab[low_bits(t) & (kTabN32 - 1)];
 }
 
-// |d| <= pi/256 = 0.0123, so sin d = d - d^3/6 and cos d = 1 - d^2/2 are
-// exact to FP32: the next terms, d^5/120 < 2.4e-12 and d^4/24 < 9.4e-10,
-// lie far below half an ulp of the results (3e-8 near 1).
+// |d| <= pi/512 = 6.14e-3, so e^{i d} = (1 - d^2/2) + i d to FP32 accuracy:
+// the dropped terms are below 3.9e-8 (d^3/6) and 6e-11 (d^4/24), within an
+// ulp of FP32 near 1 (6e-8) -- the phasor rounding the mixed kernels carry
+// anyway.
 __device__ __forceinline__ void sincos_finish_f32(float d, float2 e, float &s, float &c)
 {
-    const float z = d * d;
-    const float sd = __builtin_fmaf(d * z, -1.0f / 6.0f, d);
-    const float cd = __builtin_fmaf(z, -0.5f, 1.0f);
-    s = __builtin_fmaf(e.y, cd, e.x * sd);
-    c = __builtin_fmaf(e.x, cd, -(e.y * sd));
+    const float cd = __builtin_fmaf(d * d, -0.5f, 1.0f);
+    s = __builtin_fmaf(e.y, cd, e.x * d);
+    c = __builtin_fmaf(e.x, cd, -(e.y * d));
 }
 
 __device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restrict__ tab,

@@ -86,26 +86,6 @@ bool DeviceFailure(const char *what, const char *message)
     std::exit(2);
 }
 
-// Per-thread engine contexts, one per device, destroyed at thread exit.
-struct ThreadContexts
-{
-    struct Entry
-    {
-        mbs_ctx *ctx = nullptr;
-        std::vector<double> theta, vf;
-        double azStep = 0.0;
-        int nAz = -1, nZen = -1;
-    };
-    std::map<int, Entry> byDevice;
-    ~ThreadContexts()
-    {
-        for (auto &kv : byDevice)
-            if (kv.second.ctx)
-                mbs_ctx_destroy(kv.second.ctx);
-    }
-};
-thread_local ThreadContexts t_contexts;
-
 // Flat arrays of one batch, in the C ABI's layout (mbs_beam_batch).
 struct PackedBatch
 {
@@ -133,6 +113,29 @@ struct PackedBatch
         return b;
     }
 };
+
+// Per-thread engine contexts, one per device, destroyed at thread exit.
+struct ThreadContexts
+{
+    struct Entry
+    {
+        mbs_ctx *ctx = nullptr;
+        std::vector<double> theta, vf;
+        double azStep = 0.0;
+        int nAz = -1, nZen = -1;
+        // host staging of the packed batch, kept so that later calls reuse
+        // its pages instead of allocating and faulting in ~2 GB per batch
+        PackedBatch pk;
+    };
+    std::map<int, Entry> byDevice;
+    ~ThreadContexts()
+    {
+        for (auto &kv : byDevice)
+            if (kv.second.ctx)
+                mbs_ctx_destroy(kv.second.ctx);
+    }
+};
+thread_local ThreadContexts t_contexts;
 
 // A rescaled beam set (the multi-size path): the particle, and with it every
 // beam, is scaled by `scale` about the origin.  As the reference's pack does
@@ -492,7 +495,12 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
     if (nOrient == 0 || m_sphere.nAzimuth * (m_sphere.nZenith + 1) == 0)
         return true;
 
-    PackedBatch pk;
+    GridSpec g = MakeGrid(m_sphere);
+    bool ok = false;
+    ThreadContexts::Entry *e = Context(device, g, ok);
+    if (!e)
+        return ok;
+    PackedBatch &pk = e->pk;
     const auto tp = std::chrono::steady_clock::now();
     if (!Pack(prepared, start, nOrient, PackScale(scale, waveIndex, AbsorptionCoefficient()),
               pk))
@@ -501,11 +509,6 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
     if (pk.nv.empty())
         return true;
-    GridSpec g = MakeGrid(m_sphere);
-    bool ok = false;
-    ThreadContexts::Entry *e = Context(device, g, ok);
-    if (!e)
-        return ok;
     const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
                                        m_legacySign);
     return RunBatch(*e, pk, c, localM, ComputeNoShadow() ? &localM_noshadow : nullptr, packMs);
@@ -566,7 +569,7 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
         return ok;
     const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
                                        m_legacySign);
-    PackedBatch pk;
+    PackedBatch &pk = e->pk;
     for (size_t s = 0; s < scales.size(); ++s)
     {
         const auto tp = std::chrono::steady_clock::now();

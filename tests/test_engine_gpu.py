@@ -275,6 +275,86 @@ def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
     check_fp64(m, o, grid.theta, "synthetic D=50 FP64 vs oracle")
 
 
+def _aimed_at_poles(abi, src, seed):
+    """One orientation holding every beam of `src`, each rigidly rotated so
+    that it leaves along theta = 0 (even beams) or theta = pi (odd beams),
+    then by a random angle about that axis."""
+    rng = np.random.default_rng(seed)
+    bs = src.beam_scalars.copy()
+    for b in range(src.n_beams):
+        t = np.array([0.0, 0.0, -1.0 if b % 2 == 0 else 1.0])
+        d = bs[b, abi.B_DIR:abi.B_DIR + 3] / np.linalg.norm(bs[b, abi.B_DIR:abi.B_DIR + 3])
+        v, c = np.cross(d, t), float(d @ t)
+        K = np.array([[0, -v[2], v[1]], [v[2], 0, -v[0]], [-v[1], v[0], 0]])
+        R = np.eye(3) + K + K @ K / (1 + c) if c > -0.5 else np.diag([1.0, -1.0, -1.0]) @ (
+            np.eye(3) - K + K @ K / (1 - c))   # d -> -t first, then pi about x
+        a = rng.uniform(0, 2 * np.pi)
+        Kt = np.array([[0, -t[2], t[1]], [t[2], 0, -t[0]], [-t[1], t[0], 0]])
+        R = (np.cos(a) * np.eye(3) + np.sin(a) * Kt + (1 - np.cos(a)) * np.outer(t, t)) @ R
+        for col in (abi.B_HOR, abi.B_VER, abi.B_DIR, abi.B_CEN, abi.B_NT, abi.B_NP, abi.B_NXDT,
+                    abi.B_NXDP, abi.B_NORMAL):
+            bs[b, col:col + 3] = R @ bs[b, col:col + 3]
+    return abi.BeamBatch([0, src.n_beams], [1.0], bs, src.beam_nv, src.beam_external,
+                         src.vert_offsets, src.vx, src.vy, src.slope_yx, src.slope_xy,
+                         src.edge_flags)
+
+
+def test_small_phase_cells_near_the_poles(pkg, oracle, ctx):
+    """Every beam aimed at a pole, theta rows log-spaced from 1e-9 to 1e-2 rad
+    off it: each beam meets cells on both sides of the small-phase switch
+    (every vertex phase within 1e-3 rad) and near-singular edges everywhere
+    near its pole, and the kernels' per-beam bound that skips the vertex test
+    must leave every such decision to it.
+
+    Bars.  FP64: grid L2 1e-10 and per cell 1e-6 of M11.  Rows within ~1e-4
+    rad of a beam's direction are ill-conditioned: the phases there are
+    differences of O(k size) numbers, and a 1-ulp change of the beam
+    directions moves the oracle's own result by up to 4e-8 of M11 in them; the
+    engine, which folds the direction phase into the vertex phases as the
+    reference's CUDA path does, rounds differently (2.9e-7 measured).
+    Mixed: the P11 grid L2 bar of the path (1e-4).  Near the switch the FP32
+    edge sum cancels to ~phase^2 of its terms, so the per-cell error there
+    grows as ~6e-8/phase^2 (a few percent at 1e-3 rad): printed, not bounded."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    consts = abi.consts_from_fixture(fx)
+    batch = _aimed_at_poles(abi, abi.BeamBatch.from_fixture(fx).subset([0, 1]), seed=5)
+    off = np.logspace(-9, -2, 36)
+    grid = abi.Grid.build(np.r_[0.0, off, np.pi - off[::-1], np.pi], 8)
+    # the decision on the host: phases k (A x + B y) of every beam and cell
+    th, ph = np.meshgrid(grid.theta, np.arange(grid.n_az) * grid.az_step, indexing="ij")
+    d = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)], -1)
+    n_small = n_large = 0
+    for b in range(batch.n_beams):
+        s = batch.beam_scalars[b]
+        hor, ver = s[abi.B_HOR:abi.B_HOR + 3], s[abi.B_VER:abi.B_VER + 3]
+        bd = s[abi.B_DIR:abi.B_DIR + 3]
+        A, B = bd @ hor - d @ hor, bd @ ver - d @ ver
+        v = slice(batch.vert_offsets[b], batch.vert_offsets[b + 1])
+        p = consts.wave_index * (A[..., None] * batch.vx[v] + B[..., None] * batch.vy[v])
+        small = np.abs(p).max(-1) <= 1e-3
+        n_small += int(small.any())
+        n_large += int((~small).any())
+    print("beams with small-phase cells %d, with others %d, of %d"
+          % (n_small, n_large, batch.n_beams))
+    assert n_small == batch.n_beams and n_large == batch.n_beams
+    m, mn = _run(ctx, grid, batch, consts, noshadow=True)
+    o, on, _ = oracle.diffract(grid, batch, consts, noshadow=True)
+    for a, b, label in ((m, o, "FP64"), (mn, on, "FP64 no-shadow")):
+        err = np.abs(a - b) / np.abs(b[..., :1])
+        p, t, el = np.unravel_index(np.argmax(err), err.shape)
+        print("beams at the poles %s vs oracle: %s; largest at theta %.6g rad, element %d"
+              % (label, M.summary(a, b, grid.theta), grid.theta[t], el))
+        assert M.grid_l2(a, b, grid.theta) < 1e-10
+        assert M.cell(a, b) < 1e-6
+    mm, _ = _run(ctx, grid, batch, consts, flags=abi.PREC_MIXED)
+    l2 = M.grid_l2(mm[..., :1], o[..., :1], grid.theta)
+    e = np.abs(mm[..., 0] - o[..., 0]) / np.abs(o[..., 0])
+    print("beams at the poles mixed vs oracle: P11 grid_l2 %.2e; per-cell P11 max rel %.2e; "
+          "per row: %s" % (l2, e.max(), np.array2string(e.max(axis=0), precision=1)))
+    assert l2 < 1e-4
+
+
 def _quat_matrix(q):
     x, y, z, w = q / np.linalg.norm(q)
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],

@@ -211,6 +211,58 @@ def test_triangle_closed_form_and_forward_limit(pkg, oracle):
         assert abs(got - (-1j) * tri(A, 0.3 * A)) < 5e-10, A
 
 
+def least_width(x, y):
+    """Least width of a vertex set as the beam packing kernel computes it
+    (pack_beams_kernel, po_kernels.hip): over all vertex pairs, the spread of
+    the vertices across the line through the pair."""
+    w = np.inf
+    for i in range(len(x)):
+        for j in range(i + 1, len(x)):
+            ex, ey = x[j] - x[i], y[j] - y[i]
+            n = np.hypot(ex, ey)
+            if n > 0:
+                c = ex * (y - y[i]) - ey * (x - x[i])
+                w = min(w, (max(c.max(), 0.0) - min(c.min(), 0.0)) / n)
+    return w
+
+
+def test_small_phase_width_bound():
+    """The device kernels test the vertex phases one by one only where
+    max(|A|, |B|) <= 2.04e-3 / (k w), w the least width of the vertex set.
+    Pinned two ways: w against a brute-force sweep of directions, and the
+    implication (every vertex phase within 1e-3 rad => below the bound) on
+    random directions about the bound, for convex, non-convex and sliver
+    polygons."""
+    rng = np.random.default_rng(11)
+    k = 2 * np.pi / 0.532
+    ang = np.linspace(0, np.pi, 200001)
+    u = np.stack([np.cos(ang), np.sin(ang)])
+    polys = [(np.array([0., 2, 0]), np.array([0., 0, 1])),
+             (np.array([0., 1, 1, 0]), np.array([0., 0, 1e-4, 1e-4])),   # sliver
+             (np.array([0., 4, 4, 2, 0]), np.array([0., 0, 3, 0.5, 3]))]  # non-convex
+    for _ in range(20):
+        n = rng.integers(3, 12)
+        t = np.sort(rng.uniform(0, 2 * np.pi, n))
+        r = rng.uniform(0.2, 1.0, n) * rng.uniform(0.1, 30)
+        polys.append((r * np.cos(t) + rng.normal(), 0.3 * r * np.sin(t)))
+    for x, y in polys:
+        proj = u[0][:, None] * x + u[1][:, None] * y
+        brute = (proj.max(1) - proj.min(1)).min()
+        w = least_width(x, y)
+        # no direction is narrower than w, and the sweep (step dt) comes
+        # within diameter * dt of it
+        diam = np.hypot(x[:, None] - x, y[:, None] - y).max()
+        assert w <= brute * (1 + 1e-12) and brute - w <= diam * (ang[1] - ang[0]), (w, brute)
+        bound = 2.04e-3 / (k * w)
+        s = rng.uniform(0.0, 3.0, 20000) * bound * np.sqrt(2)
+        a = rng.uniform(0, 2 * np.pi, 20000)
+        A, B = s * np.cos(a), s * np.sin(a)
+        ph = np.abs(k * (A[:, None] * x + B[:, None] * y)).max(1)
+        small = ph <= 1e-3
+        assert small.any() and (~small).any()
+        assert np.all(np.maximum(np.abs(A), np.abs(B))[small] <= bound)
+
+
 def test_mueller_identities(oracle):
     rng = np.random.default_rng(1)
     G = np.diag([1.0, -1.0, -1.0, -1.0])
