@@ -120,6 +120,13 @@ typedef struct mbs_consts {
 #define MBS_NO_MFMA 16u       /* mixed precision: rotation and coherent sum on
                                  the vector ALU instead of the MFMA contraction
                                  (same arithmetic, for comparison)              */
+#define MBS_INCOHERENT 32u    /* FP64 sum of every beam's own Mueller matrix,
+                                 sum_o w_o sum_b Mueller(d_b), instead of the
+                                 Mueller matrix of the coherent sum; its M11 is
+                                 what the reference's theta probe sums
+                                 (theta_m11_kernel, GpuDiffraction.cu:1909-1957).
+                                 Not with MBS_PREC_MIXED, MBS_NOSHADOW or
+                                 MBS_PER_ORIENT (MBS_ERR_INVALID)               */
 
 typedef struct mbs_ctx mbs_ctx;
 

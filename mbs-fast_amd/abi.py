@@ -31,6 +31,7 @@ PREC_MIXED = 1
 NOSHADOW = 4
 PER_ORIENT = 8
 NO_MFMA = 16
+INCOHERENT = 32
 
 ORIENT_SOBOL, ORIENT_HAMMERSLEY, ORIENT_SHOEMAKE = 0, 1, 2
 ORIENT_STRIDE = 16

@@ -501,7 +501,11 @@ static bool plan_items(mbs_ctx *ctx, int n_tiles, int target, int work_mode,
 template <int PREC, int NS, int JONES, int PER>
 static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
-    if constexpr (PREC == 2)
+    if constexpr (PREC == 3)
+        diffract_incoherent_kernel<<<blocks, 256, 0, s>>>(
+            a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.vtx, a.vdd, a.vflag, a.nv,
+            a.voff, a.ooff, a.weights, a.k, a.eps1, a.out);
+    else if constexpr (PREC == 2)
         diffract_mfma_kernel<NS, JONES, PER><<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
             a.vflag,
@@ -528,6 +532,7 @@ static void launch_diffract(int prec, int ns, int jones, int per, const KernelAr
     MBS_L(1, 0, 0, 1) MBS_L(1, 1, 0, 1)
     MBS_L(2, 0, 0, 0) MBS_L(2, 1, 0, 0) MBS_L(2, 0, 1, 0) MBS_L(2, 1, 1, 0)
     MBS_L(2, 0, 0, 1) MBS_L(2, 1, 0, 1)
+    MBS_L(3, 0, 0, 0)
 #undef MBS_L
 }
 
@@ -588,19 +593,27 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     const int base = (flags & MBS_PREC_MIXED) ? 1 : 0;
     const mbs_ctx::Tuned &tn = ctx->tuned[base];
     const bool mfma = !(flags & MBS_NO_MFMA) && !(tn.valid && !tn.mfma);
-    const int prec = base ? (mfma ? 2 : 1) : 0;
+    const bool incoh = (flags & MBS_INCOHERENT) != 0;
+    const int prec = incoh ? 3 : base ? (mfma ? 2 : 1) : 0;
     const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
     const int per = (flags & MBS_PER_ORIENT) ? 1 : 0;
-    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT | MBS_NO_MFMA))
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT | MBS_NO_MFMA |
+                            MBS_INCOHERENT))
         return fail(ctx, MBS_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    if (incoh && (base || ns || per))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "MBS_INCOHERENT runs in FP64 without MBS_NOSHADOW or MBS_PER_ORIENT");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int rc = ensure_tables(ctx);
     if (rc)
         return rc;
     const int cells = ctx->cells;
     const int n_tiles = (cells + 255) / 256;
+    // the incoherent sum keeps per-item Mueller sums (orientation items)
     const bool jones = plan_items(ctx, n_tiles, tn.valid ? tn.target_blocks : ctx->target_blocks,
-                                  tn.valid ? tn.work_mode : ctx->work_mode, ctx->h_items);
+                                  incoh ? MBS_WORK_ORIENTATIONS
+                                        : tn.valid ? tn.work_mode : ctx->work_mode,
+                                  ctx->h_items);
     const int n_items = (int)ctx->h_items.size();
     const size_t mcount = (size_t)cells * 16;
     HIP_TRY(ctx, ctx->res.ensure(mcount));

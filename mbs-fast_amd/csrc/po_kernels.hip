@@ -1007,6 +1007,63 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     }
 }
 
+// Incoherent sum (MBS_INCOHERENT), FP64: per orientation the Mueller
+// matrices of the single beams are added -- sum_b Mueller(d_b) rather than
+// Mueller(sum_b d_b) -- so beams do not interfere.  Its M11 element, the sum
+// of |d_b|^2 / 2, is what the reference's theta probe accumulates
+// (theta_m11_kernel, src/cuda/GpuDiffraction.cu:1909-1957).  Orientation
+// items as in diffract_kernel's orientation mode: each item writes the
+// weighted sum of its orientations, which reduce_mueller_kernel adds.
+__global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_incoherent_kernel(
+    const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells,
+    const double *__restrict__ tab, const double *__restrict__ hdr,
+    const double *__restrict__ vtx, const double *__restrict__ vdd,
+    const int32_t *__restrict__ vflag, const int32_t *__restrict__ nv,
+    const int64_t *__restrict__ voff, const int64_t *__restrict__ ooff,
+    const double *__restrict__ weights, double k, double eps1, double *__restrict__ out)
+{
+    int item, tile;
+    if (!map_block(n_items, n_tiles, item, tile))
+        return;
+    const int cell = tile * 256 + threadIdx.x;
+    const bool live = cell < cells;
+    const int c = live ? cell : cells - 1;  // dead lanes compute, never store
+    double d[3];
+    for (int q = 0; q < 3; ++q)
+        d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
+    __shared__ double s_u[12 * 256];
+    __shared__ double2 s_tab[kTabN];
+    for (int q = 0; q < 12; ++q)
+        s_u[q * 256 + threadIdx.x] = tab[(size_t)(DT_U0 + q) * cells + c];
+    {
+        double ts, tc;
+        sincospi((double)threadIdx.x / (kTabN / 2), &ts, &tc);
+        s_tab[threadIdx.x] = make_double2(tc, ts);
+    }
+    __syncthreads();
+    const WorkItem it = items[item];
+    for (int o = it.o_begin; o < it.o_end; ++o) {
+        double m[16];
+        for (int q = 0; q < 16; ++q)
+            m[q] = 0.0;
+        for (int64_t b = ooff[o]; b < ooff[o + 1]; ++b) {
+            const int64_t v0 = voff[b];
+            double dj[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            beam_dir_f64<0>(hdr + (size_t)b * HDR_STRIDE, vtx + (size_t)v0 * VTX_STRIDE,
+                            vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nv[b], d, s_u, s_tab, k,
+                            eps1, dj, nullptr);
+            add_mueller(m, dj, 1.0);
+        }
+        if (!live)
+            continue;
+        const double w = weights[o];
+        double *dst = out + ((size_t)item * cells + cell) * 16;
+        const bool first = (o == it.o_begin);
+        for (int q = 0; q < 16; ++q)
+            dst[q] = (first ? 0.0 : dst[q]) + w * m[q];
+    }
+}
+
 // ---- coherent sum as a matrix contraction on MFMA (mixed precision) -------
 //
 // The rotation is linear in the beam's polarisation data: with the

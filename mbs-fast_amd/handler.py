@@ -155,6 +155,61 @@ class HandlerPOGpu:
             n //= 2
         return n
 
+    def DiffractThetasGpu(self, prepared, theta_rads, nPoints=None):
+        """HandlerPO::DiffractThetasGpu (HandlerPO.h:159-162), the reference's
+        theta probe (GpuDiffraction.cu:4780-4965).  Returns (ok, m11_out).
+
+        For every theta: the M11 of each single beam (no interference between
+        beams), weighted by its orientation's weight, summed over the beams
+        and the handler's nAz azimuths p * azimuth step, divided by nAz + 1
+        (theta_m11_kernel, :1909-1957).  Theta within 1e-14 of a pole is moved
+        1e-6 rad inside (:4845-4850); vf = (-sin phi, cos phi, 0) except at
+        the poles (:4862-4876).  The reference's GPU rotation leaves out the
+        1/2 of the CPU one and its Mueller kernels undo that with weight/4,
+        but the probe sums 0.5 |d|^2 unscaled: its values are 4 times the M11
+        of the CPU convention, and are reproduced so.  m11_out is untouched
+        when there is nothing to do (no points, orientations or azimuths) and
+        zero-filled otherwise; a batch the backend cannot take returns False."""
+        theta = np.asarray(theta_rads, dtype=np.float64)
+        n = len(theta) if nPoints is None else int(nPoints)
+        if n <= 0:
+            return True, None
+        g = self.grid
+        if prepared.n_orient == 0 or g.n_az <= 0:
+            return True, None
+        if np.any((prepared.beam_nv <= 0) | (prepared.beam_nv > 32)):
+            return False, None  # as :4800-4802; the engine refuses nv < 3 or > 31 below
+        m11 = np.zeros(n)
+        if prepared.n_beams == 0:
+            return True, m11
+        th = theta[:n].copy()
+        th[th <= 1e-14] = 1e-6
+        th[th >= np.pi - 1e-14] = np.pi - 1e-6
+        phi = np.arange(g.n_az) * g.az_step
+        dz = -np.cos(th)
+        vf = np.zeros((g.n_az, n, 3))
+        vf[:, :, 0] = -np.sin(phi)[:, None]
+        vf[:, :, 1] = np.cos(phi)[:, None]
+        vf[:, dz >= 1.0 - 1e-15] = (0.0, -1.0, 0.0)
+        vf[:, dz <= -1.0 + 1e-15] = (0.0, 1.0, 0.0)
+        probe = abi.Grid(th, g.az_step, vf)
+        try:
+            ctx = self._context()
+            ctx.set_grid(probe)
+            try:
+                ctx.upload(prepared, self.consts)
+                ctx.diffract(abi.INCOHERENT)
+                tmp = np.zeros((g.n_az, n, 16))
+                ctx.accumulate(tmp, None)
+            finally:
+                ctx.set_grid(g)
+        except EngineError as e:
+            if e.status in (abi.MBS_ERR_BAD_BEAM, abi.MBS_ERR_INVALID):
+                return False, None
+            return self._device_failure(e), None
+        m11[:] = 4.0 * tmp[:, :, 0].sum(axis=0) / (g.n_az + 1)
+        return True, m11
+
     def close(self):
         if self._ctx is not None:
             self._ctx.close()

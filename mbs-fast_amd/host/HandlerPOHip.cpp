@@ -33,8 +33,9 @@
 // =fp32 overrides the build default at run time.
 // Multi-size (rescaled beam sets): the scale / waveIndex arguments and the
 // fused multi-scale member, with the reference's scale-on-pack rule.
+// Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
 // Not provided (return false, as in a reference build without these paths):
-// the phi-FFT and theta-probe variants.
+// the phi-FFT variants.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
@@ -592,10 +593,104 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
     return false;
 }
 
-bool HandlerPO::DiffractThetasGpu(const std::vector<PreparedOrientation> &, const double *,
-                                  int, std::vector<double> &)
+// The reference's theta probe (src/cuda/GpuDiffraction.cu:4780-4965, kernel
+// theta_m11_kernel :1909-1957): for every theta of theta_rads, the M11 of
+// each single beam (beams do not interfere), weighted by its orientation's
+// weight, summed over the beams and the handler's nAz azimuths p * azimuth
+// step, divided by nAz + 1.  Theta within 1e-14 of a pole is moved 1e-6 rad
+// inside (:4845-4850); vf = (-sin phi, cos phi, 0) except at the poles
+// (:4862-4876).  The reference's GPU rotation leaves out the 1/2 of the CPU
+// one (its Mueller kernels weight by 1/4 instead), but the probe sums
+// 0.5 |d|^2 unscaled: its values are 4 times the M11 of the CPU convention,
+// reproduced so.  Evaluated by the engine's FP64 incoherent sum
+// (MBS_INCOHERENT) on a probe grid; the context's next call uploads its own
+// grid again.  Beams with fewer than 3 vertices, which the reference probe
+// would take, are refused like every other entry point here.
+bool HandlerPO::DiffractThetasGpu(const std::vector<PreparedOrientation> &prepared,
+                                  const double *theta_rads, int nPoints,
+                                  std::vector<double> &m11_out)
 {
-    return false;
+    if (nPoints <= 0)
+        return true;
+    const int nOrient = (int)prepared.size();
+    const int nAz = m_sphere.nAzimuth;
+    if (nOrient == 0 || nAz <= 0)
+        return true;
+    size_t nBeams = 0;
+    for (const PreparedOrientation &po : prepared)
+        for (const PreparedBeam &pb : po.beams)
+        {
+            if (!pb.edgeData.valid || pb.edgeData.nVertices <= 0 || pb.edgeData.nVertices > 32)
+                return false;
+            ++nBeams;
+        }
+    m11_out.assign(nPoints, 0.0);
+    if (nBeams == 0)
+        return true;
+
+    GridSpec g;
+    g.nAz = nAz;
+    g.nZen = nPoints - 1;
+    g.azStep = m_sphere.azinuthStep;
+    g.theta.resize(nPoints);
+    g.vf.resize((size_t)nAz * nPoints * 3);
+    for (int t = 0; t < nPoints; ++t)
+    {
+        double theta = theta_rads[t];
+        if (theta <= 1e-14)
+            theta = 1e-6;
+        else if (theta >= M_PI - 1e-14)
+            theta = M_PI - 1e-6;
+        g.theta[t] = theta;
+    }
+    for (int p = 0; p < nAz; ++p)
+    {
+        const double phi = p * g.azStep;
+        for (int t = 0; t < nPoints; ++t)
+        {
+            const double dz = -std::cos(g.theta[t]);
+            double *w = &g.vf[((size_t)p * nPoints + t) * 3];
+            w[0] = -std::sin(phi);
+            w[1] = std::cos(phi);
+            w[2] = 0.0;
+            if (dz >= 1.0 - 1e-15 || dz <= -1.0 + 1e-15)
+            {
+                w[0] = 0.0;
+                w[1] = dz > 0.0 ? -1.0 : 1.0;
+            }
+        }
+    }
+    bool ok = false;
+    ThreadContexts::Entry *e = Context(0, g, ok);
+    if (!e)
+        return ok;
+    PackedBatch &pk = e->pk;
+    if (!Pack(prepared, 0, nOrient, PackScale(), pk))
+        return false;
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    const mbs_beam_batch view = pk.View();
+    int rc = mbs_upload_beams(e->ctx, &view, &c);
+    if (rc == MBS_OK)
+        rc = mbs_diffract(e->ctx, MBS_INCOHERENT);
+    std::vector<double> m((size_t)nAz * nPoints * 16, 0.0);
+    if (rc == MBS_OK)
+        rc = mbs_accumulate_mueller(e->ctx, m.data(), nullptr);
+    if (rc == MBS_ERR_DEVICE)
+        return DeviceFailure("theta probe", mbs_last_error(e->ctx));
+    if (rc != MBS_OK)
+    {
+        std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(e->ctx));
+        return false;
+    }
+    for (int t = 0; t < nPoints; ++t)
+    {
+        double sum = 0.0;
+        for (int p = 0; p < nAz; ++p)
+            sum += m[((size_t)p * nPoints + t) * 16];
+        m11_out[t] = 4.0 * sum / (nAz + 1);
+    }
+    return true;
 }
 
 // Batches are bounded by beam count so that one call's vertex and beam

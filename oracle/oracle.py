@@ -33,6 +33,8 @@ def lib():
         L.mbs_oracle_diffract_batch.restype = C.c_int
         L.mbs_oracle_orientation_jones.argtypes = [vp, vp, vp, C.c_int, dp, dp]
         L.mbs_oracle_orientation_jones.restype = C.c_int
+        L.mbs_oracle_incoherent_batch.argtypes = [vp, vp, vp, dp]
+        L.mbs_oracle_incoherent_batch.restype = C.c_int
         L.mbs_oracle_add_mueller.argtypes = [dp, dp, C.c_double]
         L.mbs_oracle_add_mueller.restype = None
         L.mbs_oracle_azimuth_average.argtypes = [vp, dp, dp]
@@ -96,6 +98,17 @@ def diffract(grid, batch, consts, noshadow=False, per_orient=False):
         raise RuntimeError("oracle status %d" % rc)
     assert m.size == cells * 16
     return m, m_ns, per
+
+
+def incoherent(grid, batch, consts):
+    """sum_o w_o sum_b Mueller(d_b) per cell: single beams' Mueller matrices
+    added without interference (po_oracle.c, mbs_oracle_incoherent_batch)."""
+    m = np.zeros((grid.n_az, grid.n_zen + 1, 16))
+    rc = lib().mbs_oracle_incoherent_batch(C.byref(grid.struct()), C.byref(batch.struct()),
+                                           C.byref(consts), _d(m))
+    if rc != 0:
+        raise RuntimeError("oracle status %d" % rc)
+    return m
 
 
 def orientation_jones(grid, batch, consts, orient, noshadow=False):

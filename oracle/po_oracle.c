@@ -265,15 +265,19 @@ int mbs_oracle_check_batch(const mbs_beam_batch *bt)
  * cell p*(n_zen+1)+t).  jns receives the sum over non-external beams (the
  * localJ_noshadow of :2780-2783); it may be NULL.  Cells are independent, so
  * phi rows run in parallel and the result does not depend on thread count. */
-int mbs_oracle_orientation_jones(const mbs_grid *g, const mbs_beam_batch *bt,
-                                 const mbs_consts *c, int orient,
-                                 double *j, double *jns)
+void mbs_oracle_add_mueller(double *out, const double *jj, double w);
+
+/* With inc != NULL the beams' d are not summed: inc += w * Mueller(d) per
+ * beam and cell instead (the incoherent sum), and j, jns are unused. */
+static int orientation_core(const mbs_grid *g, const mbs_beam_batch *bt, const mbs_consts *c,
+                            int orient, double *j, double *jns, double *inc, double w_inc)
 {
     const int nAz = g->n_az, nZ = g->n_zen + 1;
     const int64_t b0 = bt->orient_offsets[orient], b1 = bt->orient_offsets[orient + 1];
     const double k = c->wave_index;
-    memset(j, 0, sizeof(double) * 8 * (size_t)nAz * nZ);
-    if (jns)
+    if (!inc)
+        memset(j, 0, sizeof(double) * 8 * (size_t)nAz * nZ);
+    if (jns && !inc)
         memset(jns, 0, sizeof(double) * 8 * (size_t)nAz * nZ);
     double *st = (double *)malloc(sizeof(double) * 2 * nZ);
     double *ct = st + nZ;
@@ -338,6 +342,10 @@ int mbs_oracle_orientation_jones(const mbs_grid *g, const mbs_beam_batch *bt,
                 d[5] = s10r * jp[1] + s10i * jp[0] + s11r * jp[5] + s11i * jp[4];
                 d[6] = s10r * jp[2] - s10i * jp[3] + s11r * jp[6] - s11i * jp[7];
                 d[7] = s10r * jp[3] + s10i * jp[2] + s11r * jp[7] + s11i * jp[6];
+                if (inc) {
+                    mbs_oracle_add_mueller(inc + 16 * cell, d, w_inc);
+                    continue;
+                }
                 double *jc = j + 8 * cell;
                 for (int q = 0; q < 8; ++q)
                     jc[q] += d[q];
@@ -350,6 +358,31 @@ int mbs_oracle_orientation_jones(const mbs_grid *g, const mbs_beam_batch *bt,
         }
     }
     free(st);
+    return MBS_OK;
+}
+
+int mbs_oracle_orientation_jones(const mbs_grid *g, const mbs_beam_batch *bt,
+                                 const mbs_consts *c, int orient,
+                                 double *j, double *jns)
+{
+    return orientation_core(g, bt, c, orient, j, jns, NULL, 0.0);
+}
+
+/* Incoherent sum over a batch: m += sum_o w_o sum_b Mueller(d_b) per cell,
+ * the Mueller matrices of single beams added without interference.  Its M11
+ * element is what the reference GPU theta probe sums, 0.5 |d_b|^2 w_o per
+ * beam (theta_m11_kernel, src/cuda/GpuDiffraction.cu:1909-1957). */
+int mbs_oracle_incoherent_batch(const mbs_grid *g, const mbs_beam_batch *bt,
+                                const mbs_consts *c, double *m)
+{
+    int rc = mbs_oracle_check_batch(bt);
+    if (rc)
+        return rc;
+    for (int o = 0; o < bt->n_orient; ++o) {
+        rc = orientation_core(g, bt, c, o, NULL, NULL, m, bt->weights[o]);
+        if (rc)
+            return rc;
+    }
     return MBS_OK;
 }
 

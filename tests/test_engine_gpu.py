@@ -261,6 +261,55 @@ def test_handler_mirror_semantics(pkg, oracle):
     h.close()
 
 
+def test_theta_probe_and_incoherent_sum(pkg, oracle, ctx):
+    """HandlerPO::DiffractThetasGpu (the reference's theta probe) through the
+    mirror and the engine's MBS_INCOHERENT sum, against the oracle's
+    incoherent sum on the probe grid as the reference builds it
+    (GpuDiffraction.cu:4845-4876: theta at a pole moved 1e-6 rad inside,
+    vf = (-sin phi, cos phi, 0)) and the probe's scaling: 4 x the sum over
+    azimuths / (nAz + 1) (theta_m11_kernel, :1909-1957; the factor 4 is the
+    rotation's missing 1/2 squared)."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    theta = np.r_[0.0, np.radians([0.5, 1, 2, 5, 10, 30, 60, 90, 120, 150, 175, 179.5]), np.pi]
+    th = theta.copy()
+    th[th <= 1e-14] = 1e-6
+    th[th >= np.pi - 1e-14] = np.pi - 1e-6
+    phi = np.arange(grid.n_az) * grid.az_step
+    vf = np.zeros((grid.n_az, len(th), 3))
+    vf[:, :, 0] = -np.sin(phi)[:, None]
+    vf[:, :, 1] = np.cos(phi)[:, None]
+    probe = abi.Grid(th, grid.az_step, vf)
+    inc = oracle.incoherent(probe, batch, consts)
+    # the engine's incoherent Mueller grid
+    m, _ = _run(ctx, probe, batch, consts, flags=abi.INCOHERENT)
+    check_fp64(m, inc, th, "incoherent sum FP64 vs oracle")
+    for bad in (abi.PREC_MIXED, abi.NOSHADOW, abi.PER_ORIENT):
+        with pytest.raises(pkg.engine.EngineError):
+            ctx.diffract(abi.INCOHERENT | bad)
+    # the probe through the handler mirror
+    h = pkg.handler.HandlerPOGpu(grid, consts)
+    ok, m11 = h.DiffractThetasGpu(batch, theta)
+    assert ok
+    want = 4.0 * inc[..., 0].sum(axis=0) / (grid.n_az + 1)
+    print("theta probe vs oracle: max rel %.2e" % np.max(np.abs(m11 - want) / want))
+    np.testing.assert_allclose(m11, want, rtol=1e-10, atol=0)
+    # the handler's own grid is back in place for the coherent path
+    Mg = np.zeros((grid.n_az, grid.n_zen + 1, 16))
+    assert h.HandleOrientationsToLocalGpu(batch, Mg, None)
+    o, _, _ = oracle.diffract(grid, batch, consts)
+    check_fp64(Mg, o, grid.theta, "coherent after the probe")
+    # nothing to do: untouched; a beam it cannot take: False
+    assert h.DiffractThetasGpu(batch, theta[:0]) == (True, None)
+    bad = abi.BeamBatch(batch.orient_offsets, batch.weights, batch.beam_scalars,
+                        np.where(np.arange(batch.n_beams) == 3, 40, batch.beam_nv),
+                        batch.beam_external, batch.vert_offsets, batch.vx, batch.vy,
+                        batch.slope_yx, batch.slope_xy, batch.edge_flags)
+    assert h.DiffractThetasGpu(bad, theta) == (False, None)
+    h.close()
+
+
 def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
     """A resampled batch at the orientation-average particle size (D=50 um) on
     a reduced grid (the oracle is too slow for the full one)."""

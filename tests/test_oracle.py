@@ -211,6 +211,44 @@ def test_triangle_closed_form_and_forward_limit(pkg, oracle):
         assert abs(got - (-1j) * tri(A, 0.3 * A)) < 5e-10, A
 
 
+def test_incoherent_sum_is_the_sum_of_single_beam_runs(pkg, oracle, fixtures):
+    """The oracle's incoherent sum (sum_o w_o sum_b Mueller(d_b)) against the
+    pinned coherent path run on one beam at a time: a batch of one beam has
+    no interference, so the two must agree to rounding."""
+    abi = pkg.abi
+    fx = fixtures["sobol_average"]
+    consts = abi.consts_from_fixture(fx)
+    src = abi.BeamBatch.from_fixture(fx).subset([0, 3])
+    keep = [0, 1, 2, 5, 8, 13, 21] + [src.orient_offsets[1] + i for i in (0, 2, 7, 11)]
+    vo = src.vert_offsets
+    counts = [sum(1 for b in keep if b < src.orient_offsets[1]),
+              sum(1 for b in keep if b >= src.orient_offsets[1])]
+    verts = np.concatenate([np.arange(vo[b], vo[b + 1]) for b in keep])
+    nv = src.beam_nv[keep]
+    batch = abi.BeamBatch(np.r_[0, np.cumsum(counts)], src.weights, src.beam_scalars[keep], nv,
+                          src.beam_external[keep], np.r_[0, np.cumsum(nv)], src.vx[verts],
+                          src.vy[verts], src.slope_yx[verts], src.slope_xy[verts],
+                          src.edge_flags[verts])
+    grid = abi.Grid.build(np.linspace(0, np.pi, 37), 8)
+    inc = oracle.incoherent(grid, batch, consts)
+    want = np.zeros_like(inc)
+    for i in range(batch.n_beams):
+        o = 0 if i < counts[0] else 1
+        one = batch.subset([o])
+        j = i - batch.orient_offsets[o]
+        ov = one.vert_offsets
+        b1 = abi.BeamBatch([0, 1], [one.weights[0]], one.beam_scalars[j:j + 1], one.beam_nv[j:j + 1],
+                           one.beam_external[j:j + 1], [0, one.beam_nv[j]],
+                           one.vx[ov[j]:ov[j + 1]], one.vy[ov[j]:ov[j + 1]],
+                           one.slope_yx[ov[j]:ov[j + 1]], one.slope_xy[ov[j]:ov[j + 1]],
+                           one.edge_flags[ov[j]:ov[j + 1]])
+        want += oracle.diffract(grid, b1, consts)[0]
+    np.testing.assert_allclose(inc, want, rtol=0, atol=1e-13 * np.abs(want[..., 0]).max())
+    coh = oracle.diffract(grid, batch, consts)[0]
+    assert np.abs(coh[..., 0] - inc[..., 0]).max() > 1e-3 * inc[..., 0].max()  # beams interfere
+    assert np.all(inc[..., 0] >= 0)
+
+
 def least_width(x, y):
     """Least width of a vertex set as the beam packing kernel computes it
     (pack_beams_kernel, po_kernels.hip): over all vertex pairs, the spread of
