@@ -171,6 +171,35 @@ int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *grid);
 int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *batch,
                      const mbs_consts *consts);
 
+/* Per-beam data from which the device re-derives a size-rescaled beam set
+ * (the multi-size path: the reference packs every scale on the host,
+ * src/cuda/GpuDiffraction.cu:273-358, applied when |scale - 1| > 1e-15,
+ * :4324; its fused kernel rescales phases on the device, :1204-1238). */
+typedef struct mbs_beam_scaling {
+    const double *raw_jones;     /* [n_beams*8] the beam's Jones matrix J      */
+    const double *path;          /* [n_beams] optical path (external beams) or
+                                    projected length (internal) at scale 1    */
+    const double *sign;          /* [n_beams] -1 for external beams and internal
+                                    ones with an odd number of interactions  */
+    const int64_t *abs_offsets;  /* [n_beams+1] ranges in abs_paths            */
+    const double *abs_paths;     /* absorption path lengths at scale 1         */
+    double c_abs;                /* absorption coefficient                     */
+    double wave_index;           /* k of the phase exp(i k path scale)         */
+} mbs_beam_scaling;
+
+/* Uploads the scaling data of the batch of the last mbs_upload_beams
+ * (n_beams entries); it stays until the next mbs_upload_beams.  No
+ * reference counterpart: it replaces the host re-pack of every scale. */
+int mbs_upload_beam_scaling(mbs_ctx *ctx, const mbs_beam_scaling *sc);
+
+/* Re-derives the device beam records of the uploaded batch for size scale s,
+ * taking the uploaded batch as scale 1: aperture vertices and beam centres
+ * times s, areas times s^2, and J_phased = sign * a(s) * J * exp(i k path s)
+ * with a(s) the mean over the beam's absorption paths p of exp(c_abs p s)
+ * (1 for p <= DBL_EPSILON, and 1 without paths).  For |s - 1| <= 1e-15 the
+ * batch's own records are restored.  Needs mbs_upload_beam_scaling. */
+int mbs_rescale_beams(mbs_ctx *ctx, double scale);
+
 /* Evaluates the uploaded batch on the current grid and leaves the result on
  * the device: the weighted Mueller sum over the batch's orientations (and
  * the no-shadow sum / per-orientation grids when flagged).  Asynchronous on

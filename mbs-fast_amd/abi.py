@@ -52,6 +52,12 @@ class mbs_beam_batch(C.Structure):
                 ("edge_flags", _u8p)]
 
 
+class mbs_beam_scaling(C.Structure):
+    _fields_ = [("raw_jones", _dp), ("path", _dp), ("sign", _dp),
+                ("abs_offsets", C.POINTER(C.c_int64)), ("abs_paths", _dp),
+                ("c_abs", C.c_double), ("wave_index", C.c_double)]
+
+
 class mbs_grid(C.Structure):
     _fields_ = [("n_az", C.c_int32), ("n_zen", C.c_int32), ("theta", _dp),
                 ("az_step", C.c_double), ("vf", _dp)]

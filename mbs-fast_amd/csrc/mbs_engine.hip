@@ -82,6 +82,11 @@ struct mbs_ctx {
     DevBuf<uint8_t> ef_in;
     DevBuf<double> hdr, vtx, vdd;
     DevBuf<float> hdr32, wb;
+    // size rescaling data of the batch (mbs_upload_beam_scaling)
+    bool scaling_ok = false;
+    double sc_cabs = 0.0, sc_k = 0.0;
+    DevBuf<double> sc_raw, sc_path, sc_sign, sc_apath;
+    DevBuf<int64_t> sc_aoff;
     // work and results
     DevBuf<WorkItem> items;
     DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows, orient;
@@ -230,6 +235,11 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->vf.release();
     ctx->tab.release();
     ctx->bs_in.release();
+    ctx->sc_raw.release();
+    ctx->sc_path.release();
+    ctx->sc_sign.release();
+    ctx->sc_apath.release();
+    ctx->sc_aoff.release();
     ctx->vx_in.release();
     ctx->vy_in.release();
     ctx->syx_in.release();
@@ -326,11 +336,30 @@ static int ensure_tables(mbs_ctx *ctx)
     return MBS_OK;
 }
 
+// Device records from the uploaded inputs: pack_beams_kernel on the
+// context's stream, for size scale `scale` with the scaling data when
+// `rescale`, else as uploaded.
+static int launch_pack(mbs_ctx *ctx, int nb, const mbs_consts &c, double scale, bool rescale)
+{
+    // small-phase area limit: (legacy ? invComplWave : -invComplWave) * area
+    const double sg = c.legacy_sign ? 1.0 : -1.0;
+    pack_beams_kernel<<<(nb + 127) / 128, 128, 0, ctx->stream>>>(
+        nb, ctx->bs_in.p, ctx->nv.p, ctx->ext.p, ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
+        ctx->syx_in.p, ctx->sxy_in.p, ctx->ef_in.p, c.wave_index, sg * c.inv_compl_wave[0],
+        sg * c.inv_compl_wave[1], c.compl_wave[0], c.compl_wave[1], rescale ? scale : 1.0,
+        rescale ? ctx->sc_raw.p : nullptr, ctx->sc_path.p, ctx->sc_sign.p, ctx->sc_aoff.p,
+        ctx->sc_apath.p, ctx->sc_cabs, ctx->sc_k, ctx->hdr.p, ctx->hdr32.p, ctx->vtx.p,
+        ctx->vdd.p, ctx->vflag.p, ctx->wb.p);
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
 extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts *c)
 {
     if (!ctx || !bt || !c)
         return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: NULL argument");
     ctx->beams_ok = false;
+    ctx->scaling_ok = false;
     ctx->have_result = false;
     if (bt->n_orient < 0 || bt->n_beams < 0 || (bt->n_orient > 0 && !bt->orient_offsets)
         || (bt->n_orient > 0 && !bt->weights))
@@ -414,16 +443,10 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
         HIP_TRY(ctx, hipMemcpyAsync(ctx->weights.p, bt->weights, sizeof(double) * no,
                                     hipMemcpyHostToDevice, s));
     }
-    // small-phase area limit: (legacy ? invComplWave : -invComplWave) * area
-    const double sg = c->legacy_sign ? 1.0 : -1.0;
     if (nb > 0) {
-        pack_beams_kernel<<<(nb + 127) / 128, 128, 0, s>>>(
-            nb, ctx->bs_in.p, ctx->nv.p, ctx->ext.p, ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
-            ctx->syx_in.p, ctx->sxy_in.p, ctx->ef_in.p, c->wave_index,
-            sg * c->inv_compl_wave[0], sg * c->inv_compl_wave[1], c->compl_wave[0],
-            c->compl_wave[1], ctx->hdr.p, ctx->hdr32.p,
-            ctx->vtx.p, ctx->vdd.p, ctx->vflag.p, ctx->wb.p);
-        HIP_TRY(ctx, hipGetLastError());
+        const int rc = launch_pack(ctx, nb, *c, 1.0, false);
+        if (rc)
+            return rc;
     }
     if (ctx->timing) {
         HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
@@ -439,6 +462,73 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
     ctx->n_vert = nvert;
     ctx->consts = *c;
     ctx->beams_ok = true;
+    return MBS_OK;
+}
+
+extern "C" int mbs_upload_beam_scaling(mbs_ctx *ctx, const mbs_beam_scaling *sc)
+{
+    if (!ctx || !sc)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beam_scaling: NULL argument");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    ctx->scaling_ok = false;
+    const int nb = ctx->n_beams;
+    if (nb > 0 && (!sc->raw_jones || !sc->path || !sc->sign || !sc->abs_offsets))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beam_scaling: NULL array");
+    if (!std::isfinite(sc->c_abs) || !std::isfinite(sc->wave_index))
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beam_scaling: bad constants");
+    const int64_t na = nb > 0 ? sc->abs_offsets[nb] : 0;
+    if (nb > 0 && (sc->abs_offsets[0] != 0 || (na > 0 && !sc->abs_paths)))
+        return fail(ctx, MBS_ERR_INVALID, "abs_offsets must start at 0 (and abs_paths be set)");
+    for (int b = 0; b < nb; ++b)
+        if (sc->abs_offsets[b + 1] < sc->abs_offsets[b])
+            return fail(ctx, MBS_ERR_INVALID, "abs_offsets decrease at %d", b);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, ctx->sc_raw.ensure((size_t)nb * 8));
+    HIP_TRY(ctx, ctx->sc_path.ensure(nb));
+    HIP_TRY(ctx, ctx->sc_sign.ensure(nb));
+    HIP_TRY(ctx, ctx->sc_aoff.ensure((size_t)nb + 1));
+    HIP_TRY(ctx, ctx->sc_apath.ensure((size_t)na));
+    hipStream_t s = ctx->stream;
+    if (nb > 0) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->sc_raw.p, sc->raw_jones, sizeof(double) * 8 * nb,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->sc_path.p, sc->path, sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->sc_sign.p, sc->sign, sizeof(double) * nb,
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->sc_aoff.p, sc->abs_offsets, sizeof(int64_t) * (nb + 1),
+                                    hipMemcpyHostToDevice, s));
+        if (na > 0)
+            HIP_TRY(ctx, hipMemcpyAsync(ctx->sc_apath.p, sc->abs_paths, sizeof(double) * na,
+                                        hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    ctx->sc_cabs = sc->c_abs;
+    ctx->sc_k = sc->wave_index;
+    ctx->scaling_ok = true;
+    return MBS_OK;
+}
+
+extern "C" int mbs_rescale_beams(mbs_ctx *ctx, double scale)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    if (!ctx->scaling_ok)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beam_scaling has not been called");
+    if (!(scale > 0) || !std::isfinite(scale))
+        return fail(ctx, MBS_ERR_INVALID, "scale must be positive");
+    ctx->have_result = false;
+    if (ctx->n_beams == 0)
+        return MBS_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int rc = launch_pack(ctx, ctx->n_beams, ctx->consts, scale,
+                               std::fabs(scale - 1.0) > 1e-15);
+    if (rc)
+        return rc;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return MBS_OK;
 }
 

@@ -137,7 +137,13 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
                                   const double *__restrict__ sxy,
                                   const uint8_t *__restrict__ ef, double k,
                                   double al_re, double al_im, double cw_re,
-                                  double cw_im,
+                                  double cw_im, double sc,
+                                  const double *__restrict__ raw_j,
+                                  const double *__restrict__ path,
+                                  const double *__restrict__ sign,
+                                  const int64_t *__restrict__ abs_off,
+                                  const double *__restrict__ abs_path, double c_abs,
+                                  double k_j,
                                   double *__restrict__ hdr,
                                   float *__restrict__ hdr32,
                                   double *__restrict__ vtx,
@@ -151,9 +157,14 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     const double *s = bs + (size_t)b * 36;
     const int nv = nv_in[b];
     const int64_t v0 = voff[b];
+    // size rescaling (raw_j set): vertices and centre times sc, area times
+    // sc^2, J_phased rebuilt below; sc = 1 leaves every value as it is
+    auto X = [&](int i) { return vx[v0 + i] * sc; };
+    auto Y = [&](int i) { return vy[v0 + i] * sc; };
     double *h = hdr + (size_t)b * HDR_STRIDE;
     float *h32 = hdr32 + (size_t)b * HDR32_STRIDE;
-    const double *hor = s + 0, *ver = s + 3, *bd = s + 6, *cen = s + 9;
+    const double *hor = s + 0, *ver = s + 3, *bd = s + 6;
+    const double cen[3] = {s[9] * sc, s[10] * sc, s[11] * sc};
     h[H_HOR + 0] = hor[0];
     h[H_HOR + 1] = hor[1];
     h[H_HOR + 2] = hor[2];
@@ -170,7 +181,7 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     h[H_EXT] = ext ? 1.0 : 0.0;
     // complWave is folded into J_phased below; the small-phase area limit is
     // therefore stored divided by complWave.
-    const double area = s[12];
+    const double area = s[12] * (sc * sc);
     const double cw2 = cw_re * cw_re + cw_im * cw_im;
     const double lr = al_re * area, li = al_im * area;
     h[H_AL + 0] = (lr * cw_re + li * cw_im) / cw2;
@@ -180,8 +191,8 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     double minX = 1e300, maxX = -1e300, minY = 1e300, maxY = -1e300;
     for (int i = 0; i < nv; ++i) {
         const int j = (i + 1 < nv) ? i + 1 : 0;
-        const double xi = vx[v0 + i], yi = vy[v0 + i];
-        const double xj = vx[v0 + j], yj = vy[v0 + j];
+        const double xi = X(i), yi = Y(i);
+        const double xj = X(j), yj = Y(j);
         const double cr = xi * yj - xj * yi;
         tw += cr;
         fx6 += (xi + xj) * cr;
@@ -219,15 +230,15 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     double wmin = __builtin_inf();
     if (nv <= 32) {
         for (int i = 0; i < nv; ++i) {
-            const double xi = vx[v0 + i], yi = vy[v0 + i];
+            const double xi = X(i), yi = Y(i);
             for (int j = i + 1; j < nv; ++j) {
-                const double ex = vx[v0 + j] - xi, ey = vy[v0 + j] - yi;
+                const double ex = X(j) - xi, ey = Y(j) - yi;
                 const double len = sqrt(ex * ex + ey * ey);
                 if (!(len > 0.0))
                     continue;
                 double lo = 0.0, hi = 0.0;
                 for (int l = 0; l < nv; ++l) {
-                    const double c = ex * (vy[v0 + l] - yi) - ey * (vx[v0 + l] - xi);
+                    const double c = ex * (Y(l) - yi) - ey * (X(l) - xi);
                     lo = fmin(lo, c);
                     hi = fmax(hi, c);
                 }
@@ -245,8 +256,33 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
         h[H_T1 + q] = 0.5 * s[16 + q];      // NP
         h[H_T1 + 3 + q] = 0.5 * s[22 + q];  // nxDP
     }
+    // J_phased, or for a rescaled set sign * a(sc) * J * exp(i k_j path sc),
+    // a(sc) the mean of exp(c_abs p sc) over the absorption paths p (1 for
+    // p <= DBL_EPSILON): the host rule of the multi-size pack
+    double jpv[8];
+    if (raw_j) {
+        double amp = sign[b];
+        const int64_t a0 = abs_off[b], a1 = abs_off[b + 1];
+        if (a1 > a0) {
+            double sum = 0.0;
+            for (int64_t a = a0; a < a1; ++a)
+                sum += abs_path[a] > 2.220446049250313e-16 ? exp(c_abs * abs_path[a] * sc) : 1.0;
+            amp *= sum / (double)(a1 - a0);
+        }
+        double sn, cs;
+        sincos(k_j * (path[b] * sc), &sn, &cs);
+        for (int q = 0; q < 4; ++q) {
+            const double jr = raw_j[(size_t)b * 8 + 2 * q] * amp;
+            const double ji = raw_j[(size_t)b * 8 + 2 * q + 1] * amp;
+            jpv[2 * q] = jr * cs - ji * sn;
+            jpv[2 * q + 1] = jr * sn + ji * cs;
+        }
+    } else {
+        for (int q = 0; q < 8; ++q)
+            jpv[q] = s[25 + q];
+    }
     for (int q = 0; q < 4; ++q) {  // complWave * J_phased
-        const double jr = s[25 + 2 * q], ji = s[26 + 2 * q];
+        const double jr = jpv[2 * q], ji = jpv[2 * q + 1];
         h[H_JP + 2 * q] = cw_re * jr - cw_im * ji;
         h[H_JP + 2 * q + 1] = cw_re * ji + cw_im * jr;
     }
@@ -276,14 +312,14 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     for (int i = 0; i < nv; ++i) {
         double *v = vtx + (size_t)(v0 + i) * VTX_STRIDE;
         const uint8_t f = ef[v0 + i];
-        v[V_KX] = k * vx[v0 + i];
-        v[V_KY] = k * vy[v0 + i];
+        v[V_KX] = k * X(i);
+        v[V_KY] = k * Y(i);
         v[V_SYX] = (f & 1) ? syx[v0 + i] : 0.0;
         v[V_SXY] = (f & 2) ? sxy[v0 + i] : 0.0;
         const int j = (i + 1 < nv) ? i + 1 : 0;
         double *w = vdd + (size_t)(v0 + i) * VDD_STRIDE;
-        w[D_DX] = vx[v0 + j] - vx[v0 + i];
-        w[D_DY] = vy[v0 + j] - vy[v0 + i];
+        w[D_DX] = X(j) - X(i);
+        w[D_DY] = Y(j) - Y(i);
         vflag[v0 + i] = f;
     }
 }

@@ -52,6 +52,10 @@ def lib():
         L.mbs_upload_beams.argtypes = [vp, C.POINTER(abi.mbs_beam_batch),
                                        C.POINTER(abi.mbs_consts)]
         L.mbs_upload_beams.restype = C.c_int
+        L.mbs_upload_beam_scaling.argtypes = [vp, C.POINTER(abi.mbs_beam_scaling)]
+        L.mbs_upload_beam_scaling.restype = C.c_int
+        L.mbs_rescale_beams.argtypes = [vp, C.c_double]
+        L.mbs_rescale_beams.restype = C.c_int
         L.mbs_diffract.argtypes = [vp, C.c_uint32]
         L.mbs_diffract.restype = C.c_int
         L.mbs_accumulate_mueller.argtypes = [vp, dp, dp]
@@ -184,6 +188,21 @@ class Context:
     def upload(self, batch, consts):
         self._check(self._L.mbs_upload_beams(self._h, C.byref(batch.struct()), C.byref(consts)))
         self.batch = batch
+
+    def upload_scaling(self, raw_jones, path, sign, abs_offsets, abs_paths, c_abs, wave_index):
+        """Per-beam data of the uploaded batch for mbs_rescale_beams."""
+        self._sc_arrays = [np.ascontiguousarray(raw_jones, np.float64).reshape(-1),
+                           np.ascontiguousarray(path, np.float64),
+                           np.ascontiguousarray(sign, np.float64),
+                           np.ascontiguousarray(abs_offsets, np.int64),
+                           np.ascontiguousarray(abs_paths, np.float64)]
+        r, p, sg, ao, ap = self._sc_arrays
+        sc = abi.mbs_beam_scaling(_d(r), _d(p), _d(sg), ao.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  _d(ap) if ap.size else None, float(c_abs), float(wave_index))
+        self._check(self._L.mbs_upload_beam_scaling(self._h, C.byref(sc)))
+
+    def rescale(self, scale):
+        self._check(self._L.mbs_rescale_beams(self._h, float(scale)))
 
     def diffract(self, flags=0):
         self._check(self._L.mbs_diffract(self._h, flags))

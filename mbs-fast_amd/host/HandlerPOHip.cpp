@@ -94,6 +94,9 @@ struct PackedBatch
     std::vector<double> weights, scalars, vx, vy, syx, sxy;
     std::vector<int32_t> nv, external;
     std::vector<uint8_t> flags;
+    // per-beam data of a device rescale (mbs_beam_scaling), when packed
+    std::vector<double> rawJones, path, sign, absPaths;
+    std::vector<int64_t> absOffsets;
 
     mbs_beam_batch View() const
     {
@@ -193,16 +196,19 @@ void ScaledJones(const PreparedBeam &pb, const PackScale &sc, double jp[8])
 // Two passes over the orientations, both OpenMP-parallel as the reference's
 // packing is (:4407-4428): beam and vertex counts (which also validate every
 // beam), then, after a prefix sum, every orientation fills its own slice.
-// Returns false for a beam the device path cannot take.
+// Returns false for a beam the device path cannot take.  With `scaling`
+// the per-beam data of a device rescale (mbs_beam_scaling) are packed too:
+// the raw Jones matrix, the path and sign of ScaledJones, the absorption
+// paths.
 bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
-          const PackScale &sc, PackedBatch &pk)
+          const PackScale &sc, PackedBatch &pk, bool scaling = false)
 {
-    std::vector<int64_t> nb(nOrient), nvert(nOrient);
+    std::vector<int64_t> nb(nOrient), nvert(nOrient), nabs(nOrient);
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
     for (int i = 0; i < nOrient; ++i)
     {
-        int64_t v = 0;
+        int64_t v = 0, a = 0;
         for (const PreparedBeam &pb : prepared[start + i].beams)
         {
             const BeamEdgeData &ed = pb.edgeData;
@@ -210,18 +216,31 @@ bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrie
                 || ed.nVertices > MBS_MAX_VERTICES)
                 bad = 1;
             v += ed.nVertices;
+            a += (int64_t)pb.absorptionPaths.size();
         }
         nb[i] = (int64_t)prepared[start + i].beams.size();
         nvert[i] = v;
+        nabs[i] = a;
     }
     if (bad)
         return false;
     pk.orientOffsets.assign(nOrient + 1, 0);
-    std::vector<int64_t> vbase(nOrient + 1, 0);
+    std::vector<int64_t> vbase(nOrient + 1, 0), abase(nOrient + 1, 0);
     for (int i = 0; i < nOrient; ++i)
     {
         pk.orientOffsets[i + 1] = pk.orientOffsets[i] + nb[i];
         vbase[i + 1] = vbase[i] + nvert[i];
+        abase[i + 1] = abase[i] + nabs[i];
+    }
+    if (scaling)
+    {
+        const size_t n = (size_t)pk.orientOffsets[nOrient];
+        pk.rawJones.resize(n * 8);
+        pk.path.resize(n);
+        pk.sign.resize(n);
+        pk.absOffsets.resize(n + 1);
+        pk.absOffsets[n] = abase[nOrient];
+        pk.absPaths.resize((size_t)abase[nOrient]);
     }
     const size_t nBeams = (size_t)pk.orientOffsets[nOrient];
     const size_t nVerts = (size_t)vbase[nOrient];
@@ -244,9 +263,28 @@ bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrie
         pk.weights[i] = po.sinZenith;
         size_t bi = (size_t)pk.orientOffsets[i];
         size_t vi = (size_t)vbase[i];
+        size_t ai = (size_t)abase[i];
         for (const PreparedBeam &pb : po.beams)
         {
             const BeamEdgeData &ed = pb.edgeData;
+            if (scaling)
+            {
+                const ::complex raw[4] = {pb.origBeam.J.m11, pb.origBeam.J.m12,
+                                          pb.origBeam.J.m21, pb.origBeam.J.m22};
+                for (int k = 0; k < 4; ++k)
+                {
+                    pk.rawJones[bi * 8 + 2 * k] = real(raw[k]);
+                    pk.rawJones[bi * 8 + 2 * k + 1] = imag(raw[k]);
+                }
+                pk.path[bi] = pb.isExternal ? pb.origBeam.opticalPath : pb.info.projLenght;
+                double sg = pb.isExternal ? -1.0 : 1.0;
+                if (!pb.isExternal && (pb.origBeam.nActs & 1))
+                    sg = -sg;
+                pk.sign[bi] = sg;
+                pk.absOffsets[bi] = (int64_t)ai;
+                for (double p : pb.absorptionPaths)
+                    pk.absPaths[ai++] = p;
+            }
             double jp[8] = {pb.jp00r, pb.jp00i, pb.jp01r, pb.jp01i,
                             pb.jp10r, pb.jp10i, pb.jp11r, pb.jp11i};
             if (sc.on)
@@ -412,14 +450,14 @@ ThreadContexts::Entry *Context(int device, GridSpec &g, bool &ok)
 // Diffracts one packed batch and adds the result into localM (and into
 // *localMns when it is given).
 bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
-              Arr2D &localM, Arr2D *localMns, double packMs)
+              Arr2D &localM, Arr2D *localMns, double packMs, bool upload = true)
 {
     const int nAz = e.nAz, nZen = e.nZen;
     const bool timing = EnvIs("MBS_GPU_TIMING", "1");
     const auto t0 = std::chrono::steady_clock::now();
     mbs_set_timing(e.ctx, timing ? 1 : 0);
     const mbs_beam_batch view = pk.View();
-    int rc = mbs_upload_beams(e.ctx, &view, &c);
+    int rc = upload ? mbs_upload_beams(e.ctx, &view, &c) : MBS_OK;
     if (rc == MBS_OK)
         rc = mbs_diffract(e.ctx, PrecisionFlag() | (localMns ? MBS_NOSHADOW : 0u));
     std::vector<double> mFull((size_t)nAz * (nZen + 1) * 16, 0.0);
@@ -547,9 +585,10 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
 
 // The multi-size path's fused call (src/scattering/TracerPOTotal.cpp:4380-4432):
 // one traced beam set, diffracted for every scale s into localMs[s] (full
-// output only; the caller uses it without no-shadow output).  Each scale is
-// packed as in HandleOrientationsToLocalGpu(..., scale, waveIndex) and runs
-// on the same device context.
+// output only; the caller uses it without no-shadow output).  The beam set
+// is packed and uploaded once; each scale is re-derived on the device by the
+// rule of HandleOrientationsToLocalGpu(..., scale, waveIndex)'s pack, as the
+// reference's fused kernel reuses one upload (GpuDiffraction.cu:2332-2410).
 bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOrientation> &prepared,
                                                    int start, int count,
                                                    const std::vector<double> &scales,
@@ -571,17 +610,41 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
     const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
                                        m_legacySign);
     PackedBatch &pk = e->pk;
-    for (size_t s = 0; s < scales.size(); ++s)
+    // one pack and upload at scale 1 with the scaling data; every scale is
+    // then re-derived on the device (mbs_rescale_beams) by the rule of the
+    // per-scale pack (PackScale, ScaledJones)
+    const auto tp = std::chrono::steady_clock::now();
+    if (!Pack(prepared, start, nOrient, PackScale(), pk, true))
+        return false;
+    const double packMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
+    if (pk.nv.empty())
+        return true;
+    const mbs_beam_batch view = pk.View();
+    mbs_beam_scaling bsc;
+    bsc.raw_jones = pk.rawJones.data();
+    bsc.path = pk.path.data();
+    bsc.sign = pk.sign.data();
+    bsc.abs_offsets = pk.absOffsets.data();
+    bsc.abs_paths = pk.absPaths.data();
+    bsc.c_abs = AbsorptionCoefficient();
+    bsc.wave_index = waveIndex;
+    int rc = mbs_upload_beams(e->ctx, &view, &c);
+    if (rc == MBS_OK)
+        rc = mbs_upload_beam_scaling(e->ctx, &bsc);
+    for (size_t s = 0; s < scales.size() && rc == MBS_OK; ++s)
     {
-        const auto tp = std::chrono::steady_clock::now();
-        if (!Pack(prepared, start, nOrient,
-                  PackScale(scales[s], waveIndex, AbsorptionCoefficient()), pk))
+        rc = mbs_rescale_beams(e->ctx, scales[s]);
+        if (rc == MBS_OK && !RunBatch(*e, pk, c, localMs[s], nullptr, s == 0 ? packMs : 0.0,
+                                      false))
             return false;
-        const double packMs =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp)
-                .count();
-        if (!pk.nv.empty() && !RunBatch(*e, pk, c, localMs[s], nullptr, packMs))
-            return false;
+    }
+    if (rc == MBS_ERR_DEVICE)
+        return DeviceFailure("rescaled batch", mbs_last_error(e->ctx));
+    if (rc != MBS_OK)
+    {
+        std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(e->ctx));
+        return false;
     }
     return true;
 }

@@ -50,6 +50,7 @@ int main(void) {
          sizeof(mbs_consts), sizeof(mbs_device_info));
   printf("%%zu %%zu %%zu\\n", offsetof(mbs_beam_batch, edge_flags),
          offsetof(mbs_grid, vf), offsetof(mbs_consts, legacy_sign));
+  printf("%%zu %%zu\\n", sizeof(mbs_beam_scaling), offsetof(mbs_beam_scaling, wave_index));
   return 0;
 }
 """ % HEADER)
@@ -59,7 +60,8 @@ int main(void) {
     abi = pkg.abi
     want = [C.sizeof(abi.mbs_beam_batch), C.sizeof(abi.mbs_grid), C.sizeof(abi.mbs_consts),
             C.sizeof(abi.mbs_device_info), abi.mbs_beam_batch.edge_flags.offset,
-            abi.mbs_grid.vf.offset, abi.mbs_consts.legacy_sign.offset]
+            abi.mbs_grid.vf.offset, abi.mbs_consts.legacy_sign.offset,
+            C.sizeof(abi.mbs_beam_scaling), abi.mbs_beam_scaling.wave_index.offset]
     assert [int(v) for v in out] == want
 
 

@@ -310,6 +310,71 @@ def test_theta_probe_and_incoherent_sum(pkg, oracle, ctx):
     h.close()
 
 
+def test_device_rescale_of_a_beam_set(pkg, oracle, ctx):
+    """mbs_rescale_beams (the multi-size path without a re-pack per scale)
+    against the host rule of the drop-in's pack (the reference's
+    GpuDiffraction.cu:273-358, :4324): vertices and centres times s, areas
+    times s^2, J_phased = sign a(s) J exp(i k path s), a(s) the mean of
+    exp(c_abs p s) over the absorption paths (1 for p <= DBL_EPSILON).
+    The host-scaled batch goes through the oracle and through a plain upload."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, full, consts = _setup(pkg, fx)
+    batch = full.subset([0, 1, 2])
+    nb = batch.n_beams
+    rng = np.random.default_rng(17)
+    raw = rng.normal(size=(nb, 8))
+    path = rng.uniform(1.0, 50.0, nb)
+    sign = np.where(rng.random(nb) < 0.4, -1.0, 1.0)
+    n_abs = rng.integers(0, 4, nb)
+    aoff = np.r_[0, np.cumsum(n_abs)]
+    apath = rng.uniform(0.0, 20.0, aoff[-1])
+    apath[::5] = 0.0
+    c_abs, kj = -0.02, consts.wave_index
+
+    def host_scaled(sc):
+        bs = batch.beam_scalars.copy()
+        bs[:, abi.B_CEN:abi.B_CEN + 3] *= sc
+        bs[:, abi.B_AREA] *= sc * sc
+        amp = sign.copy()
+        for b in range(nb):
+            p = apath[aoff[b]:aoff[b + 1]]
+            if len(p):
+                amp[b] *= np.mean(np.where(p > np.finfo(float).eps, np.exp(c_abs * p * sc), 1.0))
+        rot = np.exp(1j * kj * (path * sc))
+        j = (raw[:, 0::2] + 1j * raw[:, 1::2]) * (amp * rot)[:, None]
+        bs[:, abi.B_JP:abi.B_JP + 8:2] = j.real
+        bs[:, abi.B_JP + 1:abi.B_JP + 8:2] = j.imag
+        return abi.BeamBatch(batch.orient_offsets, batch.weights, bs, batch.beam_nv,
+                             batch.beam_external, batch.vert_offsets, batch.vx * sc,
+                             batch.vy * sc, batch.slope_yx, batch.slope_xy, batch.edge_flags)
+
+    ctx.set_grid(grid)
+    ctx.set_target_blocks(65536)
+    for sc in (0.5, 1.7, 3.0):
+        hb = host_scaled(sc)
+        ctx.upload(hb, consts)
+        m_host, _ = ctx.mueller(0)
+        ctx.upload(batch, consts)
+        ctx.upload_scaling(raw, path, sign, aoff, apath, c_abs, kj)
+        ctx.rescale(sc)
+        m_dev, _ = ctx.mueller(0)
+        o, _, _ = oracle.diffract(grid, hb, consts)
+        check_fp64(m_dev, o, grid.theta, "device rescale s=%g vs oracle" % sc)
+        e = M.grid_l2(m_dev, m_host, grid.theta)
+        print("device rescale s=%g vs host-scaled upload: grid_l2 %.2e" % (sc, e))
+        assert e < 1e-12
+    ctx.rescale(1.0)  # back to the batch as uploaded
+    m1, _ = ctx.mueller(0)
+    o1, _, _ = oracle.diffract(grid, batch, consts)
+    check_fp64(m1, o1, grid.theta, "rescale 1 restores the batch")
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.rescale(-1.0)
+    ctx.upload(batch, consts)  # a new upload drops the scaling data
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.rescale(2.0)
+
+
 def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
     """A resampled batch at the orientation-average particle size (D=50 um) on
     a reduced grid (the oracle is too slow for the full one)."""
