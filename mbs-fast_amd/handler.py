@@ -30,6 +30,21 @@ from . import abi
 from .engine import Context, EngineError
 
 
+def upsample_phi(low, n_full):
+    """The phi upsampling of the reference's FFT path (fft_upsample_phi_arr2d,
+    GpuDiffraction.cu:3195-3330, fft_phi_pad_kernel :794-814) along axis 0:
+    forward DFT of the n_low samples, frequency k placed at k for
+    k <= n_low/2 (the Nyquist term of an even n_low on the positive side) and
+    at n_full + k - n_low otherwise, scaled by 1/n_low, unnormalised inverse
+    DFT over n_full, real part."""
+    n_low = low.shape[0]
+    k = np.arange(n_low)
+    f = np.where(k <= n_low // 2, k, k - n_low)
+    spec = np.zeros((n_full,) + low.shape[1:], dtype=complex)
+    spec[np.where(f >= 0, f, n_full + f)] = np.fft.fft(low, axis=0) / n_low
+    return np.real(np.fft.ifft(spec, axis=0) * n_full)
+
+
 class HandlerPOGpu:
     def __init__(self, grid, consts, device=0, precision="fp64"):
         self.grid = grid
@@ -154,6 +169,56 @@ class HandlerPOGpu:
                 break
             n //= 2
         return n
+
+    def HandleOrientationsToLocalGpuFftPhi(self, prepared, localM, localM_noshadow, start=0,
+                                           count=None, scale=1.0, waveIndex=0.0,
+                                           fftPhiFactor=0):
+        """HandlerPO::HandleOrientationsToLocalGpuFftPhi (GpuDiffraction.cu:3504-3839):
+        the batch on nLow = nFull / factor azimuths (at least 16), then the
+        Mueller grids upsampled in phi (upsample_phi) and added into the
+        handler's grids.  factor: ``fftPhiFactor``, else MBS_FFT_PHI_FACTOR,
+        else by nFull (choose_fft_phi_factor, :712-727); a factor of 1,
+        nFull < 32 or nLow >= nFull runs the direct path.  The validation,
+        theta and adaptive options return False."""
+        if not self.isCoh:
+            return False
+        g = self.grid
+        n_full = g.n_az
+        factor = fftPhiFactor
+        if factor <= 0:
+            env = os.environ.get("MBS_FFT_PHI_FACTOR", "")
+            factor = int(env) if env.isdigit() and 1 <= int(env) <= 64 else 0
+        if factor <= 0:
+            factor = 8 if n_full >= 2400 else 6 if n_full >= 600 else 3 if n_full >= 240 else 2
+        n_low = n_full // factor if factor > 1 else n_full
+        if n_low < 16:
+            n_low = min(n_full, 16)
+        if factor <= 1 or n_full < 32 or n_low >= n_full:
+            return self.HandleOrientationsToLocalGpu(prepared, localM, localM_noshadow, start,
+                                                     count, scale, waveIndex)
+        if (os.environ.get("MBS_FFT_CHECK") == "1"
+                or os.environ.get("MBS_FFT_THETA_FACTOR", "1") not in ("", "1")
+                or os.environ.get("MBS_FFT_ADAPTIVE_PHI", "0") not in ("", "0")):
+            return False
+        low = abi.Grid.build(g.theta, n_low)
+        ns = self.ComputeNoShadow()
+        lm = np.zeros((n_low, g.n_zen + 1, 16))
+        lmn = np.zeros_like(lm) if ns else None
+        ctx = self._context()
+        self.grid = low
+        try:
+            ctx.set_grid(low)
+            ok = self.HandleOrientationsToLocalGpu(prepared, lm, lmn, start, count, scale,
+                                                   waveIndex)
+        finally:
+            self.grid = g
+            ctx.set_grid(g)
+        if not ok:
+            return False
+        self._cells_view(localM, g.n_az, g.n_zen + 1)[:] += upsample_phi(lm, n_full)
+        if ns:
+            self._cells_view(localM_noshadow, g.n_az, g.n_zen + 1)[:] += upsample_phi(lmn, n_full)
+        return True
 
     def DiffractThetasGpu(self, prepared, theta_rads, nPoints=None):
         """HandlerPO::DiffractThetasGpu (HandlerPO.h:159-162), the reference's

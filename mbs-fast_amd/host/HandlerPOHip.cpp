@@ -34,8 +34,10 @@
 // Multi-size (rescaled beam sets): the scale / waveIndex arguments and the
 // fused multi-scale member, with the reference's scale-on-pack rule.
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
-// Not provided (return false, as in a reference build without these paths):
-// the phi-FFT variants.
+// Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi): the direct path
+// on a coarse azimuth grid and the reference's phi upsampling; its
+// validation, theta and adaptive options and the multi-size variant return
+// false.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
@@ -576,11 +578,148 @@ bool HandlerPO::HandleBeamsToLocalGpu(const PreparedOrientation &prepared, Arr2D
     return HandleOrientationsToLocalGpu(one, 0, 1, localM, localM_noshadow, 1.0, 0.0);
 }
 
-bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &,
-                                                   int, int, Arr2D &, Arr2D &, double,
-                                                   double)
+namespace
 {
-    return false;
+int EnvInt(const char *name, int lo, int hi, int fallback)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v)
+        return fallback;
+    char *end = nullptr;
+    const long x = std::strtol(v, &end, 10);
+    return (end && *end == '\0' && x >= lo && x <= hi) ? (int)x : fallback;
+}
+
+// choose_fft_phi_factor (src/cuda/GpuDiffraction.cu:712-727, with
+// gpu_fft_phi_factor :513-531): the handler's factor, else MBS_FFT_PHI_FACTOR
+// (1..64, "auto" = unset), else by the full azimuth count.
+int ChoosePhiFactor(int nFull, int configured)
+{
+    if (configured > 0)
+        return configured;
+    const int requested = EnvInt("MBS_FFT_PHI_FACTOR", 1, 64, 0);
+    if (requested > 0)
+        return requested;
+    return nFull >= 2400 ? 8 : nFull >= 600 ? 6 : nFull >= 240 ? 3 : 2;
+}
+
+// Adds to dst(q, t), q < nFull, t <= nZen, the band-limited interpolation in
+// phi of low(p, t), p < nLow: the upsampling of the reference's phi-FFT path
+// (fft_upsample_phi_arr2d :3195-3330, fft_phi_pad_kernel :794-814): forward
+// DFT of the nLow samples; frequency k at k for k <= nLow/2 (the Nyquist term
+// of an even nLow on the positive side), else at k - nLow; scaled by 1/nLow;
+// inverse DFT over nFull; real part.  As a real matrix,
+// y[q] = sum_p K(q/nFull - p/nLow) x[p], K(u) = (1/nLow) sum_k cos(2 pi f_k u),
+// with the sum in closed form (Dirichlet kernel): sin((M + 1/2) a)/sin(a/2) for
+// nLow = 2M + 1, sin((M - 1/2) a)/sin(a/2) + cos(M a) for nLow = 2M,
+// a = 2 pi u, and nLow at u = 0.
+void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
+{
+    std::vector<double> K((size_t)nFull * nLow);
+    const int M = nLow / 2;
+    for (int q = 0; q < nFull; ++q)
+        for (int p = 0; p < nLow; ++p)
+        {
+            double k = 1.0;
+            if ((int64_t)q * nLow != (int64_t)p * nFull)
+            {
+                const double a = 2.0 * M_PI * ((double)q / nFull - (double)p / nLow);
+                const double h = std::sin(0.5 * a);
+                k = (nLow & 1) ? std::sin((M + 0.5) * a) / h / nLow
+                               : (std::sin((M - 0.5) * a) / h + std::cos(M * a)) / nLow;
+            }
+            K[(size_t)q * nLow + p] = k;
+        }
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t <= nZen; ++t)
+    {
+        std::vector<double> x((size_t)nLow * 16);
+        for (int p = 0; p < nLow; ++p)
+            std::memcpy(&x[(size_t)p * 16], low.RawCell(p, t), 16 * sizeof(double));
+        for (int q = 0; q < nFull; ++q)
+        {
+            double acc[16] = {0};
+            const double *kq = &K[(size_t)q * nLow];
+            for (int p = 0; p < nLow; ++p)
+                for (int e = 0; e < 16; ++e)
+                    acc[e] += kq[p] * x[(size_t)p * 16 + e];
+            double *cell = dst.RawCell(q, t);
+            for (int e = 0; e < 16; ++e)
+                cell[e] += acc[e];
+        }
+    }
+}
+}  // namespace
+
+// The reference's angular Fourier interpolation (src/cuda/GpuDiffraction.cu:
+// 3504-3839): the batch is diffracted on a grid of nLow = nFull / factor
+// azimuths (at least 16) with the handler's zenith rows, and the Mueller grids
+// are upsampled in phi to the handler's nFull azimuths and added into localM
+// (and localM_noshadow).  A factor of 1, nFull < 32 or nLow >= nFull runs the
+// direct path.  The reference's validation and refinement (MBS_FFT_CHECK, a
+// tolerance), theta upsampling (MBS_FFT_THETA_FACTOR) and adaptive rows
+// (MBS_FFT_ADAPTIVE_PHI) are not provided: such a call returns false.
+bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &prepared,
+                                                   int start, int count, Arr2D &localM,
+                                                   Arr2D &localM_noshadow, double scale,
+                                                   double waveIndex)
+{
+    if (!isCoh)
+        return false;
+    const int nFull = m_sphere.nAzimuth;
+    const int nZen = m_sphere.nZenith;
+    const int factor = ChoosePhiFactor(nFull, m_fftPhiFactor);
+    int nLow = factor > 1 ? nFull / factor : nFull;
+    if (nLow < 16)
+        nLow = std::min(nFull, 16);
+    if (factor <= 1 || nFull < 32 || nLow >= nFull)
+    {
+        m_fftStatistics->RecordCall(nFull, nFull, 1);
+        return HandleOrientationsToLocalGpu(prepared, start, count, localM, localM_noshadow,
+                                            scale, waveIndex);
+    }
+    const char *check = std::getenv("MBS_FFT_CHECK");
+    const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
+    if ((check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0
+        || EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1
+        || (adaptive && *adaptive && std::strcmp(adaptive, "0") != 0))
+    {
+        std::fprintf(stderr, "HIP diffraction backend: FFT validation, theta upsampling and "
+                             "adaptive phi rows are not provided\n");
+        return false;
+    }
+    m_fftStatistics->RecordCall(nFull, nLow, factor);
+    static bool printed = false;
+    if (!printed)
+    {
+        std::fprintf(stderr,
+                     "GPU FFT angular interpolation: direct Nphi=%d, output Nphi=%d (factor=%d), "
+                     "direct Ntheta=%d, output Ntheta=%d (factor=%d). "
+                     "This is angular Fourier interpolation, not aperture pFFT/FMM.\n",
+                     nLow, nFull, factor, nZen + 1, nZen + 1, 1);
+        printed = true;
+    }
+    const bool ns = ComputeNoShadow();
+    ScatteringRange lowSphere = m_sphere;
+    lowSphere.nAzimuth = nLow;
+    lowSphere.azinuthStep = 2.0 * M_PI / nLow;
+    lowSphere.ComputeSphereDirections(*m_incidentLight);
+    Arr2D lowM(nLow + 1, nZen + 1, 4, 4);
+    lowM.ClearArr();
+    Arr2D lowMns(ns ? nLow + 1 : 0, ns ? nZen + 1 : 0, ns ? 4 : 0, ns ? 4 : 0);
+    if (ns)
+        lowMns.ClearArr();
+    const ScatteringRange saved = m_sphere;
+    m_sphere = lowSphere;
+    const bool ok = HandleOrientationsToLocalGpu(prepared, start, count, lowM, lowMns, scale,
+                                                 waveIndex);
+    m_sphere = saved;
+    if (!ok)
+        return false;
+    UpsamplePhi(lowM, nLow, nFull, nZen, localM);
+    if (ns)
+        UpsamplePhi(lowMns, nLow, nFull, nZen, localM_noshadow);
+    return true;
 }
 
 // The multi-size path's fused call (src/scattering/TracerPOTotal.cpp:4380-4432):

@@ -172,6 +172,25 @@ def test_reference_cli_gpu_backend_eight_column_aggregate():
     _compare(*_case("aggregate.particle", 1e-4, 6, 4, 24, wd))
 
 
+def test_reference_cli_fft_phi_interpolation():
+    """The reference CLI's --fft path on the HIP backend
+    (HandleOrientationsToLocalGpuFftPhi: 48 direct azimuths upsampled to 96)
+    against the direct HIP backend on the same run.  The two differ by the
+    phi interpolation error of the coarse grid; the bound only catches a
+    broken upsampling (a lost or misplaced harmonic moves the averaged rows
+    by far more)."""
+    wd, args = _case((1, 20, 10), 1e-4, 6, 8, 96)
+    _run(HIP_BIN, args, wd, "direct", "gpu")
+    _run(HIP_BIN, args + ["--fft"], wd, "fft", "gpu")
+    log = open(os.path.join(wd, "fft.log")).read()
+    assert "direct Nphi=48, output Nphi=96" in log
+    for suffix in ("", "_noshadow"):
+        a, b = _dat(wd, "fft", suffix), _dat(wd, "direct", suffix)
+        g = M.gate_l2(a, b)
+        print("--fft vs direct%s: gate_l2 %.2e" % (suffix or " (full)", g))
+        assert g < 5e-2
+
+
 @pytest.mark.parametrize("fused", [False, True], ids=["per_size_scaled", "fused_multi_scale"])
 def test_reference_cli_multi_size(fused):
     """The reference's shared-trace multi-size mode (--multigrid D_MIN D_MAX N,

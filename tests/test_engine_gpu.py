@@ -375,6 +375,42 @@ def test_device_rescale_of_a_beam_set(pkg, oracle, ctx):
         ctx.rescale(2.0)
 
 
+def test_phi_fft_interpolation_path(pkg, oracle):
+    """HandleOrientationsToLocalGpuFftPhi through the mirror: the engine on
+    the coarse azimuth grid (nLow = nFull/2 here) and the phi upsampling,
+    against the oracle on the same coarse grid upsampled the same way; the
+    interpolation error against the direct full grid is printed."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    _, batch, consts = _setup(pkg, fx)
+    theta = np.radians(np.r_[0.0, np.linspace(0.5, 179.5, 30), 180.0])
+    grid = abi.Grid.build(theta, 64)
+    h = pkg.handler.HandlerPOGpu(grid, consts)
+    h.SetFullOnly(False)
+    Mg = np.zeros((grid.n_az + 1, grid.n_zen + 1, 16))
+    Mns = np.zeros_like(Mg)
+    assert h.HandleOrientationsToLocalGpuFftPhi(batch, Mg, Mns)
+    low = abi.Grid.build(theta, 32)
+    o, on, _ = oracle.diffract(low, batch, consts, noshadow=True)
+    want = pkg.handler.upsample_phi(o, grid.n_az)
+    want_ns = pkg.handler.upsample_phi(on, grid.n_az)
+    check_fp64(Mg[:-1], want, theta, "phi-FFT path vs oracle on the coarse grid, upsampled")
+    check_fp64(Mns[:-1], want_ns, theta, "phi-FFT path no-shadow")
+    direct, _, _ = oracle.diffract(grid, batch, consts)
+    print("phi-FFT interpolation vs direct 64-azimuth grid: %s"
+          % M.summary(Mg[:-1], direct, theta))
+    # the direct path below the size limit, and the options it does not provide
+    small = abi.Grid.build(theta, 24)
+    h2 = pkg.handler.HandlerPOGpu(small, consts)
+    M2 = np.zeros((24, grid.n_zen + 1, 16))
+    assert h2.HandleOrientationsToLocalGpuFftPhi(batch, M2, None)
+    check_fp64(M2, oracle.diffract(small, batch, consts)[0], theta, "phi-FFT below 32 azimuths")
+    h.isCoh = False
+    assert not h.HandleOrientationsToLocalGpuFftPhi(batch, Mg, Mns)
+    h.close()
+    h2.close()
+
+
 def test_synthetic_large_batch_against_oracle(pkg, oracle, ctx):
     """A resampled batch at the orientation-average particle size (D=50 um) on
     a reduced grid (the oracle is too slow for the full one)."""

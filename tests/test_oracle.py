@@ -249,6 +249,44 @@ def test_incoherent_sum_is_the_sum_of_single_beam_runs(pkg, oracle, fixtures):
     assert np.all(inc[..., 0] >= 0)
 
 
+def dirichlet_upsampling_matrix(n_low, n_full):
+    """The real matrix the C++ drop-in applies for the phi upsampling
+    (UpsamplePhi, mbs-fast_amd/host/HandlerPOHip.cpp): K(u) = (1/n_low)
+    sum_k cos(2 pi f_k u) in closed form."""
+    q = np.arange(n_full)[:, None]
+    p = np.arange(n_low)[None, :]
+    a = 2 * np.pi * (q / n_full - p / n_low)
+    m = n_low // 2
+    zero = q * n_low == p * n_full
+    h = np.where(zero, 1.0, np.sin(0.5 * a))
+    if n_low % 2:
+        k = np.sin((m + 0.5) * a) / h / n_low
+    else:
+        k = (np.sin((m - 0.5) * a) / h + np.cos(m * a)) / n_low
+    return np.where(zero, 1.0, k)
+
+
+def test_phi_upsampling_forms(pkg):
+    """The phi upsampling of the FFT path, two ways: the reference's FFT
+    padding (handler.upsample_phi) and the closed-form matrix of the C++
+    drop-in.  Both reproduce a trigonometric polynomial of degree < n_low/2
+    sampled on the coarse grid exactly, and agree on arbitrary data."""
+    rng = np.random.default_rng(4)
+    for n_low, n_full in ((16, 32), (48, 96), (45, 90), (80, 240), (21, 50)):
+        x = rng.normal(size=(n_low, 3, 16))
+        fft_form = pkg.handler.upsample_phi(x, n_full)
+        mat_form = np.einsum("qp,p...->q...", dirichlet_upsampling_matrix(n_low, n_full), x)
+        np.testing.assert_allclose(mat_form, fft_form, rtol=0, atol=1e-12 * np.abs(x).max())
+        deg = (n_low - 1) // 2
+        c = rng.normal(size=(deg + 1, 2))
+        def trig(phi):
+            return sum(c[j, 0] * np.cos(j * phi) + c[j, 1] * np.sin(j * phi) for j in range(deg + 1))
+        lo = trig(2 * np.pi * np.arange(n_low) / n_low)[:, None, None] * np.ones((1, 1, 16))
+        want = trig(2 * np.pi * np.arange(n_full) / n_full)
+        np.testing.assert_allclose(pkg.handler.upsample_phi(lo, n_full)[:, 0, 0], want,
+                                   rtol=0, atol=1e-12 * np.abs(c).sum())
+
+
 def least_width(x, y):
     """Least width of a vertex set as the beam packing kernel computes it
     (pack_beams_kernel, po_kernels.hip): over all vertex pairs, the spread of
