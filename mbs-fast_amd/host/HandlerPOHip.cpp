@@ -34,10 +34,10 @@
 // Multi-size (rescaled beam sets): the scale / waveIndex arguments and the
 // fused multi-scale member, with the reference's scale-on-pack rule.
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
-// Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi): the direct path
-// on a coarse azimuth grid and the reference's phi upsampling; its
-// validation, theta and adaptive options and the multi-size variant return
-// false.
+// Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi and its
+// multi-size form): the direct path on a coarse azimuth grid and the
+// reference's phi upsampling; its validation, theta and adaptive options
+// return false.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
@@ -788,11 +788,61 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
     return true;
 }
 
+// The multi-size form of the phi-FFT path (src/cuda/GpuDiffraction.cu, the
+// member after :3839): only with MBS_GPU_MULTI_K_FULL=1 and full output, as
+// in the reference; the fused multi-size call on the coarse azimuth grid,
+// then every scale's grid upsampled in phi into localMs[s].  The options
+// the single-size path does not provide return false here too.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
-    const std::vector<PreparedOrientation> &, int, int, const std::vector<double> &, double,
-    std::vector<Arr2D> &)
+    const std::vector<PreparedOrientation> &prepared, int start, int count,
+    const std::vector<double> &scales, double waveIndex, std::vector<Arr2D> &localMs)
 {
-    return false;
+    const char *fullK = std::getenv("MBS_GPU_MULTI_K_FULL");
+    if (!(fullK && std::strcmp(fullK, "1") == 0))
+        return false;
+    if (!isCoh || ComputeNoShadow())
+        return false;
+    const char *check = std::getenv("MBS_FFT_CHECK");
+    const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
+    if ((check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0
+        || (adaptive && *adaptive && std::strcmp(adaptive, "0") != 0))
+        return false;
+    const int nFull = m_sphere.nAzimuth;
+    const int nZen = m_sphere.nZenith;
+    const int factor = ChoosePhiFactor(nFull, m_fftPhiFactor);
+    int nLow = factor > 1 ? nFull / factor : nFull;
+    if (nLow < 16)
+        nLow = std::min(nFull, 16);
+    if (factor <= 1 || nFull < 32 || nLow >= nFull)
+    {
+        m_fftStatistics->RecordCall(nFull, nFull, 1);
+        return HandleOrientationsToLocalGpuMultiK(prepared, start, count, scales, waveIndex,
+                                                  localMs);
+    }
+    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1 || localMs.size() < scales.size())
+        return false;
+    m_fftStatistics->RecordCall(nFull, nLow, factor);
+    ScatteringRange lowSphere = m_sphere;
+    lowSphere.nAzimuth = nLow;
+    lowSphere.azinuthStep = 2.0 * M_PI / nLow;
+    lowSphere.ComputeSphereDirections(*m_incidentLight);
+    std::vector<Arr2D> lowMs;
+    lowMs.reserve(scales.size());
+    for (size_t s = 0; s < scales.size(); ++s)
+    {
+        lowMs.push_back(Arr2D(nLow + 1, nZen + 1, 4, 4));
+        lowMs.back().ClearArr();
+    }
+    const ScatteringRange saved = m_sphere;
+    m_sphere = lowSphere;
+    const bool ok = HandleOrientationsToLocalGpuMultiK(prepared, start, count, scales,
+                                                       waveIndex, lowMs);
+    m_sphere = saved;
+    if (!ok)
+        return false;
+    for (size_t s = 0; s < scales.size(); ++s)
+        UpsamplePhi(lowMs[s], nLow, nFull, nZen, localMs[s]);
+    return true;
 }
 
 // The reference's theta probe (src/cuda/GpuDiffraction.cu:4780-4965, kernel

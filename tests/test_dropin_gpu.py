@@ -191,6 +191,33 @@ def test_reference_cli_fft_phi_interpolation():
         assert g < 5e-2
 
 
+def test_reference_cli_multi_size_fft_phi():
+    """Multi-size mode with --fft: the fused HandleOrientationsToLocalGpuFftPhiMultiK
+    (MBS_GPU_MULTI_K_FULL=1) against the per-size phi-FFT calls the CLI makes
+    without it.  Both run the coarse grid (18 of 36 azimuths) and the same
+    upsampling, so they agree to rounding; the CPU backend has no --fft."""
+    wd = tempfile.mkdtemp(prefix="mbs_multisize_fft_")
+    args = ["--method", "po", "--particle", "1", "20", "10",
+            "--refractive-index", "1.3116", "1e-3", "--wavelength-um", "0.532",
+            "--max-reflections", "6", "--oldauto", "8", "--multigrid", "6", "10", "3",
+            "--scattering-grid", "0", "180", "36", "180", "--cutoff-profile", "off",
+            "--threads", str(_threads()), "--close", "--fft"]
+    env = {"MBS_HIP_PRECISION": "fp64"}
+    _run(HIP_BIN, args, wd, "per_size", "gpu", env)
+    env["MBS_GPU_MULTI_K_FULL"] = "1"
+    _run(HIP_BIN, args + ["--allow-experimental-environment"], wd, "fused", "gpu", env)
+    log = open(os.path.join(wd, "fused.log")).read()
+    assert "falling back to per-size GPU path" not in log
+    assert "direct Nphi=18, output Nphi=36" in open(os.path.join(wd, "per_size.log")).read()
+    sizes = sorted(f[len("per_size_"):-len(".dat")]
+                   for f in os.listdir(os.path.join(wd, "per_size")) if f.endswith(".dat"))
+    assert len(sizes) == 3, sizes
+    for size in sizes:
+        e = M.gate_l2(_dat(wd, "fused", "_" + size), _dat(wd, "per_size", "_" + size))
+        print("  size %s: fused vs per-size phi-FFT gate_l2 %.2e" % (size, e))
+        assert e < 1e-9
+
+
 @pytest.mark.parametrize("fused", [False, True], ids=["per_size_scaled", "fused_multi_scale"])
 def test_reference_cli_multi_size(fused):
     """The reference's shared-trace multi-size mode (--multigrid D_MIN D_MAX N,
