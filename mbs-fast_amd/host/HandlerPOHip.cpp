@@ -11,7 +11,7 @@
 // libmbs_hip.so; it contains no HIP code itself.
 //
 // Interfaces mirrored (signatures, argument meaning, error behaviour):
-//   src/handler/HandlerPO.h:98-146    HandlerPO GPU members
+//   src/handler/HandlerPO.h:98-162    HandlerPO GPU members
 //   src/cuda/GpuSupport.h:30-34       CheckGpuRuntime, QueryActiveGpuMemory,
 //                                     VisibleGpuDeviceCount, FormatGpuInfo
 // Semantics kept from the reference driver
@@ -31,8 +31,9 @@
 // FP32-storage switch, src/cuda/GpuDiffraction.cu:15-41) it runs the mixed
 // kernel (FP32 storage, FP64 phases and moments).  MBS_HIP_PRECISION=fp64 or
 // =fp32 overrides the build default at run time.
-// Multi-size (rescaled beam sets): the scale / waveIndex arguments and the
-// fused multi-scale member, with the reference's scale-on-pack rule.
+// Multi-size (rescaled beam sets): the scale / waveIndex arguments with the
+// reference's scale-on-pack rule, and the fused multi-scale member, which
+// packs once and applies the same rule on the device.
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
 // Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi and its
 // multi-size form): the direct path on a coarse azimuth grid and the
