@@ -37,7 +37,7 @@
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
 // Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi and its
 // multi-size form): the direct path on a coarse azimuth grid and the
-// reference's phi upsampling; its validation, theta and adaptive options
+// reference's phi upsampling and validation; its theta and adaptive options
 // return false.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
@@ -650,6 +650,58 @@ void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
         }
     }
 }
+// report_fft_check (src/cuda/GpuDiffraction.cu:3142-3195): the upsampled
+// grid against a reference grid -- M11 errors relative to the local M11 (and
+// to the cells with M11 >= 1e-3 of its maximum), and all 16 elements relative
+// to max(|M11|, that floor) -- printed; the estimate is the larger of the
+// significant-M11 and the element figures.
+double ReportFftCheck(const Arr2D &fftM, const Arr2D &refM, int nAz, int nZen, const char *name)
+{
+    double refMax = 0.0;
+    for (int p = 0; p < nAz; ++p)
+        for (int t = 0; t <= nZen; ++t)
+            refMax = std::max(refMax, std::fabs(refM.RawCell(p, t)[0]));
+    const double significant = 1e-3 * refMax;
+    double maxAbs = 0.0, maxRel = 0.0, sigRel = 0.0, normRel = 0.0, rms = 0.0;
+    int samples = 0, sigSamples = 0;
+    for (int p = 0; p < nAz; ++p)
+        for (int t = 0; t <= nZen; ++t)
+        {
+            const double *r = refM.RawCell(p, t), *f = fftM.RawCell(p, t);
+            const double absErr = std::fabs(f[0] - r[0]);
+            const double relErr = absErr / std::max(std::fabs(r[0]), 1e-12);
+            maxAbs = std::max(maxAbs, absErr);
+            maxRel = std::max(maxRel, relErr);
+            if (std::fabs(r[0]) >= significant)
+            {
+                sigRel = std::max(sigRel, relErr);
+                ++sigSamples;
+            }
+            const double scale = std::max(std::fabs(r[0]), significant);
+            for (int e = 0; e < 16; ++e)
+                normRel = std::max(normRel, std::fabs(f[e] - r[e]) / scale);
+            rms += relErr * relErr;
+            ++samples;
+        }
+    rms = std::sqrt(rms / std::max(1, samples));
+    std::fprintf(stderr,
+                 "GPU FFT check %s: M11 max_abs=%.6g max_rel=%.6g significant_max_rel=%.6g "
+                 "rms_rel=%.6g, Mueller/M11 max_rel=%.6g, samples=%d significant_samples=%d\n",
+                 name, maxAbs, maxRel, sigRel, rms, normRel, samples, sigSamples);
+    return std::max(sigRel, normRel);
+}
+
+void AddCells(const Arr2D &src, int nAz, int nZen, Arr2D &dst)
+{
+    for (int p = 0; p < nAz; ++p)
+        for (int t = 0; t <= nZen; ++t)
+        {
+            const double *a = src.RawCell(p, t);
+            double *b = dst.RawCell(p, t);
+            for (int e = 0; e < 16; ++e)
+                b[e] += a[e];
+        }
+}
 }  // namespace
 
 // The reference's angular Fourier interpolation (src/cuda/GpuDiffraction.cu:
@@ -657,9 +709,10 @@ void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
 // azimuths (at least 16) with the handler's zenith rows, and the Mueller grids
 // are upsampled in phi to the handler's nFull azimuths and added into localM
 // (and localM_noshadow).  A factor of 1, nFull < 32 or nLow >= nFull runs the
-// direct path.  The reference's validation and refinement (MBS_FFT_CHECK, a
-// tolerance), theta upsampling (MBS_FFT_THETA_FACTOR) and adaptive rows
-// (MBS_FFT_ADAPTIVE_PHI) are not provided: such a call returns false.
+// direct path.  With MBS_FFT_CHECK=1 or a tolerance the reference's
+// validation and refinement run too.  Its theta upsampling
+// (MBS_FFT_THETA_FACTOR) and adaptive rows (MBS_FFT_ADAPTIVE_PHI) are not
+// provided: such a call returns false.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &prepared,
                                                    int start, int count, Arr2D &localM,
                                                    Arr2D &localM_noshadow, double scale,
@@ -681,14 +734,14 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     }
     const char *check = std::getenv("MBS_FFT_CHECK");
     const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
-    if ((check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0
-        || EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1
+    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1
         || (adaptive && *adaptive && std::strcmp(adaptive, "0") != 0))
     {
-        std::fprintf(stderr, "HIP diffraction backend: FFT validation, theta upsampling and "
-                             "adaptive phi rows are not provided\n");
+        std::fprintf(stderr, "HIP diffraction backend: FFT theta upsampling and adaptive phi "
+                             "rows are not provided\n");
         return false;
     }
+    const bool doCheck = (check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
     static bool printed = false;
     if (!printed)
@@ -717,9 +770,63 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     m_sphere = saved;
     if (!ok)
         return false;
-    UpsamplePhi(lowM, nLow, nFull, nZen, localM);
+    if (!doCheck)
+    {
+        UpsamplePhi(lowM, nLow, nFull, nZen, localM);
+        if (ns)
+            UpsamplePhi(lowMns, nLow, nFull, nZen, localM_noshadow);
+        return true;
+    }
+    // validation (:3745-3782): a direct run on min(nFull, 2 nLow) azimuths,
+    // upsampled the same way, estimates the error; above the tolerance it
+    // replaces the interpolated grids
+    Arr2D fftM(nFull + 1, nZen + 1, 4, 4), fftMns(nFull + 1, nZen + 1, 4, 4);
+    fftM.ClearArr();
+    fftMns.ClearArr();
+    UpsamplePhi(lowM, nLow, nFull, nZen, fftM);
     if (ns)
-        UpsamplePhi(lowMns, nLow, nFull, nZen, localM_noshadow);
+        UpsamplePhi(lowMns, nLow, nFull, nZen, fftMns);
+    const int nCheck = std::min(nFull, nLow * 2);
+    ScatteringRange checkSphere = saved;
+    checkSphere.nAzimuth = nCheck;
+    checkSphere.azinuthStep = 2.0 * M_PI / nCheck;
+    checkSphere.ComputeSphereDirections(*m_incidentLight);
+    Arr2D checkLowM(nCheck + 1, nZen + 1, 4, 4), checkLowMns(nCheck + 1, nZen + 1, 4, 4);
+    checkLowM.ClearArr();
+    checkLowMns.ClearArr();
+    m_sphere = checkSphere;
+    const bool checkOk = HandleOrientationsToLocalGpu(prepared, start, count, checkLowM,
+                                                      checkLowMns, scale, waveIndex);
+    m_sphere = saved;
+    if (!checkOk)
+        return false;
+    Arr2D checkM(nFull + 1, nZen + 1, 4, 4), checkMns(nFull + 1, nZen + 1, 4, 4);
+    checkM.ClearArr();
+    checkMns.ClearArr();
+    UpsamplePhi(checkLowM, nCheck, nFull, nZen, checkM);
+    if (ns)
+        UpsamplePhi(checkLowMns, nCheck, nFull, nZen, checkMns);
+    double err = ReportFftCheck(fftM, checkM, nFull, nZen, "full");
+    if (ns)
+        err = std::max(err, ReportFftCheck(fftMns, checkMns, nFull, nZen, "no-shadow"));
+    const bool refine = m_fftTolerance > 0.0 && err > m_fftTolerance;
+    m_fftStatistics->RecordValidation(err, refine);
+    if (refine)
+    {
+        std::fprintf(stderr,
+                     "GPU FFT refinement: estimated error %.6g above tolerance %.6g, using the "
+                     "direct Nphi=%d grid\n",
+                     err, m_fftTolerance, nCheck);
+        AddCells(checkM, nFull, nZen, localM);
+        if (ns)
+            AddCells(checkMns, nFull, nZen, localM_noshadow);
+    }
+    else
+    {
+        AddCells(fftM, nFull, nZen, localM);
+        if (ns)
+            AddCells(fftMns, nFull, nZen, localM_noshadow);
+    }
     return true;
 }
 

@@ -189,6 +189,20 @@ def test_reference_cli_fft_phi_interpolation():
         g = M.gate_l2(a, b)
         print("--fft vs direct%s: gate_l2 %.2e" % (suffix or " (full)", g))
         assert g < 5e-2
+    # validation: the check grid here is min(96, 2 x 48) = the full grid, so
+    # a tolerance the estimate exceeds (--fft-tolerance) returns the direct
+    # result, and the check alone (MBS_FFT_CHECK=1, no tolerance) reports and
+    # keeps the interpolated one
+    _run(HIP_BIN, args + ["--fft", "--fft-tolerance", "1e-9"], wd, "fft_refined", "gpu")
+    _run(HIP_BIN, args + ["--fft", "--allow-experimental-environment"], wd, "fft_kept", "gpu",
+         {"MBS_FFT_CHECK": "1"})
+    for out in ("fft_refined", "fft_kept"):
+        assert "GPU FFT check full" in open(os.path.join(wd, out + ".log")).read()
+    for suffix in ("", "_noshadow"):
+        r = M.gate_l2(_dat(wd, "fft_refined", suffix), _dat(wd, "direct", suffix))
+        k = M.gate_l2(_dat(wd, "fft_kept", suffix), _dat(wd, "fft", suffix))
+        print("refined vs direct%s %.2e; kept vs --fft %.2e" % (suffix, r, k))
+        assert r < 1e-12 and k < 1e-12
 
 
 def test_reference_cli_multi_size_fft_phi():
