@@ -37,8 +37,8 @@
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
 // Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi and its
 // multi-size form): the direct path on a coarse azimuth grid and the
-// reference's phi upsampling and validation; its theta and adaptive options
-// return false.
+// reference's phi upsampling, validation and adaptive rows; its theta
+// upsampling option returns false.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
@@ -691,6 +691,86 @@ double ReportFftCheck(const Arr2D &fftM, const Arr2D &refM, int nAz, int nZen, c
     return std::max(sigRel, normRel);
 }
 
+double EnvDouble(const char *name, double fallback)
+{
+    const char *v = std::getenv(name);
+    if (!v || !*v)
+        return fallback;
+    char *end = nullptr;
+    const double x = std::strtod(v, &end);
+    return (end && *end == '\0' && x > 0.0) ? x : fallback;
+}
+
+// azimuth_average_output_cell (:3052-3084): one theta row's azimuth mean,
+// the linear-polarisation columns rotated by L(-phi) off the poles.
+void RowAverage(const Arr2D &src, int nAz, int nZen, int t, double out[16])
+{
+    for (int k = 0; k < 16; ++k)
+        out[k] = 0.0;
+    const bool pole = t == 0 || t == nZen;
+    for (int p = 0; p < nAz; ++p)
+    {
+        const double *m = src.RawCell(p, t);
+        if (pole)
+        {
+            for (int k = 0; k < 16; ++k)
+                out[k] += m[k];
+            continue;
+        }
+        const double a = -p * (2.0 * M_PI / (double)nAz);
+        const double cs = std::cos(2.0 * a), sn = std::sin(2.0 * a);
+        for (int r = 0; r < 4; ++r)
+        {
+            out[4 * r] += m[4 * r];
+            out[4 * r + 1] += m[4 * r + 1] * cs - m[4 * r + 2] * sn;
+            out[4 * r + 2] += m[4 * r + 1] * sn + m[4 * r + 2] * cs;
+            out[4 * r + 3] += m[4 * r + 3];
+        }
+    }
+    for (int k = 0; k < 16; ++k)
+        out[k] *= 1.0 / (double)nAz;
+}
+
+// detect_fft_phi_spike_rows (:3086-3135): interior theta rows whose averaged
+// off-diagonal ratios M_ij/M11 (ij = 23, 24, 32, 34, 42, 43, 13, 14) leave
+// the mean of the two neighbouring rows by at least `threshold`; the
+// `maxRows` highest scores, in row order.
+std::vector<int> SpikeRows(const Arr2D &m, int nAz, int nZen, double threshold, int maxRows)
+{
+    std::vector<int> rows;
+    if (maxRows <= 0 || nAz <= 0 || nZen < 2)
+        return rows;
+    const int elems[8] = {6, 7, 9, 11, 13, 14, 2, 3};
+    std::vector<std::pair<double, int>> scored;
+    double prev[16], cur[16], next[16];
+    for (int t = 1; t < nZen; ++t)
+    {
+        RowAverage(m, nAz, nZen, t - 1, prev);
+        RowAverage(m, nAz, nZen, t, cur);
+        RowAverage(m, nAz, nZen, t + 1, next);
+        const double c11 = std::max(std::fabs(cur[0]), 1e-30);
+        const double p11 = std::max(std::fabs(prev[0]), 1e-30);
+        const double n11 = std::max(std::fabs(next[0]), 1e-30);
+        double score = 0.0;
+        for (int e : elems)
+            score = std::max(score, std::fabs(cur[e] / c11 - 0.5 * (prev[e] / p11 + next[e] / n11)));
+        if (score >= threshold)
+            scored.push_back(std::make_pair(score, t));
+    }
+    std::sort(scored.begin(), scored.end(),
+              [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+                  return a.first > b.first;
+              });
+    for (const auto &s : scored)
+    {
+        rows.push_back(s.second);
+        if ((int)rows.size() >= maxRows)
+            break;
+    }
+    std::sort(rows.begin(), rows.end());
+    return rows;
+}
+
 void AddCells(const Arr2D &src, int nAz, int nZen, Arr2D &dst)
 {
     for (int p = 0; p < nAz; ++p)
@@ -710,9 +790,9 @@ void AddCells(const Arr2D &src, int nAz, int nZen, Arr2D &dst)
 // are upsampled in phi to the handler's nFull azimuths and added into localM
 // (and localM_noshadow).  A factor of 1, nFull < 32 or nLow >= nFull runs the
 // direct path.  With MBS_FFT_CHECK=1 or a tolerance the reference's
-// validation and refinement run too.  Its theta upsampling
-// (MBS_FFT_THETA_FACTOR) and adaptive rows (MBS_FFT_ADAPTIVE_PHI) are not
-// provided: such a call returns false.
+// validation and refinement run too, and with MBS_FFT_ADAPTIVE_PHI its
+// direct recomputation of spiking theta rows.  Its theta upsampling
+// (MBS_FFT_THETA_FACTOR) is not provided: such a call returns false.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &prepared,
                                                    int start, int count, Arr2D &localM,
                                                    Arr2D &localM_noshadow, double scale,
@@ -734,14 +814,13 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     }
     const char *check = std::getenv("MBS_FFT_CHECK");
     const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
-    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1
-        || (adaptive && *adaptive && std::strcmp(adaptive, "0") != 0))
+    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1)
     {
-        std::fprintf(stderr, "HIP diffraction backend: FFT theta upsampling and adaptive phi "
-                             "rows are not provided\n");
+        std::fprintf(stderr, "HIP diffraction backend: FFT theta upsampling is not provided\n");
         return false;
     }
     const bool doCheck = (check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0;
+    const bool adaptiveRows = adaptive && *adaptive && std::strcmp(adaptive, "0") != 0;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
     static bool printed = false;
     if (!printed)
@@ -770,7 +849,7 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     m_sphere = saved;
     if (!ok)
         return false;
-    if (!doCheck)
+    if (!doCheck && !adaptiveRows)
     {
         UpsamplePhi(lowM, nLow, nFull, nZen, localM);
         if (ns)
@@ -786,47 +865,96 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     UpsamplePhi(lowM, nLow, nFull, nZen, fftM);
     if (ns)
         UpsamplePhi(lowMns, nLow, nFull, nZen, fftMns);
-    const int nCheck = std::min(nFull, nLow * 2);
-    ScatteringRange checkSphere = saved;
-    checkSphere.nAzimuth = nCheck;
-    checkSphere.azinuthStep = 2.0 * M_PI / nCheck;
-    checkSphere.ComputeSphereDirections(*m_incidentLight);
-    Arr2D checkLowM(nCheck + 1, nZen + 1, 4, 4), checkLowMns(nCheck + 1, nZen + 1, 4, 4);
-    checkLowM.ClearArr();
-    checkLowMns.ClearArr();
-    m_sphere = checkSphere;
-    const bool checkOk = HandleOrientationsToLocalGpu(prepared, start, count, checkLowM,
-                                                      checkLowMns, scale, waveIndex);
-    m_sphere = saved;
-    if (!checkOk)
-        return false;
-    Arr2D checkM(nFull + 1, nZen + 1, 4, 4), checkMns(nFull + 1, nZen + 1, 4, 4);
-    checkM.ClearArr();
-    checkMns.ClearArr();
-    UpsamplePhi(checkLowM, nCheck, nFull, nZen, checkM);
-    if (ns)
-        UpsamplePhi(checkLowMns, nCheck, nFull, nZen, checkMns);
-    double err = ReportFftCheck(fftM, checkM, nFull, nZen, "full");
-    if (ns)
-        err = std::max(err, ReportFftCheck(fftMns, checkMns, nFull, nZen, "no-shadow"));
-    const bool refine = m_fftTolerance > 0.0 && err > m_fftTolerance;
-    m_fftStatistics->RecordValidation(err, refine);
-    if (refine)
+    if (doCheck)
     {
-        std::fprintf(stderr,
-                     "GPU FFT refinement: estimated error %.6g above tolerance %.6g, using the "
-                     "direct Nphi=%d grid\n",
-                     err, m_fftTolerance, nCheck);
-        AddCells(checkM, nFull, nZen, localM);
+        const int nCheck = std::min(nFull, nLow * 2);
+        ScatteringRange checkSphere = saved;
+        checkSphere.nAzimuth = nCheck;
+        checkSphere.azinuthStep = 2.0 * M_PI / nCheck;
+        checkSphere.ComputeSphereDirections(*m_incidentLight);
+        Arr2D checkLowM(nCheck + 1, nZen + 1, 4, 4), checkLowMns(nCheck + 1, nZen + 1, 4, 4);
+        checkLowM.ClearArr();
+        checkLowMns.ClearArr();
+        m_sphere = checkSphere;
+        const bool checkOk = HandleOrientationsToLocalGpu(prepared, start, count, checkLowM,
+                                                          checkLowMns, scale, waveIndex);
+        m_sphere = saved;
+        if (!checkOk)
+            return false;
+        Arr2D checkM(nFull + 1, nZen + 1, 4, 4), checkMns(nFull + 1, nZen + 1, 4, 4);
+        checkM.ClearArr();
+        checkMns.ClearArr();
+        UpsamplePhi(checkLowM, nCheck, nFull, nZen, checkM);
         if (ns)
-            AddCells(checkMns, nFull, nZen, localM_noshadow);
+            UpsamplePhi(checkLowMns, nCheck, nFull, nZen, checkMns);
+        double err = ReportFftCheck(fftM, checkM, nFull, nZen, "full");
+        if (ns)
+            err = std::max(err, ReportFftCheck(fftMns, checkMns, nFull, nZen, "no-shadow"));
+        const bool refine = m_fftTolerance > 0.0 && err > m_fftTolerance;
+        m_fftStatistics->RecordValidation(err, refine);
+        if (refine)
+        {
+            std::fprintf(stderr,
+                         "GPU FFT refinement: estimated error %.6g above tolerance %.6g, using the "
+                         "direct Nphi=%d grid\n",
+                         err, m_fftTolerance, nCheck);
+            fftM = checkM;
+            if (ns)
+                fftMns = checkMns;
+        }
     }
-    else
+    // adaptive rows (:3783-3832, MBS_FFT_ADAPTIVE_PHI): theta rows whose
+    // averaged polarisation ratios spike against their neighbours are
+    // computed directly on all nFull azimuths and replace the interpolated
+    // rows
+    if (adaptiveRows)
     {
-        AddCells(fftM, nFull, nZen, localM);
-        if (ns)
-            AddCells(fftMns, nFull, nZen, localM_noshadow);
+        const std::vector<int> rows =
+            SpikeRows(fftM, nFull, nZen, EnvDouble("MBS_FFT_ADAPTIVE_PHI_THRESHOLD", 0.04),
+                      EnvInt("MBS_FFT_ADAPTIVE_PHI_MAX_ROWS", 0, 128, 8));
+        if (!rows.empty())
+        {
+            ScatteringRange rowSphere = saved;
+            rowSphere.isNonUniform = true;
+            rowSphere.thetaValues.clear();
+            for (int t : rows)
+                rowSphere.thetaValues.push_back(saved.GetZenith(t));
+            rowSphere.nAzimuth = nFull;
+            rowSphere.azinuthStep = 2.0 * M_PI / nFull;
+            rowSphere.nZenith = (int)rows.size() - 1;
+            rowSphere.zenithStart = rowSphere.thetaValues.front();
+            rowSphere.zenithEnd = rowSphere.thetaValues.back();
+            rowSphere.zenithStep = rowSphere.nZenith > 0
+                                       ? (rowSphere.zenithEnd - rowSphere.zenithStart) /
+                                             rowSphere.nZenith
+                                       : 0.0;
+            rowSphere.ComputeSphereDirections(*m_incidentLight);
+            Arr2D directM(nFull + 1, (int)rows.size(), 4, 4);
+            Arr2D directMns(nFull + 1, (int)rows.size(), 4, 4);
+            directM.ClearArr();
+            directMns.ClearArr();
+            m_sphere = rowSphere;
+            const bool rowsOk = HandleOrientationsToLocalGpu(prepared, start, count, directM,
+                                                             directMns, scale, waveIndex);
+            m_sphere = saved;
+            if (!rowsOk)
+                return false;
+            for (size_t i = 0; i < rows.size(); ++i)
+                for (int p = 0; p < nFull; ++p)
+                {
+                    std::memcpy(fftM.RawCell(p, rows[i]), directM.RawCell(p, (int)i),
+                                16 * sizeof(double));
+                    if (ns)
+                        std::memcpy(fftMns.RawCell(p, rows[i]), directMns.RawCell(p, (int)i),
+                                    16 * sizeof(double));
+                }
+            std::fprintf(stderr, "GPU FFT adaptive phi: %zu of %d theta rows computed directly\n",
+                         rows.size(), nZen + 1);
+        }
     }
+    AddCells(fftM, nFull, nZen, localM);
+    if (ns)
+        AddCells(fftMns, nFull, nZen, localM_noshadow);
     return true;
 }
 

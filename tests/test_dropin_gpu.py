@@ -203,6 +203,21 @@ def test_reference_cli_fft_phi_interpolation():
         k = M.gate_l2(_dat(wd, "fft_kept", suffix), _dat(wd, "fft", suffix))
         print("refined vs direct%s %.2e; kept vs --fft %.2e" % (suffix, r, k))
         assert r < 1e-12 and k < 1e-12
+    # adaptive rows (MBS_FFT_ADAPTIVE_PHI, threshold low enough to take the
+    # maximum of 8): those theta rows are the direct ones, the others stay
+    # interpolated
+    _run(HIP_BIN, args + ["--fft", "--allow-experimental-environment"], wd, "fft_adaptive",
+         "gpu", {"MBS_FFT_ADAPTIVE_PHI": "1", "MBS_FFT_ADAPTIVE_PHI_THRESHOLD": "1e-9"})
+    assert "8 of %d theta rows computed directly" % len(THETA_CONFIG2) in open(
+        os.path.join(wd, "fft_adaptive.log")).read()
+    for suffix in ("", "_noshadow"):
+        a, d, f = (_dat(wd, n, suffix) for n in ("fft_adaptive", "direct", "fft"))
+        scale = np.abs(d[:, 2:3])
+        is_d = np.all(np.abs(a - d)[:, 2:] <= 1e-12 * scale, axis=1)
+        is_f = np.all(np.abs(a - f)[:, 2:] <= 1e-12 * scale, axis=1)
+        print("adaptive%s: %d rows direct, %d interpolated" % (suffix, (is_d & ~is_f).sum(),
+                                                              is_f.sum()))
+        assert np.all(is_d | is_f) and 1 <= (is_d & ~is_f).sum() <= 8
 
 
 def test_reference_cli_multi_size_fft_phi():
