@@ -45,6 +45,18 @@ def upsample_phi(low, n_full):
     return np.real(np.fft.ifft(spec, axis=0) * n_full)
 
 
+def upsample_theta(low, n_theta_full):
+    """The theta upsampling of the same path (fft_upsample_theta_arr2d,
+    GpuDiffraction.cu:3343-3420) along axis 1 of [n_az, n_theta_low+1, ...]:
+    each column extended evenly to 2 n_theta_low samples (row t and row
+    2 n_theta_low - t), upsampled by upsample_phi's rule to 2 n_theta_full,
+    rows 0..n_theta_full kept."""
+    n_low = 2 * (low.shape[1] - 1)
+    ext = np.concatenate([low, low[:, -2:0:-1]], axis=1)
+    up = upsample_phi(np.moveaxis(ext, 1, 0), 2 * n_theta_full)
+    return np.moveaxis(up, 0, 1)[:, :n_theta_full + 1]
+
+
 class HandlerPOGpu:
     def __init__(self, grid, consts, device=0, precision="fp64"):
         self.grid = grid
@@ -178,8 +190,9 @@ class HandlerPOGpu:
         Mueller grids upsampled in phi (upsample_phi) and added into the
         handler's grids.  factor: ``fftPhiFactor``, else MBS_FFT_PHI_FACTOR,
         else by nFull (choose_fft_phi_factor, :712-727); a factor of 1,
-        nFull < 32 or nLow >= nFull runs the direct path.  The validation,
-        theta and adaptive options return False."""
+        nFull < 32 or nLow >= nFull runs the direct path.  With
+        MBS_FFT_THETA_FACTOR the coarse grid also has nZen/factor rows
+        (upsample_theta).  The validation and adaptive options return False."""
         if not self.isCoh:
             return False
         g = self.grid
@@ -197,12 +210,20 @@ class HandlerPOGpu:
             return self.HandleOrientationsToLocalGpu(prepared, localM, localM_noshadow, start,
                                                      count, scale, waveIndex)
         if (os.environ.get("MBS_FFT_CHECK") == "1"
-                or os.environ.get("MBS_FFT_THETA_FACTOR", "1") not in ("", "1")
                 or os.environ.get("MBS_FFT_ADAPTIVE_PHI", "0") not in ("", "0")):
             return False
-        low = abi.Grid.build(g.theta, n_low)
+        # theta factor (gpu_fft_theta_factor, :532-542; :3543-3550): a uniform
+        # zenith grid of at least 32 rows that the factor divides
+        env = os.environ.get("MBS_FFT_THETA_FACTOR", "")
+        tf = int(env) if env.isdigit() and 1 <= int(env) <= 16 else 1
+        steps = np.diff(g.theta)
+        if tf > 1 and (g.n_zen < 32 or g.n_zen % tf or not np.allclose(steps, steps[0])):
+            tf = 1
+        n_zen_low = max(2, g.n_zen // tf) if tf > 1 else g.n_zen
+        theta_low = np.linspace(g.theta[0], g.theta[-1], n_zen_low + 1) if tf > 1 else g.theta
+        low = abi.Grid.build(theta_low, n_low)
         ns = self.ComputeNoShadow()
-        lm = np.zeros((n_low, g.n_zen + 1, 16))
+        lm = np.zeros((n_low, n_zen_low + 1, 16))
         lmn = np.zeros_like(lm) if ns else None
         ctx = self._context()
         self.grid = low
@@ -215,6 +236,9 @@ class HandlerPOGpu:
             ctx.set_grid(g)
         if not ok:
             return False
+        if tf > 1:
+            lm = upsample_theta(lm, g.n_zen)
+            lmn = upsample_theta(lmn, g.n_zen) if ns else None
         self._cells_view(localM, g.n_az, g.n_zen + 1)[:] += upsample_phi(lm, n_full)
         if ns:
             self._cells_view(localM_noshadow, g.n_az, g.n_zen + 1)[:] += upsample_phi(lmn, n_full)

@@ -37,8 +37,7 @@
 // Theta probe (DiffractThetasGpu): on the engine's incoherent FP64 sum.
 // Phi-FFT interpolation (HandleOrientationsToLocalGpuFftPhi and its
 // multi-size form): the direct path on a coarse azimuth grid and the
-// reference's phi upsampling, validation and adaptive rows; its theta
-// upsampling option returns false.
+// reference's phi and theta upsampling, validation and adaptive rows.
 #include "HandlerPO.h"
 #include "GpuSupport.h"
 
@@ -614,7 +613,7 @@ int ChoosePhiFactor(int nFull, int configured)
 // with the sum in closed form (Dirichlet kernel): sin((M + 1/2) a)/sin(a/2) for
 // nLow = 2M + 1, sin((M - 1/2) a)/sin(a/2) + cos(M a) for nLow = 2M,
 // a = 2 pi u, and nLow at u = 0.
-void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
+std::vector<double> UpsamplingMatrix(int nLow, int nFull)
 {
     std::vector<double> K((size_t)nFull * nLow);
     const int M = nLow / 2;
@@ -631,6 +630,12 @@ void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
             }
             K[(size_t)q * nLow + p] = k;
         }
+    return K;
+}
+
+void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
+{
+    const std::vector<double> K = UpsamplingMatrix(nLow, nFull);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t <= nZen; ++t)
     {
@@ -650,6 +655,60 @@ void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
         }
     }
 }
+// The theta upsampling of the same path (fft_upsample_theta_arr2d,
+// :3343-3420) for a uniform zenith grid: the nThetaLow + 1 rows of each
+// azimuth column extended evenly to a period of 2 nThetaLow (row t and row
+// 2 nThetaLow - t), upsampled by the rule above to 2 nThetaFull, rows
+// 0..nThetaFull added into dst (p < nAz).
+void UpsampleTheta(const Arr2D &low, int nAz, int nThetaLow, int nThetaFull, Arr2D &dst)
+{
+    const int nLow = 2 * nThetaLow, nFull = 2 * nThetaFull;
+    const std::vector<double> K = UpsamplingMatrix(nLow, nFull);
+#pragma omp parallel for schedule(static)
+    for (int p = 0; p < nAz; ++p)
+    {
+        std::vector<double> x((size_t)nLow * 16);
+        for (int t = 0; t < nLow; ++t)
+            std::memcpy(&x[(size_t)t * 16], low.RawCell(p, t <= nThetaLow ? t : nLow - t),
+                        16 * sizeof(double));
+        for (int q = 0; q <= nThetaFull; ++q)
+        {
+            double acc[16] = {0};
+            const double *kq = &K[(size_t)q * nLow];
+            for (int t = 0; t < nLow; ++t)
+                for (int e = 0; e < 16; ++e)
+                    acc[e] += kq[t] * x[(size_t)t * 16 + e];
+            double *cell = dst.RawCell(p, q);
+            for (int e = 0; e < 16; ++e)
+                cell[e] += acc[e];
+        }
+    }
+}
+
+// The coarse-grid sphere of the phi-FFT path (:3563-3570): nLow azimuths and,
+// with a theta factor, nZen / factor zenith rows.
+ScatteringRange CoarseSphere(const ScatteringRange &full, int nLow, int nZenLow,
+                             const Light &incident)
+{
+    ScatteringRange low = full;
+    low.nAzimuth = nLow;
+    low.azinuthStep = 2.0 * M_PI / nLow;
+    low.nZenith = nZenLow;
+    low.zenithStep = (low.zenithEnd - low.zenithStart) / nZenLow;
+    low.ComputeSphereDirections(incident);
+    return low;
+}
+
+// The theta factor (gpu_fft_theta_factor, :532-542, and :3543-3550): 1 for
+// non-uniform or short grids or when it does not divide nZen.
+int ThetaFactor(const ScatteringRange &sphere)
+{
+    const int f = EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1);
+    if (f <= 1 || sphere.isNonUniform || sphere.nZenith < 32 || sphere.nZenith % f != 0)
+        return 1;
+    return f;
+}
+
 // report_fft_check (src/cuda/GpuDiffraction.cu:3142-3195): the upsampled
 // grid against a reference grid -- M11 errors relative to the local M11 (and
 // to the cells with M11 >= 1e-3 of its maximum), and all 16 elements relative
@@ -790,9 +849,10 @@ void AddCells(const Arr2D &src, int nAz, int nZen, Arr2D &dst)
 // are upsampled in phi to the handler's nFull azimuths and added into localM
 // (and localM_noshadow).  A factor of 1, nFull < 32 or nLow >= nFull runs the
 // direct path.  With MBS_FFT_CHECK=1 or a tolerance the reference's
-// validation and refinement run too, and with MBS_FFT_ADAPTIVE_PHI its
-// direct recomputation of spiking theta rows.  Its theta upsampling
-// (MBS_FFT_THETA_FACTOR) is not provided: such a call returns false.
+// validation and refinement run too, with MBS_FFT_ADAPTIVE_PHI its direct
+// recomputation of spiking theta rows, and with MBS_FFT_THETA_FACTOR (a
+// uniform zenith grid of at least 32 rows that the factor divides) the
+// coarse grid has nZen / factor rows, upsampled in theta first.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &prepared,
                                                    int start, int count, Arr2D &localM,
                                                    Arr2D &localM_noshadow, double scale,
@@ -814,11 +874,8 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     }
     const char *check = std::getenv("MBS_FFT_CHECK");
     const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
-    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1)
-    {
-        std::fprintf(stderr, "HIP diffraction backend: FFT theta upsampling is not provided\n");
-        return false;
-    }
+    const int thetaFactor = ThetaFactor(m_sphere);
+    const int nZenLow = thetaFactor > 1 ? std::max(2, nZen / thetaFactor) : nZen;
     const bool doCheck = (check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0;
     const bool adaptiveRows = adaptive && *adaptive && std::strcmp(adaptive, "0") != 0;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
@@ -829,26 +886,41 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
                      "GPU FFT angular interpolation: direct Nphi=%d, output Nphi=%d (factor=%d), "
                      "direct Ntheta=%d, output Ntheta=%d (factor=%d). "
                      "This is angular Fourier interpolation, not aperture pFFT/FMM.\n",
-                     nLow, nFull, factor, nZen + 1, nZen + 1, 1);
+                     nLow, nFull, factor, nZenLow + 1, nZen + 1, thetaFactor);
         printed = true;
     }
     const bool ns = ComputeNoShadow();
-    ScatteringRange lowSphere = m_sphere;
-    lowSphere.nAzimuth = nLow;
-    lowSphere.azinuthStep = 2.0 * M_PI / nLow;
-    lowSphere.ComputeSphereDirections(*m_incidentLight);
-    Arr2D lowM(nLow + 1, nZen + 1, 4, 4);
-    lowM.ClearArr();
-    Arr2D lowMns(ns ? nLow + 1 : 0, ns ? nZen + 1 : 0, ns ? 4 : 0, ns ? 4 : 0);
+    const ScatteringRange lowSphere = CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight);
+    Arr2D coarseM(nLow + 1, nZenLow + 1, 4, 4);
+    coarseM.ClearArr();
+    Arr2D coarseMns(ns ? nLow + 1 : 0, ns ? nZenLow + 1 : 0, ns ? 4 : 0, ns ? 4 : 0);
     if (ns)
-        lowMns.ClearArr();
+        coarseMns.ClearArr();
     const ScatteringRange saved = m_sphere;
     m_sphere = lowSphere;
-    const bool ok = HandleOrientationsToLocalGpu(prepared, start, count, lowM, lowMns, scale,
-                                                 waveIndex);
+    const bool ok = HandleOrientationsToLocalGpu(prepared, start, count, coarseM, coarseMns,
+                                                 scale, waveIndex);
     m_sphere = saved;
     if (!ok)
         return false;
+    // theta first (to the handler's rows), then phi
+    Arr2D lowM(nLow + 1, nZen + 1, 4, 4), lowMns(nLow + 1, nZen + 1, 4, 4);
+    if (thetaFactor > 1)
+    {
+        lowM.ClearArr();
+        UpsampleTheta(coarseM, nLow, nZenLow, nZen, lowM);
+        if (ns)
+        {
+            lowMns.ClearArr();
+            UpsampleTheta(coarseMns, nLow, nZenLow, nZen, lowMns);
+        }
+    }
+    else
+    {
+        lowM = coarseM;
+        if (ns)
+            lowMns = coarseMns;
+    }
     if (!doCheck && !adaptiveRows)
     {
         UpsamplePhi(lowM, nLow, nFull, nZen, localM);
@@ -1027,8 +1099,8 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
 // The multi-size form of the phi-FFT path (src/cuda/GpuDiffraction.cu, the
 // member after :3839): only with MBS_GPU_MULTI_K_FULL=1 and full output, as
 // in the reference; the fused multi-size call on the coarse azimuth grid,
-// then every scale's grid upsampled in phi into localMs[s].  The options
-// the single-size path does not provide return false here too.
+// then every scale's grid upsampled (theta, then phi) into localMs[s].  As
+// in the reference, its validation and adaptive options return false.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
     const std::vector<PreparedOrientation> &prepared, int start, int count,
     const std::vector<double> &scales, double waveIndex, std::vector<Arr2D> &localMs)
@@ -1055,18 +1127,17 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
         return HandleOrientationsToLocalGpuMultiK(prepared, start, count, scales, waveIndex,
                                                   localMs);
     }
-    if (EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1) > 1 || localMs.size() < scales.size())
+    if (localMs.size() < scales.size())
         return false;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
-    ScatteringRange lowSphere = m_sphere;
-    lowSphere.nAzimuth = nLow;
-    lowSphere.azinuthStep = 2.0 * M_PI / nLow;
-    lowSphere.ComputeSphereDirections(*m_incidentLight);
+    const int thetaFactor = ThetaFactor(m_sphere);
+    const int nZenLow = thetaFactor > 1 ? std::max(2, nZen / thetaFactor) : nZen;
+    const ScatteringRange lowSphere = CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight);
     std::vector<Arr2D> lowMs;
     lowMs.reserve(scales.size());
     for (size_t s = 0; s < scales.size(); ++s)
     {
-        lowMs.push_back(Arr2D(nLow + 1, nZen + 1, 4, 4));
+        lowMs.push_back(Arr2D(nLow + 1, nZenLow + 1, 4, 4));
         lowMs.back().ClearArr();
     }
     const ScatteringRange saved = m_sphere;
@@ -1077,7 +1148,17 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
     if (!ok)
         return false;
     for (size_t s = 0; s < scales.size(); ++s)
-        UpsamplePhi(lowMs[s], nLow, nFull, nZen, localMs[s]);
+    {
+        if (thetaFactor > 1)
+        {
+            Arr2D thetaM(nLow + 1, nZen + 1, 4, 4);
+            thetaM.ClearArr();
+            UpsampleTheta(lowMs[s], nLow, nZenLow, nZen, thetaM);
+            UpsamplePhi(thetaM, nLow, nFull, nZen, localMs[s]);
+        }
+        else
+            UpsamplePhi(lowMs[s], nLow, nFull, nZen, localMs[s]);
+    }
     return true;
 }
 

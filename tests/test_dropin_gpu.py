@@ -220,6 +220,32 @@ def test_reference_cli_fft_phi_interpolation():
         assert np.all(is_d | is_f) and 1 <= (is_d & ~is_f).sum() <= 8
 
 
+def test_reference_cli_fft_theta_factor():
+    """--fft with MBS_FFT_THETA_FACTOR=2 on a uniform 181-row grid: the
+    coarse grid has 91 rows and 48 azimuths, upsampled in theta, then in phi.
+    Trigonometric interpolation passes through its samples, so the even rows
+    (the coarse rows) must equal plain --fft's (181 direct rows, the same phi
+    upsampling) to rounding; the odd rows are interpolated.  (Their error is
+    large here -- 2 degree rows undersample this column's forward peak; the
+    oracle restatement of the same upsampling gives the same figure.)"""
+    wd = tempfile.mkdtemp(prefix="mbs_fft_theta_")
+    args = ["--method", "po", "--particle", "1", "20", "10", "--refractive-index", "1.3116",
+            "1e-4", "--wavelength-um", "0.532", "--max-reflections", "6", "--sobol-seed", "8",
+            "1", "--scattering-grid", "0", "180", "96", "180", "--cutoff-profile", "off",
+            "--threads", str(_threads()), "--close", "--fft"]
+    _run(HIP_BIN, args, wd, "fft", "gpu")
+    _run(HIP_BIN, args + ["--allow-experimental-environment"], wd, "fft_theta", "gpu",
+         {"MBS_FFT_THETA_FACTOR": "2"})
+    log = open(os.path.join(wd, "fft_theta.log")).read()
+    assert "direct Ntheta=91, output Ntheta=181 (factor=2)" in log
+    a, b = _dat(wd, "fft_theta"), _dat(wd, "fft")
+    np.testing.assert_array_equal(a[:, :2], b[:, :2])
+    rel = np.max(np.abs(a[:, 2:] - b[:, 2:]), axis=1) / np.abs(b[:, 2])
+    print("--fft theta factor 2 vs --fft: even rows max rel %.2e, odd rows max rel %.2e, "
+          "gate_l2 %.2e" % (rel[0::2].max(), rel[1::2].max(), M.gate_l2(a, b)))
+    assert rel[0::2].max() < 1e-9 and rel[1::2].max() > 1e-6
+
+
 def test_reference_cli_multi_size_fft_phi():
     """Multi-size mode with --fft: the fused HandleOrientationsToLocalGpuFftPhiMultiK
     (MBS_GPU_MULTI_K_FULL=1) against the per-size phi-FFT calls the CLI makes

@@ -266,6 +266,22 @@ def dirichlet_upsampling_matrix(n_low, n_full):
     return np.where(zero, 1.0, k)
 
 
+def test_theta_upsampling_reproduces_cosine_series(pkg):
+    """The theta upsampling's even extension: a cosine series of degree
+    < n_theta_low / 2 sampled on the coarse uniform rows is reproduced on
+    the fine rows."""
+    rng = np.random.default_rng(8)
+    for nt_low, nt_full in ((16, 32), (45, 90), (60, 180)):
+        c = rng.normal(size=nt_low // 2)
+        def f(t):
+            return sum(c[j] * np.cos(j * t) for j in range(len(c)))
+        low = np.tile(f(np.linspace(0, np.pi, nt_low + 1))[None, :, None], (2, 1, 16))
+        up = pkg.handler.upsample_theta(low, nt_full)
+        assert up.shape == (2, nt_full + 1, 16)
+        np.testing.assert_allclose(up[1, :, 5], f(np.linspace(0, np.pi, nt_full + 1)),
+                                   rtol=0, atol=1e-12 * np.abs(c).sum())
+
+
 def test_phi_upsampling_forms(pkg):
     """The phi upsampling of the FFT path, two ways: the reference's FFT
     padding (handler.upsample_phi) and the closed-form matrix of the C++
