@@ -220,6 +220,43 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     h[H_MOM + 2] = k * k * mXX;
     h[H_MOM + 3] = 2.0 * k * k * mXY;
     h[H_MOM + 4] = k * k * mYY;
+    // Orders 3-5.  Over the triangle (0, P, Q) of each edge, with x = s Px +
+    // t Qx, y = s Py + t Qy and the simplex integral of s^i t^j equal to
+    // i! j! / (i + j + 2)!, the integral of x^a y^b is det(P, Q) times
+    //   sum_{i<=a, j<=b} C(a,i) C(b,j) Px^i Qx^(a-i) Py^j Qy^(b-j)
+    //                    (i+j)! (a+b-i-j)! / (a+b+2)!.
+    // Summed over the edges it is the polygon's moment; divided by the area
+    // (tw / 2), the mean <x^a y^b>.
+    {
+        const double fact[8] = {1, 1, 2, 6, 24, 120, 720, 5040};
+        const double binom[6][6] = {{1, 0, 0, 0, 0, 0}, {1, 1, 0, 0, 0, 0},
+                                    {1, 2, 1, 0, 0, 0}, {1, 3, 3, 1, 0, 0},
+                                    {1, 4, 6, 4, 1, 0}, {1, 5, 10, 10, 5, 1}};
+        const bool ok = !(fabs(tw) <= 0x1p-40 * areaScale);
+        const int base_of[6] = {0, 0, 0, H_MOM3, H_MOM4, H_MOM5};
+        double kn = k * k;
+        for (int n = 3; n <= 5; ++n) {
+            kn *= k;
+            for (int a = 0; a <= n; ++a) {
+                const int b2 = n - a;
+                double sum = 0.0;
+                for (int e = 0; e < nv; ++e) {
+                    const int f = (e + 1 < nv) ? e + 1 : 0;
+                    const double px = X(e), py = Y(e), qx = X(f), qy = Y(f);
+                    const double det = px * qy - qx * py;
+                    double acc = 0.0;
+                    for (int i = 0; i <= a; ++i)
+                        for (int j = 0; j <= b2; ++j)
+                            acc += binom[a][i] * binom[b2][j] * pow(px, i) * pow(qx, a - i) *
+                                   pow(py, j) * pow(qy, b2 - j) * fact[i + j] *
+                                   fact[n - i - j];
+                    sum += det * acc;
+                }
+                const double mean = ok ? sum / fact[n + 2] * (2.0 / tw) : 0.0;
+                h[base_of[n] + a] = kn * binom[n][a] * mean / fact[n];
+            }
+        }
+    }
     // Small-phase bound.  The vertex phases k (A x + B y) spread over at
     // least k |(A, B)| w, w the least width of the vertex set, so some phase
     // exceeds 1e-3 rad once max(|A|, |B|) > 2e-3 / (k w); the kernels test
@@ -354,12 +391,12 @@ __device__ __forceinline__ double uniform_f64(double x)
 // (through parallel facets) meets it with A = B = 0 on the theta = 0 row,
 // whose azimuth cells lie in as many different waves.
 __device__ __forceinline__ bool all_phases_small(const double *__restrict__ vtx, int nv, double A,
-                                                 double B)
+                                                 double B, double limit = kSmallPhase)
 {
     bool small = true;
     for (int e = 0; e < nv; ++e) {
         const double *v = vtx + (size_t)e * VTX_STRIDE;
-        small = small && __builtin_fabs(__builtin_fma(A, v[V_KX], B * v[V_KY])) <= kSmallPhase;
+        small = small && __builtin_fabs(__builtin_fma(A, v[V_KX], B * v[V_KY])) <= limit;
     }
     return small;
 }
@@ -668,9 +705,16 @@ __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__
     const float nrm = rcp_f32((float)(useB ? B : -A));
     float fr = sr * nrm, fi = si * nrm;
     if (__builtin_expect(small, 0)) {
+        // <e^{i phi}> = sum_n i^n <phi^n> / n! to fifth order
         const double p1 = A * h[H_MOM] + B * h[H_MOM + 1];
         const double p2 = A * A * h[H_MOM + 2] + A * B * h[H_MOM + 3] + B * B * h[H_MOM + 4];
-        const double gr = 1.0 - 0.5 * p2, gi = p1;
+        const double A2 = A * A, B2 = B * B, AB = A * B;
+        const double *c3 = h + H_MOM3, *c4 = h + H_MOM4, *c5 = h + H_MOM5;
+        const double q3 = B * (B2 * c3[0] + AB * c3[1]) + A * (AB * c3[2] + A2 * c3[3]);
+        const double q4 = B2 * (B2 * c4[0] + AB * c4[1]) + A2 * (B2 * c4[2] + AB * c4[3] + A2 * c4[4]);
+        const double q5 = B * B2 * (B2 * c5[0] + AB * c5[1] + A2 * c5[2]) +
+                          A * A2 * (B2 * c5[3] + AB * c5[4] + A2 * c5[5]);
+        const double gr = 1.0 - 0.5 * p2 + q4, gi = p1 - q3 + q5;
         const double ar = h[H_AL] * gr - h[H_AL + 1] * gi;
         const double ai = h[H_AL] * gi + h[H_AL + 1] * gr;
         float sd, cd;
@@ -777,7 +821,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
 #pragma unroll
         for (int e = 0; e < NV; ++e) {
             const double p = __builtin_fma(A, kx[e], B * ky[e]);
-            small = small && __builtin_fabs(p) <= kSmallPhase;
+            small = small && __builtin_fabs(p) <= kSmallPhaseMixed;
             sincos_reduce_f32(p + psi, tab, dv[e], ev[e]);
         }
         float sv[NV], cv[NV];
@@ -796,8 +840,11 @@ __device__ __forceinline__ void aperture_mixed_nv(
                               si);
     } else {
         small = false;
-        if (__builtin_expect(__builtin_fabsf((float)(useB ? B : -A)) <= (float)h[H_PHB], 0))
-            small = all_phases_small(vtx, nv, A, B);
+        // H_PHB is the bound for 1e-3 rad; it scales with the phase limit
+        if (__builtin_expect(__builtin_fabsf((float)(useB ? B : -A)) <=
+                                 (float)(h[H_PHB] * (kSmallPhaseMixed / kSmallPhase)),
+                             0))
+            small = all_phases_small(vtx, nv, A, B, kSmallPhaseMixed);
         float s0, c0;
         sincos_table_f32(__builtin_fma(A, vtx[V_KX], __builtin_fma(B, vtx[V_KY], psi)), tab, s0,
                          c0);

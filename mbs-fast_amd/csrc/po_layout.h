@@ -20,7 +20,12 @@ enum {
     H_T0 = 20,   // 1/2 (NT, nxDT)   (6)
     H_T1 = 26,   // 1/2 (NP, nxDP)   (6)
     H_JP = 32,   // complWave * J_phased (8)
-    HDR_STRIDE = 40
+    // moments of orders 3, 4, 5 for the mixed kernels' moment form: entry a
+    // of order n multiplies A^a B^(n-a) and holds k^n C(n,a) <x^a y^(n-a)> / n!
+    H_MOM3 = 40,  // (4)
+    H_MOM4 = 44,  // (5)
+    H_MOM5 = 49,  // (6)
+    HDR_STRIDE = 56
 };
 
 // FP32 copy of the rotation data (T0, T1, complWave * J_phased) used by the

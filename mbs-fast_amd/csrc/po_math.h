@@ -19,6 +19,13 @@ constexpr double kPi = 3.141592653589793116;            // double(pi) = M_PI
 // Small-phase switch: every vertex phase within 1e-3 rad
 // (HandlerPO_fast.h:161-216).
 constexpr double kSmallPhase = 1e-3;
+// The mixed-precision kernels' switch.  Their edge sum differences FP32
+// phasors (error ~1e-7 each) whose sum cancels to O(phase^2) of a single
+// term, so its relative error grows like 1e-7 / phase^2 -- 1e-1 at 1e-3 rad.
+// They take the moment form up to 0.25 rad instead, with polygon moments to
+// fifth order (truncation below max|phase|^6 / 6! = 3.4e-7 there), where the
+// edge sum's error has fallen to ~1e-5.
+constexpr double kSmallPhaseMixed = 0.25;
 // One Horner step p*z + c as a single three-operand v_fma_f64 with the
 // coefficient in an SGPR pair.  Written as inline assembly because the
 // compiler otherwise keeps loop-invariant FP64 coefficients in VGPRs and
