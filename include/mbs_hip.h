@@ -234,6 +234,31 @@ int mbs_azimuth_average(mbs_ctx *ctx, int noshadow, double *rows);
  * orientation's weight, as mbs_fetch_per_orient returns it). */
 int mbs_azimuth_average_per_orient(mbs_ctx *ctx, double *rows);
 
+/* Band-limited (Fourier) upsampling of the last mbs_diffract result on the
+ * device: the interpolation step of the reference's angular FFT path
+ * (fft_upsample_phi_arr2d / fft_upsample_theta_arr2d and the pad kernel,
+ * src/cuda/GpuDiffraction.cu:3195-3420, :794-814).  The result grid is taken
+ * as samples of a function periodic in azimuth (n_az samples per period)
+ * and, when n_zen_out > n_zen, even about both ends of the zenith range
+ * (2 n_zen samples per period; the zenith rows must be uniformly spaced);
+ * each line is resampled to n_az_out azimuths and n_zen_out + 1 rows over
+ * the same range: forward DFT, the spectrum zero-padded (an even count's
+ * Nyquist term split evenly between +n/2 and -n/2, which for real data is
+ * the real part of the one-sided placement), inverse DFT, scale 1/n.
+ * Zenith first, then azimuth; FP64 throughout.  Sizes may only grow; every
+ * line that is resampled is limited to 4096 samples (n_az_out, 2 n_zen_out),
+ * else MBS_ERR_CAPACITY.
+ *   noshadow    upsample the no-shadow result (needs MBS_NOSHADOW)
+ *   grid_out    NULL, or [n_az_out*(n_zen_out+1)*16], written (accumulate =
+ *               0) or added to (accumulate != 0), cell (q, t) at
+ *               q*(n_zen_out+1)+t
+ *   rows_out    NULL, or [n_zen_out+1][18]: the azimuth average of the
+ *               upsampled grid as mbs_azimuth_average computes it; theta_out
+ *               [n_zen_out+1] (radians) gives its rows' zenith angles */
+int mbs_upsample_result(mbs_ctx *ctx, int noshadow, int32_t n_az_out, int32_t n_zen_out,
+                        const double *theta_out, int accumulate, double *grid_out,
+                        double *rows_out);
+
 /* Orientation sets of the orientation average, generated on the device
  * (one row of MBS_ORIENT_STRIDE doubles per orientation: beta, gamma, weight,
  * qx, qy, qz, qw, R00..R22 of the particle rotation):
@@ -348,7 +373,7 @@ int mbs_set_work_mode(mbs_ctx *ctx, int mode);
 /* Autotuner of the launch shape (no counterpart in the reference, whose CUDA
  * path sizes its launch from fixed heuristics, GpuDiffraction.cu:2948-3029):
  * times mbs_diffract(flags) on the uploaded batch for every candidate
- * (target workgroups 1024..32768 x whole-orientation / beam-segment work
+ * (target workgroups 4096..131072 x whole-orientation / beam-segment work
  * items x, for mixed precision, MFMA / vector-ALU coherent sum), keeps the
  * fastest for later mbs_diffract calls of the same precision, and reports
  * what it timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.

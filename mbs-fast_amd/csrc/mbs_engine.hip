@@ -22,6 +22,7 @@
 #include "orient_kernels.hip"
 #include "fft_kernels.hip"
 #include "mlp_kernels.hip"
+#include "upsample_kernels.hip"
 
 using namespace mbs;
 
@@ -90,6 +91,9 @@ struct mbs_ctx {
     // work and results
     DevBuf<WorkItem> items;
     DevBuf<double> part, part_ns, res, res_ns, per_orient, avg_rows, orient;
+    // Fourier upsampling of a result (mbs_upsample_result)
+    DevBuf<double2> up_roots_in, up_roots_out;
+    DevBuf<double> up_a, up_b, up_theta;
     // aperture FFT
     DevBuf<float2> fft_grid, fft_tw;
     DevBuf<double> fft_rec, fft_ab, fft_out;
@@ -266,6 +270,11 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->orient.release();
     if (ctx->graph_exec)
         (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->up_roots_in.release();
+    ctx->up_roots_out.release();
+    ctx->up_a.release();
+    ctx->up_b.release();
+    ctx->up_theta.release();
     ctx->fft_grid.release();
     ctx->fft_tw.release();
     ctx->fft_rec.release();
@@ -628,9 +637,13 @@ static void launch_diffract(int prec, int ns, int jones, int per, const KernelAr
 
 // Launches the diffraction kernel and its reduction on the context's stream:
 // kernels only (no copies, no memsets), so the sequence can be captured in a
-// graph.  With `events`, ev[2..3] bracket the main kernel.
-static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, int per, bool jones, int per_k,
-                             const KernelArgs &a, int blocks, int n_items, bool events)
+// graph.  The partial sums and the work list are a.out / a.out_ns / a.items;
+// the reduction writes res (res_ns with NS; per_res, the per-orientation
+// grids, in Jones mode with PER).  With `events`, ev[2..3] bracket the main
+// kernel.
+static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, bool jones, int per_k,
+                             const KernelArgs &a, int blocks, int n_items, double *res,
+                             double *res_ns, double *per_res, bool events)
 {
     hipStream_t s = ctx->stream;
     const int cells = ctx->cells;
@@ -644,28 +657,52 @@ static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, int per, bool jones
     const int rt = 256;
     if (jones) {
         jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-            ctx->part.p, ctx->items.p, n_items, cells, ctx->n_az, ctx->weights.p, ctx->res.p,
-            per ? ctx->per_orient.p : nullptr);
+            a.out, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res, per_res);
         HIP_TRY(ctx, hipGetLastError());
         if (ns) {
             jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-                ctx->part_ns.p, ctx->items.p, n_items, cells, ctx->n_az, ctx->weights.p,
-                ctx->res_ns.p,
-                nullptr);
+                a.out_ns, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res_ns, nullptr);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {
         const int rb = (int)((mcount + rt - 1) / rt);
-        reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part.p, n_items, cells, ctx->n_az,
-                                                ctx->res.p);
+        reduce_mueller_kernel<<<rb, rt, 0, s>>>(a.out, n_items, cells, ctx->n_az, res);
         HIP_TRY(ctx, hipGetLastError());
         if (ns) {
-            reduce_mueller_kernel<<<rb, rt, 0, s>>>(ctx->part_ns.p, n_items, cells, ctx->n_az,
-                                                    ctx->res_ns.p);
+            reduce_mueller_kernel<<<rb, rt, 0, s>>>(a.out_ns, n_items, cells, ctx->n_az, res_ns);
             HIP_TRY(ctx, hipGetLastError());
         }
     }
     return MBS_OK;
+}
+
+// The kernel arguments that come from the grid and the uploaded batch.
+static void batch_args(const mbs_ctx *ctx, int prec, int n_items, int n_tiles, KernelArgs &a)
+{
+    std::memset(&a, 0, sizeof a);  // padding too: the struct is a graph-cache key
+    a.n_items = n_items;
+    a.n_tiles = n_tiles;
+    a.cells = ctx->cells;
+    a.n_az = ctx->n_az;
+    a.tab = ctx->tab.p;
+    a.hdr = ctx->hdr.p;
+    a.hdr32 = ctx->hdr32.p;
+    a.wb = ctx->wb.p;
+    a.vtx = ctx->vtx.p;
+    a.vdd = ctx->vdd.p;
+    a.vflag = ctx->vflag.p;
+    a.nv = ctx->nv.p;
+    a.voff = ctx->voff.p;
+    a.ext = ctx->ext.p;
+    a.ooff = ctx->ooff.p;
+    a.weights = ctx->weights.p;
+    a.k = ctx->consts.wave_index;
+    // The mixed kernel forms edge phasor differences in FP32, so its
+    // switch to the exact half-angle edge form happens at |C| <= 3e-5, the
+    // floor the reference applies to its FP32 build (GpuDiffraction.cu:498-512).
+    // The FP64 kernels (prec 0, and 3 = the FP64 incoherent sum) keep the
+    // handler's eps1.
+    a.eps1 = (prec == 1 || prec == 2) ? std::max(ctx->consts.eps1, 3e-5) : ctx->consts.eps1;
 }
 
 extern "C" {
@@ -731,29 +768,8 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     HIP_TRY(ctx, hipMemcpyAsync(ctx->items.p, ctx->h_items.data(), sizeof(WorkItem) * n_items,
                                 hipMemcpyHostToDevice, s));
     KernelArgs a;
-    std::memset(&a, 0, sizeof a);  // padding too: the struct is a graph-cache key
+    batch_args(ctx, prec, n_items, n_tiles, a);
     a.items = ctx->items.p;
-    a.n_items = n_items;
-    a.n_tiles = n_tiles;
-    a.cells = cells;
-    a.n_az = ctx->n_az;
-    a.tab = ctx->tab.p;
-    a.hdr = ctx->hdr.p;
-    a.hdr32 = ctx->hdr32.p;
-    a.wb = ctx->wb.p;
-    a.vtx = ctx->vtx.p;
-    a.vdd = ctx->vdd.p;
-    a.vflag = ctx->vflag.p;
-    a.nv = ctx->nv.p;
-    a.voff = ctx->voff.p;
-    a.ext = ctx->ext.p;
-    a.ooff = ctx->ooff.p;
-    a.weights = ctx->weights.p;
-    a.k = ctx->consts.wave_index;
-    // The mixed kernel forms edge phasor differences in FP32, so its
-    // switch to the exact half-angle edge form happens at |C| <= 3e-5, the
-    // floor the reference applies to its FP32 build (GpuDiffraction.cu:498-512).
-    a.eps1 = prec ? std::max(ctx->consts.eps1, 3e-5) : ctx->consts.eps1;
     a.out = ctx->part.p;
     a.out_ns = ns ? ctx->part_ns.p : nullptr;
     a.per_orient = (per && !jones) ? ctx->per_orient.p : nullptr;
@@ -780,8 +796,9 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
             }
             hipGraph_t g = nullptr;
             HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc2 = issue_diffraction(ctx, prec, ns, per, jones, per_k, a, blocks,
-                                              n_items, false);
+            const int rc2 = issue_diffraction(ctx, prec, ns, jones, per_k, a, blocks, n_items,
+                                              ctx->res.p, ctx->res_ns.p,
+                                              per ? ctx->per_orient.p : nullptr, false);
             hipError_t ce = hipStreamEndCapture(s, &g);
             if (rc2)
                 return rc2;
@@ -801,8 +818,9 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         if (ctx->timing)
             HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     } else {
-        const int rc2 = issue_diffraction(ctx, prec, ns, per, jones, per_k, a, blocks, n_items,
-                                          ctx->timing);
+        const int rc2 = issue_diffraction(ctx, prec, ns, jones, per_k, a, blocks, n_items,
+                                          ctx->res.p, ctx->res_ns.p,
+                                          per ? ctx->per_orient.p : nullptr, ctx->timing);
         if (rc2)
             return rc2;
     }
@@ -982,6 +1000,119 @@ int mbs_azimuth_average_per_orient(mbs_ctx *ctx, double *rows)
         ctx->avg_rows.p);
     HIP_TRY(ctx, hipGetLastError());
     return fetch(ctx, ctx->avg_rows.p, n, rows, false);
+}
+
+}  // extern "C"
+
+// Radices of an FFT of length n: 4, 2, 3, 5, 7, then the other primes.
+static bool up_factors(int n, UpFactors &f)
+{
+    f.n = n;
+    f.count = 0;
+    int m = n;
+    for (int r : {4, 2, 3, 5, 7})
+        while (m > 1 && m % r == 0) {
+            if (f.count == kUpMaxFactors)
+                return false;
+            f.r[f.count++] = r;
+            m /= r;
+        }
+    for (int p = 11; m > 1; p += 2)
+        while (m % p == 0) {
+            if (f.count == kUpMaxFactors)
+                return false;
+            f.r[f.count++] = p;
+            m /= p;
+        }
+    return true;
+}
+
+// One pass of upsample_lines_kernel over `lines` lines.
+static int upsample_pass(mbs_ctx *ctx, const UpLines &g, int lines, const double *src,
+                         double *dst)
+{
+    UpFactors fin, fout;
+    if (!up_factors(g.n_in, fin) || !up_factors(g.n_out, fout))
+        return fail(ctx, MBS_ERR_CAPACITY, "upsampling: too many factors (%d -> %d)", g.n_in,
+                    g.n_out);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, ctx->up_roots_in.ensure(g.n_in));
+    HIP_TRY(ctx, ctx->up_roots_out.ensure(g.n_out));
+    up_roots_kernel<<<(g.n_in + 255) / 256, 256, 0, s>>>(g.n_in, ctx->up_roots_in.p);
+    up_roots_kernel<<<(g.n_out + 255) / 256, 256, 0, s>>>(g.n_out, ctx->up_roots_out.p);
+    HIP_TRY(ctx, hipGetLastError());
+    const size_t lds = (size_t)2 * g.nbuf * sizeof(double2);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)upsample_lines_kernel,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    upsample_lines_kernel<<<lines, kUpThreads, lds, s>>>(g, fin, fout, ctx->up_roots_in.p,
+                                                         ctx->up_roots_out.p, src, dst);
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
+extern "C" {
+
+int mbs_upsample_result(mbs_ctx *ctx, int noshadow, int32_t n_az_out, int32_t n_zen_out,
+                        const double *theta_out, int accumulate, double *grid_out,
+                        double *rows_out)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->have_result)
+        return fail(ctx, MBS_ERR_INVALID, "no result: call mbs_diffract first");
+    if (noshadow && !ctx->have_ns)
+        return fail(ctx, MBS_ERR_INVALID, "no-shadow result was not requested (MBS_NOSHADOW)");
+    const int n_az = ctx->n_az, n_zen = ctx->n_zen1 - 1;
+    if (n_az_out < n_az || n_zen_out < n_zen)
+        return fail(ctx, MBS_ERR_INVALID, "upsampling %d x %d to %d x %d: sizes may only grow",
+                    n_az, n_zen + 1, n_az_out, n_zen_out + 1);
+    if (n_zen_out > n_zen && n_zen < 1)
+        return fail(ctx, MBS_ERR_INVALID, "zenith upsampling needs at least two rows");
+    if (rows_out && !theta_out)
+        return fail(ctx, MBS_ERR_INVALID, "rows_out needs theta_out");
+    const bool up_az = n_az_out > n_az, up_zen = n_zen_out > n_zen;
+    if ((up_az && n_az_out > kUpMaxN) || (up_zen && 2 * n_zen_out > kUpMaxN))
+        return fail(ctx, MBS_ERR_CAPACITY,
+                    "upsampling lines are limited to %d samples (%d azimuths, 2 x %d rows "
+                    "asked)", kUpMaxN, n_az_out, n_zen_out);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const double *cur = noshadow ? ctx->res_ns.p : ctx->res.p;
+    const size_t mid = (size_t)n_az * (n_zen_out + 1) * 16;
+    const size_t out = (size_t)n_az_out * (n_zen_out + 1) * 16;
+    if (up_zen) {
+        HIP_TRY(ctx, ctx->up_a.ensure(mid));
+        UpLines g{1, 2 * n_zen, 2 * n_zen_out, 2 * n_zen_out, n_zen + 1, n_zen, n_zen_out};
+        int rc = upsample_pass(ctx, g, n_az * 8, cur, ctx->up_a.p);
+        if (rc)
+            return rc;
+        cur = ctx->up_a.p;
+    }
+    if (up_az) {
+        HIP_TRY(ctx, ctx->up_b.ensure(out));
+        UpLines g{0, n_az, n_az_out, n_az_out, n_zen_out + 1, 0, 0};
+        int rc = upsample_pass(ctx, g, (n_zen_out + 1) * 8, cur, ctx->up_b.p);
+        if (rc)
+            return rc;
+        cur = ctx->up_b.p;
+    }
+    if (rows_out) {
+        const size_t n = (size_t)(n_zen_out + 1) * 18;
+        HIP_TRY(ctx, ctx->up_theta.ensure(n_zen_out + 1));
+        HIP_TRY(ctx, ctx->avg_rows.ensure(n));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->up_theta.p, theta_out, sizeof(double) * (n_zen_out + 1),
+                                    hipMemcpyHostToDevice, ctx->stream));
+        azimuth_average_kernel<<<n_zen_out + 1, 64, 0, ctx->stream>>>(
+            cur, ctx->up_theta.p, n_az_out, n_zen_out, ctx->az_step * n_az / n_az_out,
+            ctx->avg_rows.p);
+        HIP_TRY(ctx, hipGetLastError());
+        int rc = fetch(ctx, ctx->avg_rows.p, n, rows_out, false);
+        if (rc)
+            return rc;
+    }
+    if (grid_out)
+        return fetch(ctx, cur, out, grid_out, accumulate != 0);
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBS_OK;
 }
 
 int mbs_orientation_set(mbs_ctx *ctx, int kind, int32_t n, uint32_t seed, double beta_sym,

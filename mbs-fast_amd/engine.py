@@ -103,6 +103,8 @@ def lib():
         L.mbs_mlp_train.restype = C.c_int
         L.mbs_azimuth_average_per_orient.argtypes = [vp, dp]
         L.mbs_azimuth_average_per_orient.restype = C.c_int
+        L.mbs_upsample_result.argtypes = [vp, C.c_int, C.c_int32, C.c_int32, dp, C.c_int, dp, dp]
+        L.mbs_upsample_result.restype = C.c_int
         L.mbs_azimuth_average.argtypes = [vp, C.c_int, dp]
         L.mbs_azimuth_average.restype = C.c_int
         L.mbs_device_result.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
@@ -225,6 +227,18 @@ class Context:
         mn = np.zeros_like(m) if noshadow else None
         self.accumulate(m, mn)
         return m, mn
+
+    def upsample(self, n_az_out, n_zen_out, noshadow=False, theta_out=None, rows=False):
+        """mbs_upsample_result: the last result Fourier-upsampled on the
+        device to n_az_out x (n_zen_out + 1) cells.  Returns (grid
+        [n_az_out, n_zen_out+1, 16], rows [n_zen_out+1, 18] or None)."""
+        grid = np.zeros((int(n_az_out), int(n_zen_out) + 1, 16))
+        r = np.zeros((int(n_zen_out) + 1, 18)) if rows else None
+        th = np.ascontiguousarray(theta_out, np.float64) if theta_out is not None else None
+        self._check(self._L.mbs_upsample_result(
+            self._h, 1 if noshadow else 0, int(n_az_out), int(n_zen_out),
+            _d(th) if th is not None else None, 0, _d(grid), _d(r) if r is not None else None))
+        return grid, r
 
     def azimuth_average(self, noshadow=False):
         """Rows [n_zen+1, 18] (theta deg, 2 pi dcos, M11..M44) of the last

@@ -30,33 +30,6 @@ from . import abi
 from .engine import Context, EngineError
 
 
-def upsample_phi(low, n_full):
-    """The phi upsampling of the reference's FFT path (fft_upsample_phi_arr2d,
-    GpuDiffraction.cu:3195-3330, fft_phi_pad_kernel :794-814) along axis 0:
-    forward DFT of the n_low samples, frequency k placed at k for
-    k <= n_low/2 (the Nyquist term of an even n_low on the positive side) and
-    at n_full + k - n_low otherwise, scaled by 1/n_low, unnormalised inverse
-    DFT over n_full, real part."""
-    n_low = low.shape[0]
-    k = np.arange(n_low)
-    f = np.where(k <= n_low // 2, k, k - n_low)
-    spec = np.zeros((n_full,) + low.shape[1:], dtype=complex)
-    spec[np.where(f >= 0, f, n_full + f)] = np.fft.fft(low, axis=0) / n_low
-    return np.real(np.fft.ifft(spec, axis=0) * n_full)
-
-
-def upsample_theta(low, n_theta_full):
-    """The theta upsampling of the same path (fft_upsample_theta_arr2d,
-    GpuDiffraction.cu:3343-3420) along axis 1 of [n_az, n_theta_low+1, ...]:
-    each column extended evenly to 2 n_theta_low samples (row t and row
-    2 n_theta_low - t), upsampled by upsample_phi's rule to 2 n_theta_full,
-    rows 0..n_theta_full kept."""
-    n_low = 2 * (low.shape[1] - 1)
-    ext = np.concatenate([low, low[:, -2:0:-1]], axis=1)
-    up = upsample_phi(np.moveaxis(ext, 1, 0), 2 * n_theta_full)
-    return np.moveaxis(up, 0, 1)[:, :n_theta_full + 1]
-
-
 class HandlerPOGpu:
     def __init__(self, grid, consts, device=0, precision="fp64"):
         self.grid = grid
@@ -187,12 +160,13 @@ class HandlerPOGpu:
                                            fftPhiFactor=0):
         """HandlerPO::HandleOrientationsToLocalGpuFftPhi (GpuDiffraction.cu:3504-3839):
         the batch on nLow = nFull / factor azimuths (at least 16), then the
-        Mueller grids upsampled in phi (upsample_phi) and added into the
-        handler's grids.  factor: ``fftPhiFactor``, else MBS_FFT_PHI_FACTOR,
+        Mueller grids Fourier-upsampled on the device (mbs_upsample_result)
+        and added into the handler's grids.  factor: ``fftPhiFactor``, else MBS_FFT_PHI_FACTOR,
         else by nFull (choose_fft_phi_factor, :712-727); a factor of 1,
         nFull < 32 or nLow >= nFull runs the direct path.  With
-        MBS_FFT_THETA_FACTOR the coarse grid also has nZen/factor rows
-        (upsample_theta).  The validation and adaptive options return False."""
+        MBS_FFT_THETA_FACTOR the coarse grid also has nZen/factor rows,
+        upsampled in zenith first.  The validation and adaptive options return
+        False."""
         if not self.isCoh:
             return False
         g = self.grid
@@ -223,25 +197,30 @@ class HandlerPOGpu:
         theta_low = np.linspace(g.theta[0], g.theta[-1], n_zen_low + 1) if tf > 1 else g.theta
         low = abi.Grid.build(theta_low, n_low)
         ns = self.ComputeNoShadow()
-        lm = np.zeros((n_low, n_zen_low + 1, 16))
-        lmn = np.zeros_like(lm) if ns else None
+        count = prepared.n_orient - start if count is None else count
+        if start < 0 or count < 0 or start > prepared.n_orient:
+            return False
+        orients = list(range(start, min(prepared.n_orient, start + count)))
+        if not orients:
+            return True
+        if scale != 1.0 or waveIndex != 0.0:
+            return False  # size-rescaled batches are the multi-size path
+        batch = prepared.subset(orients)
         ctx = self._context()
-        self.grid = low
         try:
             ctx.set_grid(low)
-            ok = self.HandleOrientationsToLocalGpu(prepared, lm, lmn, start, count, scale,
-                                                   waveIndex)
+            ctx.upload(batch, self.consts)
+            ctx.diffract(self._flags())
+            # the device upsampler (mbs_upsample_result): zenith, then azimuth
+            up, _ = ctx.upsample(n_full, g.n_zen)
+            up_ns = ctx.upsample(n_full, g.n_zen, noshadow=True)[0] if ns else None
+        except EngineError as e:
+            return self._device_failure(e)
         finally:
-            self.grid = g
             ctx.set_grid(g)
-        if not ok:
-            return False
-        if tf > 1:
-            lm = upsample_theta(lm, g.n_zen)
-            lmn = upsample_theta(lmn, g.n_zen) if ns else None
-        self._cells_view(localM, g.n_az, g.n_zen + 1)[:] += upsample_phi(lm, n_full)
+        self._cells_view(localM, g.n_az, g.n_zen + 1)[:] += up
         if ns:
-            self._cells_view(localM_noshadow, g.n_az, g.n_zen + 1)[:] += upsample_phi(lmn, n_full)
+            self._cells_view(localM_noshadow, g.n_az, g.n_zen + 1)[:] += up_ns
         return True
 
     def DiffractThetasGpu(self, prepared, theta_rads, nPoints=None):

@@ -132,6 +132,21 @@ struct ThreadContexts
         // host staging of the packed batch, kept so that later calls reuse
         // its pages instead of allocating and faulting in ~2 GB per batch
         PackedBatch pk;
+        // the resident batch's pack-cache key (token 0: not reusable), the
+        // reference's packed-beam cache (GpuDiffraction.cu:4274-4282,
+        // :4588-4601): the same token, orientation range, scale and wave
+        // index mean the same packed beams
+        struct CacheKey
+        {
+            unsigned long long token = 0;
+            int start = 0, count = 0;
+            double scale = 0.0, waveIndex = 0.0;
+            bool operator==(const CacheKey &o) const
+            {
+                return token == o.token && start == o.start && count == o.count
+                       && scale == o.scale && waveIndex == o.waveIndex;
+            }
+        } resident;
     };
     std::map<int, Entry> byDevice;
     ~ThreadContexts()
@@ -449,58 +464,88 @@ ThreadContexts::Entry *Context(int device, GridSpec &g, bool &ok)
     return &e;
 }
 
-// Diffracts one packed batch and adds the result into localM (and into
-// *localMns when it is given).
-bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
-              Arr2D &localM, Arr2D *localMns, double packMs, bool upload = true)
+// An engine status under the reference's conventions: a device error exits
+// (or, with MBS_GPU_ALLOW_FALLBACK=1, returns false); any other failure is
+// reported and returns false.
+bool Status(int rc, const char *what, mbs_ctx *ctx)
 {
-    const int nAz = e.nAz, nZen = e.nZen;
+    if (rc == MBS_OK)
+        return true;
+    if (rc == MBS_ERR_DEVICE)
+        return DeviceFailure(what, mbs_last_error(ctx));
+    std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(ctx));
+    return false;
+}
+
+// Uploads (unless `upload` is false: the batch is resident) and diffracts
+// one packed batch on the context's grid; the result stays on the device.
+// MBS_GPU_TIMING=1 prints the stage times.
+bool DiffractBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
+                   bool ns, double packMs, bool upload = true)
+{
     const bool timing = EnvIs("MBS_GPU_TIMING", "1");
     const auto t0 = std::chrono::steady_clock::now();
     mbs_set_timing(e.ctx, timing ? 1 : 0);
     const mbs_beam_batch view = pk.View();
     int rc = upload ? mbs_upload_beams(e.ctx, &view, &c) : MBS_OK;
     if (rc == MBS_OK)
-        rc = mbs_diffract(e.ctx, PrecisionFlag() | (localMns ? MBS_NOSHADOW : 0u));
-    std::vector<double> mFull((size_t)nAz * (nZen + 1) * 16, 0.0);
-    std::vector<double> mNs(localMns ? mFull.size() : 0, 0.0);
-    if (rc == MBS_OK)
-        rc = mbs_accumulate_mueller(e.ctx, mFull.data(), localMns ? mNs.data() : nullptr);
-    if (rc == MBS_ERR_DEVICE)
-        return DeviceFailure("diffraction", mbs_last_error(e.ctx));
-    if (rc != MBS_OK)
-    {
-        std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(e.ctx));
+        rc = mbs_diffract(e.ctx, PrecisionFlag() | (ns ? MBS_NOSHADOW : 0u));
+    if (!Status(rc, "diffraction", e.ctx))
         return false;
-    }
-    const double callMs =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (timing)
     {
         mbs_timing tm;
         if (mbs_get_timing(e.ctx, &tm) == MBS_OK)
+        {
+            const double callMs = std::chrono::duration<double, std::milli>(
+                                      std::chrono::steady_clock::now() - t0)
+                                      .count();
             std::fprintf(stderr,
                          "GPU timing: %d orientations, %zu beams: pack %.3f ms, upload %.3f ms, "
                          "kernel %.3f ms, reduce %.3f ms, engine calls %.3f ms (%d work items, "
                          "%d workgroups)\n",
                          (int)pk.weights.size(), pk.nv.size(), packMs, tm.upload_ms,
                          tm.kernel_ms, tm.reduce_ms, callMs, tm.n_items, tm.n_blocks);
+        }
     }
+    return true;
+}
+
+// Adds a flat grid [nAz][nZen+1][16] into the Arr2D cells p < nAz, t <= nZen.
+void AddGrid(const std::vector<double> &g, int nAz, int nZen, Arr2D &dst)
+{
     for (int p = 0; p < nAz; ++p)
         for (int t = 0; t <= nZen; ++t)
         {
-            const size_t cell = ((size_t)p * (nZen + 1) + t) * 16;
-            double *dst = localM.RawCell(p, t);
+            const double *src = &g[((size_t)p * (nZen + 1) + t) * 16];
+            double *cell = dst.RawCell(p, t);
             for (int k = 0; k < 16; ++k)
-                dst[k] += mFull[cell + k];
-            if (localMns)
-            {
-                double *dn = localMns->RawCell(p, t);
-                for (int k = 0; k < 16; ++k)
-                    dn[k] += mNs[cell + k];
-            }
+                cell[k] += src[k];
         }
+}
+
+// Adds the device result of the context's last diffraction into localM (and
+// *localMns when it is given).
+bool AddResult(ThreadContexts::Entry &e, Arr2D &localM, Arr2D *localMns)
+{
+    const size_t n = (size_t)e.nAz * (e.nZen + 1) * 16;
+    std::vector<double> mFull(n, 0.0), mNs(localMns ? n : 0, 0.0);
+    if (!Status(mbs_accumulate_mueller(e.ctx, mFull.data(), localMns ? mNs.data() : nullptr),
+                "result copy", e.ctx))
+        return false;
+    AddGrid(mFull, e.nAz, e.nZen, localM);
+    if (localMns)
+        AddGrid(mNs, e.nAz, e.nZen, *localMns);
     return true;
+}
+
+// Diffracts one packed batch and adds the result into localM (and into
+// *localMns when it is given).
+bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts &c,
+              Arr2D &localM, Arr2D *localMns, double packMs, bool upload = true)
+{
+    return DiffractBatch(e, pk, c, localMns != nullptr, packMs, upload)
+           && AddResult(e, localM, localMns);
 }
 }  // namespace
 
@@ -526,7 +571,7 @@ static mbs_consts HandlerConsts(double waveIndex, double eps1, const ::complex &
 bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
     const std::vector<PreparedOrientation> &prepared, int start, int count, int device,
     Arr2D &localM, Arr2D &localM_noshadow, double scale, double waveIndex,
-    unsigned long long /*cacheToken*/)
+    unsigned long long cacheToken)
 {
     if (!isCoh)
         return false;
@@ -542,6 +587,19 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
     if (!e)
         return ok;
     PackedBatch &pk = e->pk;
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    Arr2D *ns = ComputeNoShadow() ? &localM_noshadow : nullptr;
+    ThreadContexts::Entry::CacheKey key;
+    key.token = cacheToken;
+    key.start = start;
+    key.count = nOrient;
+    key.scale = scale;
+    key.waveIndex = waveIndex;
+    // the batch this token names is resident: diffract it on this grid
+    if (cacheToken != 0 && e->resident == key)
+        return pk.nv.empty() || RunBatch(*e, pk, c, localM, ns, 0.0, false);
+    e->resident = ThreadContexts::Entry::CacheKey();
     const auto tp = std::chrono::steady_clock::now();
     if (!Pack(prepared, start, nOrient, PackScale(scale, waveIndex, AbsorptionCoefficient()),
               pk))
@@ -549,10 +607,16 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
     const double packMs =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
     if (pk.nv.empty())
+    {
+        if (cacheToken != 0)
+            e->resident = key;
         return true;
-    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
-                                       m_legacySign);
-    return RunBatch(*e, pk, c, localM, ComputeNoShadow() ? &localM_noshadow : nullptr, packMs);
+    }
+    if (!RunBatch(*e, pk, c, localM, ns, packMs))
+        return false;
+    if (cacheToken != 0)
+        e->resident = key;
+    return true;
 }
 
 bool HandlerPO::HandleOrientationsToLocalGpu(const std::vector<PreparedOrientation> &prepared,
@@ -590,103 +654,55 @@ int EnvInt(const char *name, int lo, int hi, int fallback)
     return (end && *end == '\0' && x >= lo && x <= hi) ? (int)x : fallback;
 }
 
-// choose_fft_phi_factor (src/cuda/GpuDiffraction.cu:712-727, with
-// gpu_fft_phi_factor :513-531): the handler's factor, else MBS_FFT_PHI_FACTOR
-// (1..64, "auto" = unset), else by the full azimuth count.
-int ChoosePhiFactor(int nFull, int configured)
+double EnvDouble(const char *name, double fallback)
 {
+    const char *v = std::getenv(name);
+    if (!v || !*v)
+        return fallback;
+    char *end = nullptr;
+    const double x = std::strtod(v, &end);
+    return (end && *end == '\0' && x > 0.0) ? x : fallback;
+}
+
+// Longest line the device upsampler takes (mbs_upsample_result).
+const int kMaxUpsampleLine = 4096;
+
+// The azimuth factor of the angular FFT path: the handler's own, else
+// MBS_FFT_PHI_FACTOR (1..64; unset or "auto" chooses), else by the full
+// azimuth count with the reference's defaults (gpu_fft_phi_factor and
+// choose_fft_phi_factor, src/cuda/GpuDiffraction.cu:513-531, :712-727).
+int PhiFactor(int nFull, int configured)
+{
+    static const struct
+    {
+        int minAzimuths, factor;
+    } kDefaults[] = {{2400, 8}, {600, 6}, {240, 3}};
     if (configured > 0)
         return configured;
-    const int requested = EnvInt("MBS_FFT_PHI_FACTOR", 1, 64, 0);
-    if (requested > 0)
-        return requested;
-    return nFull >= 2400 ? 8 : nFull >= 600 ? 6 : nFull >= 240 ? 3 : 2;
+    const int fromEnv = EnvInt("MBS_FFT_PHI_FACTOR", 1, 64, 0);
+    if (fromEnv > 0)
+        return fromEnv;
+    for (const auto &d : kDefaults)
+        if (nFull >= d.minAzimuths)
+            return d.factor;
+    return 2;
 }
 
-// Adds to dst(q, t), q < nFull, t <= nZen, the band-limited interpolation in
-// phi of low(p, t), p < nLow: the upsampling of the reference's phi-FFT path
-// (fft_upsample_phi_arr2d :3195-3330, fft_phi_pad_kernel :794-814): forward
-// DFT of the nLow samples; frequency k at k for k <= nLow/2 (the Nyquist term
-// of an even nLow on the positive side), else at k - nLow; scaled by 1/nLow;
-// inverse DFT over nFull; real part.  As a real matrix,
-// y[q] = sum_p K(q/nFull - p/nLow) x[p], K(u) = (1/nLow) sum_k cos(2 pi f_k u),
-// with the sum in closed form (Dirichlet kernel): sin((M + 1/2) a)/sin(a/2) for
-// nLow = 2M + 1, sin((M - 1/2) a)/sin(a/2) + cos(M a) for nLow = 2M,
-// a = 2 pi u, and nLow at u = 0.
-std::vector<double> UpsamplingMatrix(int nLow, int nFull)
+// The zenith factor (MBS_FFT_THETA_FACTOR, gpu_fft_theta_factor :532-542 and
+// :3543-3550): 1 unless the zenith grid is uniform, has at least 32 rows and
+// the factor divides them -- and, here, unless the device upsampler takes
+// the extended zenith lines.
+int ThetaFactor(const ScatteringRange &sphere)
 {
-    std::vector<double> K((size_t)nFull * nLow);
-    const int M = nLow / 2;
-    for (int q = 0; q < nFull; ++q)
-        for (int p = 0; p < nLow; ++p)
-        {
-            double k = 1.0;
-            if ((int64_t)q * nLow != (int64_t)p * nFull)
-            {
-                const double a = 2.0 * M_PI * ((double)q / nFull - (double)p / nLow);
-                const double h = std::sin(0.5 * a);
-                k = (nLow & 1) ? std::sin((M + 0.5) * a) / h / nLow
-                               : (std::sin((M - 0.5) * a) / h + std::cos(M * a)) / nLow;
-            }
-            K[(size_t)q * nLow + p] = k;
-        }
-    return K;
+    const int f = EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1);
+    if (f <= 1 || sphere.isNonUniform || sphere.nZenith < 32 || sphere.nZenith % f != 0
+        || 2 * sphere.nZenith > kMaxUpsampleLine)
+        return 1;
+    return f;
 }
 
-void UpsamplePhi(const Arr2D &low, int nLow, int nFull, int nZen, Arr2D &dst)
-{
-    const std::vector<double> K = UpsamplingMatrix(nLow, nFull);
-#pragma omp parallel for schedule(static)
-    for (int t = 0; t <= nZen; ++t)
-    {
-        std::vector<double> x((size_t)nLow * 16);
-        for (int p = 0; p < nLow; ++p)
-            std::memcpy(&x[(size_t)p * 16], low.RawCell(p, t), 16 * sizeof(double));
-        for (int q = 0; q < nFull; ++q)
-        {
-            double acc[16] = {0};
-            const double *kq = &K[(size_t)q * nLow];
-            for (int p = 0; p < nLow; ++p)
-                for (int e = 0; e < 16; ++e)
-                    acc[e] += kq[p] * x[(size_t)p * 16 + e];
-            double *cell = dst.RawCell(q, t);
-            for (int e = 0; e < 16; ++e)
-                cell[e] += acc[e];
-        }
-    }
-}
-// The theta upsampling of the same path (fft_upsample_theta_arr2d,
-// :3343-3420) for a uniform zenith grid: the nThetaLow + 1 rows of each
-// azimuth column extended evenly to a period of 2 nThetaLow (row t and row
-// 2 nThetaLow - t), upsampled by the rule above to 2 nThetaFull, rows
-// 0..nThetaFull added into dst (p < nAz).
-void UpsampleTheta(const Arr2D &low, int nAz, int nThetaLow, int nThetaFull, Arr2D &dst)
-{
-    const int nLow = 2 * nThetaLow, nFull = 2 * nThetaFull;
-    const std::vector<double> K = UpsamplingMatrix(nLow, nFull);
-#pragma omp parallel for schedule(static)
-    for (int p = 0; p < nAz; ++p)
-    {
-        std::vector<double> x((size_t)nLow * 16);
-        for (int t = 0; t < nLow; ++t)
-            std::memcpy(&x[(size_t)t * 16], low.RawCell(p, t <= nThetaLow ? t : nLow - t),
-                        16 * sizeof(double));
-        for (int q = 0; q <= nThetaFull; ++q)
-        {
-            double acc[16] = {0};
-            const double *kq = &K[(size_t)q * nLow];
-            for (int t = 0; t < nLow; ++t)
-                for (int e = 0; e < 16; ++e)
-                    acc[e] += kq[t] * x[(size_t)t * 16 + e];
-            double *cell = dst.RawCell(p, q);
-            for (int e = 0; e < 16; ++e)
-                cell[e] += acc[e];
-        }
-    }
-}
-
-// The coarse-grid sphere of the phi-FFT path (:3563-3570): nLow azimuths and,
-// with a theta factor, nZen / factor zenith rows.
+// The coarse-grid sphere of the path (:3563-3570): nLow azimuths and nZenLow
+// zenith rows over the same range.
 ScatteringRange CoarseSphere(const ScatteringRange &full, int nLow, int nZenLow,
                              const Light &incident)
 {
@@ -699,160 +715,194 @@ ScatteringRange CoarseSphere(const ScatteringRange &full, int nLow, int nZenLow,
     return low;
 }
 
-// The theta factor (gpu_fft_theta_factor, :532-542, and :3543-3550): 1 for
-// non-uniform or short grids or when it does not divide nZen.
-int ThetaFactor(const ScatteringRange &sphere)
+// Packs prepared[start, start + nOrient) and diffracts it on `sphere` with
+// this thread's context on device 0; the result stays on the device.  e is
+// left null when there is nothing to diffract (no beams).
+bool DiffractResident(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
+                      const ScatteringRange &sphere, const PackScale &sc, const mbs_consts &c,
+                      bool ns, ThreadContexts::Entry *&e)
 {
-    const int f = EnvInt("MBS_FFT_THETA_FACTOR", 1, 16, 1);
-    if (f <= 1 || sphere.isNonUniform || sphere.nZenith < 32 || sphere.nZenith % f != 0)
-        return 1;
-    return f;
+    e = nullptr;
+    GridSpec g = MakeGrid(sphere);
+    bool ok = false;
+    ThreadContexts::Entry *ctx = Context(0, g, ok);
+    if (!ctx)
+        return ok;
+    ctx->resident = ThreadContexts::Entry::CacheKey();
+    const auto tp = std::chrono::steady_clock::now();
+    if (!Pack(prepared, start, nOrient, sc, ctx->pk))
+        return false;
+    const double packMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
+    if (ctx->pk.nv.empty())
+        return true;
+    if (!DiffractBatch(*ctx, ctx->pk, c, ns, packMs))
+        return false;
+    e = ctx;
+    return true;
 }
 
-// report_fft_check (src/cuda/GpuDiffraction.cu:3142-3195): the upsampled
-// grid against a reference grid -- M11 errors relative to the local M11 (and
-// to the cells with M11 >= 1e-3 of its maximum), and all 16 elements relative
-// to max(|M11|, that floor) -- printed; the estimate is the larger of the
-// significant-M11 and the element figures.
-double ReportFftCheck(const Arr2D &fftM, const Arr2D &refM, int nAz, int nZen, const char *name)
+// The batch resident on e diffracted again, on another grid.
+bool Rediffract(ThreadContexts::Entry &e, const ScatteringRange &sphere, const mbs_consts &c,
+                bool ns)
 {
-    double refMax = 0.0;
-    for (int p = 0; p < nAz; ++p)
-        for (int t = 0; t <= nZen; ++t)
-            refMax = std::max(refMax, std::fabs(refM.RawCell(p, t)[0]));
-    const double significant = 1e-3 * refMax;
-    double maxAbs = 0.0, maxRel = 0.0, sigRel = 0.0, normRel = 0.0, rms = 0.0;
-    int samples = 0, sigSamples = 0;
-    for (int p = 0; p < nAz; ++p)
-        for (int t = 0; t <= nZen; ++t)
-        {
-            const double *r = refM.RawCell(p, t), *f = fftM.RawCell(p, t);
-            const double absErr = std::fabs(f[0] - r[0]);
-            const double relErr = absErr / std::max(std::fabs(r[0]), 1e-12);
-            maxAbs = std::max(maxAbs, absErr);
-            maxRel = std::max(maxRel, relErr);
-            if (std::fabs(r[0]) >= significant)
-            {
-                sigRel = std::max(sigRel, relErr);
-                ++sigSamples;
-            }
-            const double scale = std::max(std::fabs(r[0]), significant);
-            for (int e = 0; e < 16; ++e)
-                normRel = std::max(normRel, std::fabs(f[e] - r[e]) / scale);
-            rms += relErr * relErr;
-            ++samples;
-        }
-    rms = std::sqrt(rms / std::max(1, samples));
-    std::fprintf(stderr,
-                 "GPU FFT check %s: M11 max_abs=%.6g max_rel=%.6g significant_max_rel=%.6g "
-                 "rms_rel=%.6g, Mueller/M11 max_rel=%.6g, samples=%d significant_samples=%d\n",
-                 name, maxAbs, maxRel, sigRel, rms, normRel, samples, sigSamples);
-    return std::max(sigRel, normRel);
+    GridSpec g = MakeGrid(sphere);
+    bool ok = false;
+    if (!Context(0, g, ok))
+        return ok;
+    return DiffractBatch(e, e.pk, c, ns, 0.0, false);
 }
 
-double EnvDouble(const char *name, double fallback)
+// The last result on e, Fourier-upsampled on the device to the grid of
+// `full` (its azimuths and zenith rows): grid [nAz][nZen+1][16] and, when
+// rows is given, the grid's azimuth-averaged rows [nZen+1][18].
+bool Upsampled(ThreadContexts::Entry &e, bool ns, const ScatteringRange &full,
+               std::vector<double> &grid, std::vector<double> *rows)
 {
-    const char *v = std::getenv(name);
-    if (!v || !*v)
-        return fallback;
-    char *end = nullptr;
-    const double x = std::strtod(v, &end);
-    return (end && *end == '\0' && x > 0.0) ? x : fallback;
+    const int nAz = full.nAzimuth, nZen = full.nZenith;
+    std::vector<double> theta(nZen + 1);
+    for (int t = 0; t <= nZen; ++t)
+        theta[t] = full.GetZenith(t);
+    grid.assign((size_t)nAz * (nZen + 1) * 16, 0.0);
+    if (rows)
+        rows->assign((size_t)(nZen + 1) * 18, 0.0);
+    return Status(mbs_upsample_result(e.ctx, ns ? 1 : 0, nAz, nZen, theta.data(), 0, grid.data(),
+                                      rows ? rows->data() : nullptr),
+                  "upsampling", e.ctx);
 }
 
-// azimuth_average_output_cell (:3052-3084): one theta row's azimuth mean,
-// the linear-polarisation columns rotated by L(-phi) off the poles.
-void RowAverage(const Arr2D &src, int nAz, int nZen, int t, double out[16])
+// How far an interpolated grid is from a reference grid, in the figures of
+// the reference's FFT check (report_fft_check, :3142-3195), whose stderr
+// line is kept as it is: M11 errors relative to the cell's own M11, over
+// all cells and over the "significant" ones (M11 at least 1e-3 of the
+// grid's largest), their RMS, and every element's error relative to
+// max(|M11|, that floor).  The estimate is the larger of the significant-M11
+// and the element figures.
+struct InterpolationError
 {
-    for (int k = 0; k < 16; ++k)
-        out[k] = 0.0;
-    const bool pole = t == 0 || t == nZen;
-    for (int p = 0; p < nAz; ++p)
+    double m11MaxAbs = 0.0, m11MaxRel = 0.0, m11SignificantRel = 0.0, m11Rms = 0.0;
+    double elementRel = 0.0;
+    int cells = 0, significantCells = 0;
+
+    double Estimate() const { return std::max(m11SignificantRel, elementRel); }
+    void Print(const char *name) const
     {
-        const double *m = src.RawCell(p, t);
-        if (pole)
-        {
-            for (int k = 0; k < 16; ++k)
-                out[k] += m[k];
-            continue;
-        }
-        const double a = -p * (2.0 * M_PI / (double)nAz);
-        const double cs = std::cos(2.0 * a), sn = std::sin(2.0 * a);
-        for (int r = 0; r < 4; ++r)
-        {
-            out[4 * r] += m[4 * r];
-            out[4 * r + 1] += m[4 * r + 1] * cs - m[4 * r + 2] * sn;
-            out[4 * r + 2] += m[4 * r + 1] * sn + m[4 * r + 2] * cs;
-            out[4 * r + 3] += m[4 * r + 3];
-        }
+        std::fprintf(stderr,
+                     "GPU FFT check %s: M11 max_abs=%.6g max_rel=%.6g significant_max_rel=%.6g "
+                     "rms_rel=%.6g, Mueller/M11 max_rel=%.6g, samples=%d significant_samples=%d\n",
+                     name, m11MaxAbs, m11MaxRel, m11SignificantRel, m11Rms, elementRel, cells,
+                     significantCells);
     }
-    for (int k = 0; k < 16; ++k)
-        out[k] *= 1.0 / (double)nAz;
+};
+
+InterpolationError Compare(const std::vector<double> &got, const std::vector<double> &ref)
+{
+    InterpolationError r;
+    const size_t n = ref.size() / 16;
+    double largest = 0.0;
+    for (size_t i = 0; i < n; ++i)
+        largest = std::max(largest, std::fabs(ref[16 * i]));
+    const double floor = 1e-3 * largest;
+    double sumSq = 0.0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        const double *a = &got[16 * i], *b = &ref[16 * i];
+        const double m11 = std::fabs(b[0]);
+        const double d = std::fabs(a[0] - b[0]);
+        const double rel = d / std::max(m11, 1e-12);
+        r.m11MaxAbs = std::max(r.m11MaxAbs, d);
+        r.m11MaxRel = std::max(r.m11MaxRel, rel);
+        sumSq += rel * rel;
+        if (m11 >= floor)
+        {
+            r.m11SignificantRel = std::max(r.m11SignificantRel, rel);
+            ++r.significantCells;
+        }
+        const double den = std::max(m11, floor);
+        if (den > 0.0)
+            for (int k = 0; k < 16; ++k)
+                r.elementRel = std::max(r.elementRel, std::fabs(a[k] - b[k]) / den);
+    }
+    r.cells = (int)n;
+    r.m11Rms = std::sqrt(sumSq / std::max<size_t>(1, n));
+    return r;
 }
 
-// detect_fft_phi_spike_rows (:3086-3135): interior theta rows whose averaged
-// off-diagonal ratios M_ij/M11 (ij = 23, 24, 32, 34, 42, 43, 13, 14) leave
-// the mean of the two neighbouring rows by at least `threshold`; the
-// `maxRows` highest scores, in row order.
-std::vector<int> SpikeRows(const Arr2D &m, int nAz, int nZen, double threshold, int maxRows)
+// The rows the adaptive option recomputes directly (the reference's
+// MBS_FFT_ADAPTIVE_PHI rule, detect_fft_phi_spike_rows :3086-3135): interior
+// zenith rows whose azimuth-averaged ratios M_ij/M11 for ij = 23, 24, 32, 34,
+// 42, 43, 13, 14 leave the mean of the neighbouring rows' ratios by at least
+// `threshold`; the maxRows highest scores, in row order.  rows: the grid's
+// azimuth-averaged rows [nZen+1][18] (theta, 2 pi dcos, M11..M44), from the
+// device; every row's ratios are formed once.
+std::vector<int> RowsToRecompute(const std::vector<double> &rows, int nZen, double threshold,
+                                 int maxRows)
 {
-    std::vector<int> rows;
-    if (maxRows <= 0 || nAz <= 0 || nZen < 2)
-        return rows;
-    const int elems[8] = {6, 7, 9, 11, 13, 14, 2, 3};
+    static const int kElements[8] = {6, 7, 9, 11, 13, 14, 2, 3};
+    std::vector<int> picked;
+    if (maxRows <= 0 || nZen < 2)
+        return picked;
+    std::vector<double> ratio((size_t)(nZen + 1) * 8);
+    for (int t = 0; t <= nZen; ++t)
+    {
+        const double *m = &rows[(size_t)t * 18 + 2];
+        const double m11 = std::max(std::fabs(m[0]), 1e-30);
+        for (int i = 0; i < 8; ++i)
+            ratio[(size_t)t * 8 + i] = m[kElements[i]] / m11;
+    }
     std::vector<std::pair<double, int>> scored;
-    double prev[16], cur[16], next[16];
     for (int t = 1; t < nZen; ++t)
     {
-        RowAverage(m, nAz, nZen, t - 1, prev);
-        RowAverage(m, nAz, nZen, t, cur);
-        RowAverage(m, nAz, nZen, t + 1, next);
-        const double c11 = std::max(std::fabs(cur[0]), 1e-30);
-        const double p11 = std::max(std::fabs(prev[0]), 1e-30);
-        const double n11 = std::max(std::fabs(next[0]), 1e-30);
+        const double *lo = &ratio[(size_t)(t - 1) * 8], *mid = lo + 8, *hi = lo + 16;
         double score = 0.0;
-        for (int e : elems)
-            score = std::max(score, std::fabs(cur[e] / c11 - 0.5 * (prev[e] / p11 + next[e] / n11)));
+        for (int i = 0; i < 8; ++i)
+            score = std::max(score, std::fabs(mid[i] - 0.5 * (lo[i] + hi[i])));
         if (score >= threshold)
-            scored.push_back(std::make_pair(score, t));
+            scored.emplace_back(score, t);
     }
-    std::sort(scored.begin(), scored.end(),
-              [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
-                  return a.first > b.first;
-              });
-    for (const auto &s : scored)
-    {
-        rows.push_back(s.second);
-        if ((int)rows.size() >= maxRows)
-            break;
-    }
-    std::sort(rows.begin(), rows.end());
-    return rows;
+    const size_t keep = std::min(scored.size(), (size_t)maxRows);
+    std::partial_sort(scored.begin(), scored.begin() + keep, scored.end(),
+                      [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+                          return a.first > b.first;
+                      });
+    for (size_t i = 0; i < keep; ++i)
+        picked.push_back(scored[i].second);
+    std::sort(picked.begin(), picked.end());
+    return picked;
 }
 
-void AddCells(const Arr2D &src, int nAz, int nZen, Arr2D &dst)
+// A sphere of the handler's azimuths on the listed zenith rows only.
+ScatteringRange RowSphere(const ScatteringRange &full, const std::vector<int> &rows,
+                          const Light &incident)
 {
-    for (int p = 0; p < nAz; ++p)
-        for (int t = 0; t <= nZen; ++t)
-        {
-            const double *a = src.RawCell(p, t);
-            double *b = dst.RawCell(p, t);
-            for (int e = 0; e < 16; ++e)
-                b[e] += a[e];
-        }
+    ScatteringRange s = full;
+    s.isNonUniform = true;
+    s.thetaValues.clear();
+    for (int t : rows)
+        s.thetaValues.push_back(full.GetZenith(t));
+    s.nZenith = (int)rows.size() - 1;
+    s.zenithStart = s.thetaValues.front();
+    s.zenithEnd = s.thetaValues.back();
+    s.zenithStep = s.nZenith > 0 ? (s.zenithEnd - s.zenithStart) / s.nZenith : 0.0;
+    s.ComputeSphereDirections(incident);
+    return s;
 }
 }  // namespace
 
 // The reference's angular Fourier interpolation (src/cuda/GpuDiffraction.cu:
 // 3504-3839): the batch is diffracted on a grid of nLow = nFull / factor
-// azimuths (at least 16) with the handler's zenith rows, and the Mueller grids
-// are upsampled in phi to the handler's nFull azimuths and added into localM
-// (and localM_noshadow).  A factor of 1, nFull < 32 or nLow >= nFull runs the
-// direct path.  With MBS_FFT_CHECK=1 or a tolerance the reference's
-// validation and refinement run too, with MBS_FFT_ADAPTIVE_PHI its direct
-// recomputation of spiking theta rows, and with MBS_FFT_THETA_FACTOR (a
-// uniform zenith grid of at least 32 rows that the factor divides) the
-// coarse grid has nZen / factor rows, upsampled in theta first.
+// azimuths (at least 16) -- and, with MBS_FFT_THETA_FACTOR on a uniform
+// zenith grid, nZen / factor rows -- and the result is Fourier-upsampled on
+// the device to the handler's grid (mbs_upsample_result: zenith, then
+// azimuth) and added into localM (and localM_noshadow).  A factor of 1,
+// nFull < 32, nLow >= nFull or an azimuth count beyond the device
+// upsampler's lines runs the direct path.  MBS_FFT_CHECK=1 or a tolerance:
+// the same batch is also diffracted on min(nFull, 2 nLow) azimuths and
+// upsampled, the difference is reported, and above the tolerance that grid
+// replaces the interpolated one.  MBS_FFT_ADAPTIVE_PHI: zenith rows whose
+// polarisation ratios spike against their neighbours are diffracted
+// directly on all nFull azimuths and replace the interpolated rows.  The
+// batch is packed and uploaded once for all of these runs.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOrientation> &prepared,
                                                    int start, int count, Arr2D &localM,
                                                    Arr2D &localM_noshadow, double scale,
@@ -862,16 +912,19 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
         return false;
     const int nFull = m_sphere.nAzimuth;
     const int nZen = m_sphere.nZenith;
-    const int factor = ChoosePhiFactor(nFull, m_fftPhiFactor);
+    const int factor = PhiFactor(nFull, m_fftPhiFactor);
     int nLow = factor > 1 ? nFull / factor : nFull;
     if (nLow < 16)
         nLow = std::min(nFull, 16);
-    if (factor <= 1 || nFull < 32 || nLow >= nFull)
+    if (factor <= 1 || nFull < 32 || nLow >= nFull || nFull > kMaxUpsampleLine)
     {
         m_fftStatistics->RecordCall(nFull, nFull, 1);
         return HandleOrientationsToLocalGpu(prepared, start, count, localM, localM_noshadow,
                                             scale, waveIndex);
     }
+    if (start < 0 || count < 0 || start > (int)prepared.size())
+        return false;
+    const int nOrient = std::min(count, (int)prepared.size() - start);
     const char *check = std::getenv("MBS_FFT_CHECK");
     const char *adaptive = std::getenv("MBS_FFT_ADAPTIVE_PHI");
     const int thetaFactor = ThetaFactor(m_sphere);
@@ -879,89 +932,53 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
     const bool doCheck = (check && std::strcmp(check, "1") == 0) || m_fftTolerance > 0.0;
     const bool adaptiveRows = adaptive && *adaptive && std::strcmp(adaptive, "0") != 0;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
-    static bool printed = false;
-    if (!printed)
+    static bool announced = false;
+    if (!announced)
     {
         std::fprintf(stderr,
                      "GPU FFT angular interpolation: direct Nphi=%d, output Nphi=%d (factor=%d), "
                      "direct Ntheta=%d, output Ntheta=%d (factor=%d). "
                      "This is angular Fourier interpolation, not aperture pFFT/FMM.\n",
                      nLow, nFull, factor, nZenLow + 1, nZen + 1, thetaFactor);
-        printed = true;
+        announced = true;
     }
-    const bool ns = ComputeNoShadow();
-    const ScatteringRange lowSphere = CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight);
-    Arr2D coarseM(nLow + 1, nZenLow + 1, 4, 4);
-    coarseM.ClearArr();
-    Arr2D coarseMns(ns ? nLow + 1 : 0, ns ? nZenLow + 1 : 0, ns ? 4 : 0, ns ? 4 : 0);
-    if (ns)
-        coarseMns.ClearArr();
-    const ScatteringRange saved = m_sphere;
-    m_sphere = lowSphere;
-    const bool ok = HandleOrientationsToLocalGpu(prepared, start, count, coarseM, coarseMns,
-                                                 scale, waveIndex);
-    m_sphere = saved;
-    if (!ok)
-        return false;
-    // theta first (to the handler's rows), then phi
-    Arr2D lowM(nLow + 1, nZen + 1, 4, 4), lowMns(nLow + 1, nZen + 1, 4, 4);
-    if (thetaFactor > 1)
-    {
-        lowM.ClearArr();
-        UpsampleTheta(coarseM, nLow, nZenLow, nZen, lowM);
-        if (ns)
-        {
-            lowMns.ClearArr();
-            UpsampleTheta(coarseMns, nLow, nZenLow, nZen, lowMns);
-        }
-    }
-    else
-    {
-        lowM = coarseM;
-        if (ns)
-            lowMns = coarseMns;
-    }
-    if (!doCheck && !adaptiveRows)
-    {
-        UpsamplePhi(lowM, nLow, nFull, nZen, localM);
-        if (ns)
-            UpsamplePhi(lowMns, nLow, nFull, nZen, localM_noshadow);
+    if (nOrient == 0)
         return true;
-    }
-    // validation (:3745-3782): a direct run on min(nFull, 2 nLow) azimuths,
-    // upsampled the same way, estimates the error; above the tolerance it
-    // replaces the interpolated grids
-    Arr2D fftM(nFull + 1, nZen + 1, 4, 4), fftMns(nFull + 1, nZen + 1, 4, 4);
-    fftM.ClearArr();
-    fftMns.ClearArr();
-    UpsamplePhi(lowM, nLow, nFull, nZen, fftM);
-    if (ns)
-        UpsamplePhi(lowMns, nLow, nFull, nZen, fftMns);
+    const bool ns = ComputeNoShadow();
+    const PackScale sc(scale, waveIndex, AbsorptionCoefficient());
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    ThreadContexts::Entry *e = nullptr;
+    if (!DiffractResident(prepared, start, nOrient,
+                          CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight), sc, c, ns, e))
+        return false;
+    if (!e)
+        return true;
+    std::vector<double> grid, gridNs, rows;
+    if (!Upsampled(*e, false, m_sphere, grid, adaptiveRows ? &rows : nullptr)
+        || (ns && !Upsampled(*e, true, m_sphere, gridNs, nullptr)))
+        return false;
     if (doCheck)
     {
         const int nCheck = std::min(nFull, nLow * 2);
-        ScatteringRange checkSphere = saved;
+        ScatteringRange checkSphere = m_sphere;
         checkSphere.nAzimuth = nCheck;
         checkSphere.azinuthStep = 2.0 * M_PI / nCheck;
         checkSphere.ComputeSphereDirections(*m_incidentLight);
-        Arr2D checkLowM(nCheck + 1, nZen + 1, 4, 4), checkLowMns(nCheck + 1, nZen + 1, 4, 4);
-        checkLowM.ClearArr();
-        checkLowMns.ClearArr();
-        m_sphere = checkSphere;
-        const bool checkOk = HandleOrientationsToLocalGpu(prepared, start, count, checkLowM,
-                                                          checkLowMns, scale, waveIndex);
-        m_sphere = saved;
-        if (!checkOk)
+        std::vector<double> check, checkNs, checkRows;
+        if (!Rediffract(*e, checkSphere, c, ns)
+            || !Upsampled(*e, false, m_sphere, check, adaptiveRows ? &checkRows : nullptr)
+            || (ns && !Upsampled(*e, true, m_sphere, checkNs, nullptr)))
             return false;
-        Arr2D checkM(nFull + 1, nZen + 1, 4, 4), checkMns(nFull + 1, nZen + 1, 4, 4);
-        checkM.ClearArr();
-        checkMns.ClearArr();
-        UpsamplePhi(checkLowM, nCheck, nFull, nZen, checkM);
+        const InterpolationError full = Compare(grid, check);
+        full.Print("full");
+        double err = full.Estimate();
         if (ns)
-            UpsamplePhi(checkLowMns, nCheck, nFull, nZen, checkMns);
-        double err = ReportFftCheck(fftM, checkM, nFull, nZen, "full");
-        if (ns)
-            err = std::max(err, ReportFftCheck(fftMns, checkMns, nFull, nZen, "no-shadow"));
+        {
+            const InterpolationError shadowless = Compare(gridNs, checkNs);
+            shadowless.Print("no-shadow");
+            err = std::max(err, shadowless.Estimate());
+        }
         const bool refine = m_fftTolerance > 0.0 && err > m_fftTolerance;
         m_fftStatistics->RecordValidation(err, refine);
         if (refine)
@@ -970,65 +987,94 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhi(const std::vector<PreparedOri
                          "GPU FFT refinement: estimated error %.6g above tolerance %.6g, using the "
                          "direct Nphi=%d grid\n",
                          err, m_fftTolerance, nCheck);
-            fftM = checkM;
-            if (ns)
-                fftMns = checkMns;
+            grid.swap(check);
+            gridNs.swap(checkNs);
+            rows.swap(checkRows);
         }
     }
-    // adaptive rows (:3783-3832, MBS_FFT_ADAPTIVE_PHI): theta rows whose
-    // averaged polarisation ratios spike against their neighbours are
-    // computed directly on all nFull azimuths and replace the interpolated
-    // rows
     if (adaptiveRows)
     {
-        const std::vector<int> rows =
-            SpikeRows(fftM, nFull, nZen, EnvDouble("MBS_FFT_ADAPTIVE_PHI_THRESHOLD", 0.04),
-                      EnvInt("MBS_FFT_ADAPTIVE_PHI_MAX_ROWS", 0, 128, 8));
-        if (!rows.empty())
+        const std::vector<int> redo =
+            RowsToRecompute(rows, nZen, EnvDouble("MBS_FFT_ADAPTIVE_PHI_THRESHOLD", 0.04),
+                            EnvInt("MBS_FFT_ADAPTIVE_PHI_MAX_ROWS", 0, 128, 8));
+        if (!redo.empty())
         {
-            ScatteringRange rowSphere = saved;
-            rowSphere.isNonUniform = true;
-            rowSphere.thetaValues.clear();
-            for (int t : rows)
-                rowSphere.thetaValues.push_back(saved.GetZenith(t));
-            rowSphere.nAzimuth = nFull;
-            rowSphere.azinuthStep = 2.0 * M_PI / nFull;
-            rowSphere.nZenith = (int)rows.size() - 1;
-            rowSphere.zenithStart = rowSphere.thetaValues.front();
-            rowSphere.zenithEnd = rowSphere.thetaValues.back();
-            rowSphere.zenithStep = rowSphere.nZenith > 0
-                                       ? (rowSphere.zenithEnd - rowSphere.zenithStart) /
-                                             rowSphere.nZenith
-                                       : 0.0;
-            rowSphere.ComputeSphereDirections(*m_incidentLight);
-            Arr2D directM(nFull + 1, (int)rows.size(), 4, 4);
-            Arr2D directMns(nFull + 1, (int)rows.size(), 4, 4);
-            directM.ClearArr();
-            directMns.ClearArr();
-            m_sphere = rowSphere;
-            const bool rowsOk = HandleOrientationsToLocalGpu(prepared, start, count, directM,
-                                                             directMns, scale, waveIndex);
-            m_sphere = saved;
-            if (!rowsOk)
+            if (!Rediffract(*e, RowSphere(m_sphere, redo, *m_incidentLight), c, ns))
                 return false;
-            for (size_t i = 0; i < rows.size(); ++i)
+            const int nRows = (int)redo.size();
+            const size_t n = (size_t)nFull * nRows * 16;
+            std::vector<double> direct(n, 0.0), directNs(ns ? n : 0, 0.0);
+            if (!Status(mbs_accumulate_mueller(e->ctx, direct.data(),
+                                               ns ? directNs.data() : nullptr),
+                        "result copy", e->ctx))
+                return false;
+            for (int i = 0; i < nRows; ++i)
                 for (int p = 0; p < nFull; ++p)
                 {
-                    std::memcpy(fftM.RawCell(p, rows[i]), directM.RawCell(p, (int)i),
-                                16 * sizeof(double));
+                    const size_t from = ((size_t)p * nRows + i) * 16;
+                    const size_t to = ((size_t)p * (nZen + 1) + redo[i]) * 16;
+                    std::copy(&direct[from], &direct[from] + 16, &grid[to]);
                     if (ns)
-                        std::memcpy(fftMns.RawCell(p, rows[i]), directMns.RawCell(p, (int)i),
-                                    16 * sizeof(double));
+                        std::copy(&directNs[from], &directNs[from] + 16, &gridNs[to]);
                 }
-            std::fprintf(stderr, "GPU FFT adaptive phi: %zu of %d theta rows computed directly\n",
-                         rows.size(), nZen + 1);
+            std::fprintf(stderr, "GPU FFT adaptive phi: %d of %d theta rows computed directly\n",
+                         nRows, nZen + 1);
         }
     }
-    AddCells(fftM, nFull, nZen, localM);
+    AddGrid(grid, nFull, nZen, localM);
     if (ns)
-        AddCells(fftMns, nFull, nZen, localM_noshadow);
+        AddGrid(gridNs, nFull, nZen, localM_noshadow);
     return true;
 }
+
+namespace
+{
+// The multi-size members' common part: the beam set packed and uploaded
+// once at scale 1 with its scaling data, then for every scale s the device
+// records re-derived (mbs_rescale_beams, the rule of PackScale and
+// ScaledJones), diffracted on `sphere`, and handed to use(s, context).
+template <class Use>
+bool ForEachScale(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
+                  const ScatteringRange &sphere, const std::vector<double> &scales,
+                  double waveIndex, double cAbs, const mbs_consts &c, Use use)
+{
+    GridSpec g = MakeGrid(sphere);
+    bool ok = false;
+    ThreadContexts::Entry *e = Context(0, g, ok);
+    if (!e)
+        return ok;
+    e->resident = ThreadContexts::Entry::CacheKey();
+    PackedBatch &pk = e->pk;
+    const auto tp = std::chrono::steady_clock::now();
+    if (!Pack(prepared, start, nOrient, PackScale(), pk, true))
+        return false;
+    const double packMs =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
+    if (pk.nv.empty())
+        return true;
+    const mbs_beam_batch view = pk.View();
+    mbs_beam_scaling bsc;
+    bsc.raw_jones = pk.rawJones.data();
+    bsc.path = pk.path.data();
+    bsc.sign = pk.sign.data();
+    bsc.abs_offsets = pk.absOffsets.data();
+    bsc.abs_paths = pk.absPaths.data();
+    bsc.c_abs = cAbs;
+    bsc.wave_index = waveIndex;
+    int rc = mbs_upload_beams(e->ctx, &view, &c);
+    if (rc == MBS_OK)
+        rc = mbs_upload_beam_scaling(e->ctx, &bsc);
+    if (!Status(rc, "upload", e->ctx))
+        return false;
+    for (size_t s = 0; s < scales.size(); ++s)
+    {
+        if (!Status(mbs_rescale_beams(e->ctx, scales[s]), "rescaled batch", e->ctx)
+            || !DiffractBatch(*e, pk, c, false, s == 0 ? packMs : 0.0, false) || !use(s, *e))
+            return false;
+    }
+    return true;
+}
+}  // namespace
 
 // The multi-size path's fused call (src/scattering/TracerPOTotal.cpp:4380-4432):
 // one traced beam set, diffracted for every scale s into localMs[s] (full
@@ -1049,58 +1095,20 @@ bool HandlerPO::HandleOrientationsToLocalGpuMultiK(const std::vector<PreparedOri
     const int nOrient = std::min(count, (int)prepared.size() - start);
     if (nOrient == 0 || m_sphere.nAzimuth * (m_sphere.nZenith + 1) == 0)
         return true;
-    GridSpec g = MakeGrid(m_sphere);
-    bool ok = false;
-    ThreadContexts::Entry *e = Context(0, g, ok);
-    if (!e)
-        return ok;
     const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
                                        m_legacySign);
-    PackedBatch &pk = e->pk;
-    // one pack and upload at scale 1 with the scaling data; every scale is
-    // then re-derived on the device (mbs_rescale_beams) by the rule of the
-    // per-scale pack (PackScale, ScaledJones)
-    const auto tp = std::chrono::steady_clock::now();
-    if (!Pack(prepared, start, nOrient, PackScale(), pk, true))
-        return false;
-    const double packMs =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
-    if (pk.nv.empty())
-        return true;
-    const mbs_beam_batch view = pk.View();
-    mbs_beam_scaling bsc;
-    bsc.raw_jones = pk.rawJones.data();
-    bsc.path = pk.path.data();
-    bsc.sign = pk.sign.data();
-    bsc.abs_offsets = pk.absOffsets.data();
-    bsc.abs_paths = pk.absPaths.data();
-    bsc.c_abs = AbsorptionCoefficient();
-    bsc.wave_index = waveIndex;
-    int rc = mbs_upload_beams(e->ctx, &view, &c);
-    if (rc == MBS_OK)
-        rc = mbs_upload_beam_scaling(e->ctx, &bsc);
-    for (size_t s = 0; s < scales.size() && rc == MBS_OK; ++s)
-    {
-        rc = mbs_rescale_beams(e->ctx, scales[s]);
-        if (rc == MBS_OK && !RunBatch(*e, pk, c, localMs[s], nullptr, s == 0 ? packMs : 0.0,
-                                      false))
-            return false;
-    }
-    if (rc == MBS_ERR_DEVICE)
-        return DeviceFailure("rescaled batch", mbs_last_error(e->ctx));
-    if (rc != MBS_OK)
-    {
-        std::fprintf(stderr, "HIP diffraction backend: %s\n", mbs_last_error(e->ctx));
-        return false;
-    }
-    return true;
+    return ForEachScale(prepared, start, nOrient, m_sphere, scales, waveIndex,
+                        AbsorptionCoefficient(), c,
+                        [&](size_t s, ThreadContexts::Entry &e) {
+                            return AddResult(e, localMs[s], nullptr);
+                        });
 }
 
-// The multi-size form of the phi-FFT path (src/cuda/GpuDiffraction.cu, the
-// member after :3839): only with MBS_GPU_MULTI_K_FULL=1 and full output, as
-// in the reference; the fused multi-size call on the coarse azimuth grid,
-// then every scale's grid upsampled (theta, then phi) into localMs[s].  As
-// in the reference, its validation and adaptive options return false.
+// The multi-size form of the angular FFT path (src/cuda/GpuDiffraction.cu,
+// the member after :3839): only with MBS_GPU_MULTI_K_FULL=1 and full output,
+// as in the reference; every scale is diffracted on the coarse grid and
+// upsampled on the device into localMs[s].  As in the reference, its
+// validation and adaptive options return false.
 bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
     const std::vector<PreparedOrientation> &prepared, int start, int count,
     const std::vector<double> &scales, double waveIndex, std::vector<Arr2D> &localMs)
@@ -1117,49 +1125,37 @@ bool HandlerPO::HandleOrientationsToLocalGpuFftPhiMultiK(
         return false;
     const int nFull = m_sphere.nAzimuth;
     const int nZen = m_sphere.nZenith;
-    const int factor = ChoosePhiFactor(nFull, m_fftPhiFactor);
+    const int factor = PhiFactor(nFull, m_fftPhiFactor);
     int nLow = factor > 1 ? nFull / factor : nFull;
     if (nLow < 16)
         nLow = std::min(nFull, 16);
-    if (factor <= 1 || nFull < 32 || nLow >= nFull)
+    if (factor <= 1 || nFull < 32 || nLow >= nFull || nFull > kMaxUpsampleLine)
     {
         m_fftStatistics->RecordCall(nFull, nFull, 1);
         return HandleOrientationsToLocalGpuMultiK(prepared, start, count, scales, waveIndex,
                                                   localMs);
     }
-    if (localMs.size() < scales.size())
+    if (start < 0 || count < 0 || start > (int)prepared.size() || localMs.size() < scales.size())
         return false;
     m_fftStatistics->RecordCall(nFull, nLow, factor);
+    const int nOrient = std::min(count, (int)prepared.size() - start);
+    if (nOrient == 0 || scales.empty())
+        return true;
     const int thetaFactor = ThetaFactor(m_sphere);
     const int nZenLow = thetaFactor > 1 ? std::max(2, nZen / thetaFactor) : nZen;
-    const ScatteringRange lowSphere = CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight);
-    std::vector<Arr2D> lowMs;
-    lowMs.reserve(scales.size());
-    for (size_t s = 0; s < scales.size(); ++s)
-    {
-        lowMs.push_back(Arr2D(nLow + 1, nZenLow + 1, 4, 4));
-        lowMs.back().ClearArr();
-    }
-    const ScatteringRange saved = m_sphere;
-    m_sphere = lowSphere;
-    const bool ok = HandleOrientationsToLocalGpuMultiK(prepared, start, count, scales,
-                                                       waveIndex, lowMs);
-    m_sphere = saved;
-    if (!ok)
-        return false;
-    for (size_t s = 0; s < scales.size(); ++s)
-    {
-        if (thetaFactor > 1)
-        {
-            Arr2D thetaM(nLow + 1, nZen + 1, 4, 4);
-            thetaM.ClearArr();
-            UpsampleTheta(lowMs[s], nLow, nZenLow, nZen, thetaM);
-            UpsamplePhi(thetaM, nLow, nFull, nZen, localMs[s]);
-        }
-        else
-            UpsamplePhi(lowMs[s], nLow, nFull, nZen, localMs[s]);
-    }
-    return true;
+    const mbs_consts c = HandlerConsts(m_waveIndex, m_eps1, m_complWave, m_invComplWave,
+                                       m_legacySign);
+    const ScatteringRange &full = m_sphere;
+    return ForEachScale(prepared, start, nOrient,
+                        CoarseSphere(m_sphere, nLow, nZenLow, *m_incidentLight), scales,
+                        waveIndex, AbsorptionCoefficient(), c,
+                        [&](size_t s, ThreadContexts::Entry &e) {
+                            std::vector<double> grid;
+                            if (!Upsampled(e, false, full, grid, nullptr))
+                                return false;
+                            AddGrid(grid, nFull, nZen, localMs[s]);
+                            return true;
+                        });
 }
 
 // The reference's theta probe (src/cuda/GpuDiffraction.cu:4780-4965, kernel
@@ -1233,6 +1229,7 @@ bool HandlerPO::DiffractThetasGpu(const std::vector<PreparedOrientation> &prepar
     ThreadContexts::Entry *e = Context(0, g, ok);
     if (!e)
         return ok;
+    e->resident = ThreadContexts::Entry::CacheKey();
     PackedBatch &pk = e->pk;
     if (!Pack(prepared, 0, nOrient, PackScale(), pk))
         return false;

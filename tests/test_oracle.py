@@ -6,6 +6,7 @@ the Mueller grids its CPU routines (and its unmodified CLI) produced for them.
 import numpy as np
 import pytest
 
+import fourier
 from conftest import load_fixture
 
 
@@ -54,8 +55,12 @@ def test_fixed_orientation_matches_reference(pkg, oracle, fixtures, name):
     assert weighted_l2(ma, fx["mueller_full"]) < 2e-7
 
 
-def test_sobol_average_matches_reference(pkg, oracle, fixtures):
-    fx = fixtures["sobol_average"]
+@pytest.mark.parametrize("name", ["sobol_average", "config5_near_forward"])
+def test_sobol_average_matches_reference(pkg, oracle, fixtures, name):
+    """Orientation averages: the small-particle average on the non-uniform
+    grid, and the large-particle near-forward case (D=200 um, L=400 um,
+    m=1.3116+1e-3i, 12 reflections, theta 0-5 deg)."""
+    fx = fixtures[name]
     grid, batch, consts = _setup(pkg, fx)
     with oracle.ref_sincos():
         m, m_ns, per = oracle.diffract(grid, batch, consts, noshadow=True, per_orient=True)
@@ -250,9 +255,9 @@ def test_incoherent_sum_is_the_sum_of_single_beam_runs(pkg, oracle, fixtures):
 
 
 def dirichlet_upsampling_matrix(n_low, n_full):
-    """The real matrix the C++ drop-in applies for the phi upsampling
-    (UpsamplePhi, mbs-fast_amd/host/HandlerPOHip.cpp): K(u) = (1/n_low)
-    sum_k cos(2 pi f_k u) in closed form."""
+    """The upsampling rule as a real matrix in closed form (Dirichlet
+    kernel): K(u) = (1/n_low) sum_k cos(2 pi f_k u) over the frequencies the
+    rule keeps."""
     q = np.arange(n_full)[:, None]
     p = np.arange(n_low)[None, :]
     a = 2 * np.pi * (q / n_full - p / n_low)
@@ -276,21 +281,22 @@ def test_theta_upsampling_reproduces_cosine_series(pkg):
         def f(t):
             return sum(c[j] * np.cos(j * t) for j in range(len(c)))
         low = np.tile(f(np.linspace(0, np.pi, nt_low + 1))[None, :, None], (2, 1, 16))
-        up = pkg.handler.upsample_theta(low, nt_full)
+        up = fourier.upsample_theta(low, nt_full)
         assert up.shape == (2, nt_full + 1, 16)
         np.testing.assert_allclose(up[1, :, 5], f(np.linspace(0, np.pi, nt_full + 1)),
                                    rtol=0, atol=1e-12 * np.abs(c).sum())
 
 
 def test_phi_upsampling_forms(pkg):
-    """The phi upsampling of the FFT path, two ways: the reference's FFT
-    padding (handler.upsample_phi) and the closed-form matrix of the C++
-    drop-in.  Both reproduce a trigonometric polynomial of degree < n_low/2
-    sampled on the coarse grid exactly, and agree on arbitrary data."""
+    """The checker of the device upsampler (tests/fourier.py, the reference's
+    FFT padding) pinned two ways: it reproduces a trigonometric polynomial
+    of degree < n_low/2 sampled on the coarse grid exactly, and it equals
+    the closed-form (Dirichlet) matrix of the same rule on arbitrary data,
+    odd and even n_low."""
     rng = np.random.default_rng(4)
     for n_low, n_full in ((16, 32), (48, 96), (45, 90), (80, 240), (21, 50)):
         x = rng.normal(size=(n_low, 3, 16))
-        fft_form = pkg.handler.upsample_phi(x, n_full)
+        fft_form = fourier.upsample_phi(x, n_full)
         mat_form = np.einsum("qp,p...->q...", dirichlet_upsampling_matrix(n_low, n_full), x)
         np.testing.assert_allclose(mat_form, fft_form, rtol=0, atol=1e-12 * np.abs(x).max())
         deg = (n_low - 1) // 2
@@ -299,7 +305,7 @@ def test_phi_upsampling_forms(pkg):
             return sum(c[j, 0] * np.cos(j * phi) + c[j, 1] * np.sin(j * phi) for j in range(deg + 1))
         lo = trig(2 * np.pi * np.arange(n_low) / n_low)[:, None, None] * np.ones((1, 1, 16))
         want = trig(2 * np.pi * np.arange(n_full) / n_full)
-        np.testing.assert_allclose(pkg.handler.upsample_phi(lo, n_full)[:, 0, 0], want,
+        np.testing.assert_allclose(fourier.upsample_phi(lo, n_full)[:, 0, 0], want,
                                    rtol=0, atol=1e-12 * np.abs(c).sum())
 
 
