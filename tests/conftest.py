@@ -34,4 +34,5 @@ def load_fixture(name):
 
 @pytest.fixture(scope="session")
 def fixtures():
-    return {n: load_fixture(n) for n in ("config1_fixed", "fixed_absorbing", "sobol_average")}
+    return {n: load_fixture(n) for n in ("config1_fixed", "fixed_absorbing", "sobol_average",
+                                         "config5_near_forward")}

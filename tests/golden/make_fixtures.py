@@ -34,13 +34,15 @@ import shutil
 import struct
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 REF_BIN = os.path.join(REPO, "oracle", "_ref")
-WORK = os.path.join(REPO, "_work", "fixtures")
+# scratch directory of the reference runs (set by main)
+WORK = None
 
 
 class _Reader:
@@ -261,11 +263,20 @@ def save(name, fx):
 
 def main(only=()):
     """Writes every fixture, or only the named ones (command-line arguments)."""
-    os.makedirs(WORK, exist_ok=True)
+    global WORK
+    WORK = tempfile.mkdtemp(prefix="mbs_fixtures_")
+    try:
+        _main(only)
+    finally:
+        shutil.rmtree(WORK, ignore_errors=True)
+
+
+def _main(only):
     want = lambda name: not only or name in only  # noqa: E731
     if want("sobol_points"):
         save("sobol_points", sobol_points())
-    if not (want("config1_fixed") or want("fixed_absorbing") or want("sobol_average")):
+    if not any(want(n) for n in ("config1_fixed", "fixed_absorbing", "sobol_average",
+                                 "config5_near_forward")):
         return
     # Configuration 1: hexagonal column L=20 D=10 um, m=1.3116, lambda=0.532,
     # 8 reflections, beta=30 gamma=10, theta 0..180 in 720 steps x 48 phi.
@@ -282,6 +293,13 @@ def main(only=()):
     # Orientation average: reference Owen-Sobol (seed 1) orientations of the
     # configuration-2 particle shape scaled down (L=20 D=10 um) with absorption,
     # the non-uniform theta grid of configuration 2 on 24 phi bins.
+    # Large-particle near-forward case: the hexagonal column D=200 L=400 um,
+    # m=1.3116+1e-3i, 12 reflections, 4 Sobol orientations (seed 1); the
+    # expected grids on theta 0..5 deg at 0.05 deg x 12 phi (the tests run
+    # the full 0.005 deg grid against the oracle).
+    if want("config5_near_forward"):
+        save("config5_near_forward", sobol_case("config5_near_forward", 400, 200, 1.3116, 1e-3,
+                                                0.532, 12, 4, 1, 12, 100, th0=0, th1=5))
     tg = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
     if want("sobol_average"):
         save("sobol_average", sobol_case("sobol_average", 20, 10, 1.3116, 1e-4, 0.532, 6,
