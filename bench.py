@@ -43,7 +43,12 @@ sys.path.insert(0, ROOT)
 
 ORIENT_PER_STEP = 16
 BEAMS_MIN, BEAMS_MAX = 2000, 15000
-FP64_PEAK = 78.6e12   # MI355X FP64 vector (= FP64 MFMA) FLOP/s
+# MI355X published vector-ALU peaks (FLOP/s).  The mixed kernel's aperture
+# integral runs FP32 on the vector ALU (its phase arguments FP64); its bound
+# is the FP32 vector peak.  Without packed FP32 (not used by this build) an
+# FP32 FMA issues at the FP64 rate, 78.6 TFLOP/s.
+FP32_PEAK = 157.3e12
+FP64_PEAK = 78.6e12
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "diffract_hbm_traffic_d50_541x96.txt")
 
 
@@ -71,6 +76,28 @@ def calibrate_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"calibrate_copy_bytes": n, "launches": 3}))
+
+
+def algorithmic_bytes(batch, cells):
+    """HBM bytes one launch must move at least: the device beam records read
+    once (FP64 header 320 B, FP32 header 80 B, MFMA operand 256 B per beam;
+    vertex records 32 + 16 + 4 B per vertex), the grid tables read once
+    (15 FP64 per direction) and the Mueller grid written once (16 FP64 per
+    direction)."""
+    n_vert = int(batch.vert_offsets[-1])
+    return float(batch.n_beams) * (320 + 80 + 256) + n_vert * 52.0 + cells * (15 + 16) * 8.0
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def unit_flops(batch):
@@ -241,6 +268,7 @@ def run(args):
     grid = syn.config2_grid()
     flops_b, ne = unit_flops(batch)
     flops_launch = float(flops_b.sum()) * grid.cells
+    alg_bytes = algorithmic_bytes(batch, grid.cells)
     units = float(batch.n_beams) * grid.cells
     hip = Hip()
     ctx = eng.Context(0)
@@ -252,9 +280,10 @@ def run(args):
     upload_ms = 1e3 * (time.perf_counter() - t_up)
     results = {}
     traffic = measured_traffic()
-    legs = (("mixed", abi.PREC_MIXED), ("mixed_valu", abi.PREC_MIXED | abi.NO_MFMA),
-            ("fp64", abi.PREC_FP64))
-    for name, flags in legs:
+    legs = (("mixed", abi.PREC_MIXED, FP32_PEAK),
+            ("mixed_valu", abi.PREC_MIXED | abi.NO_MFMA, FP32_PEAK),
+            ("fp64", abi.PREC_FP64, FP64_PEAK))
+    for name, flags, peak in legs:
         for _ in range(args.warmup):
             ctx.diffract(flags)
         hip.sync()
@@ -271,10 +300,13 @@ def run(args):
             "value": units * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
             "kernel_ms": kms,
-            "roofline": {"bound": "mfma", "achieved": flops_launch / (kms * 1e-3) / 1e12,
-                         "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                         "frac": flops_launch / (kms * 1e-3) / FP64_PEAK,
-                         "traffic": traffic.get(name)},
+            "roofline": {"bound": "valu", "achieved": flops_launch / (kms * 1e-3) / 1e12,
+                         "peak": peak / 1e12, "unit": "TFLOP/s",
+                         "frac": flops_launch / (kms * 1e-3) / peak,
+                         "traffic": traffic.get(name),
+                         "algorithmic_bytes": alg_bytes,
+                         "traffic_over_algorithmic": (traffic[name] / alg_bytes
+                                                      if name in traffic else None)},
             "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
         }
     fft = bench_aperture_fft(ctx, batch, args)
@@ -326,6 +358,7 @@ def run(args):
             except Exception as e:  # the oracle port stays the baseline
                 port["reference_error"] = str(e)
         out["cpu_baseline"] = ref if ref is not None else port
+        out["cpu_baseline"]["cpu_model"] = cpu_model()
         if ref is not None:
             out["cpu_oracle_port"] = port
         out["gpu_over_cpu_fp64"] = results["fp64"]["value"] / out["cpu_baseline"]["value"]
