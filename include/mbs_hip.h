@@ -337,6 +337,29 @@ int mbs_mlp_gradient(mbs_mlp *m, const float *x, const float *t, int32_t rows, d
 int mbs_mlp_train(mbs_mlp *m, const float *x, const float *t, int32_t n, int32_t batch,
                   int32_t epochs, float lr, uint64_t seed, double *final_loss);
 
+/* ---- the entry point that allocates nothing ------------------------------
+ * No counterpart in the reference.  It is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  mbs_diffract_noalloc evaluates the uploaded batch on
+ * the current grid like mbs_diffract(flags) and WRITES (does not add) the
+ * weighted Mueller sum to caller-owned device memory d_m_full
+ * ([n_az*(n_zen+1)*16] FP64, row-major 4x4 per cell, cell (p, t) at
+ * p*(n_zen+1)+t), and with MBS_NOSHADOW the no-shadow sum to d_m_noshadow
+ * (NULL without it).  It calls no hipMalloc, hipFree or allocating library
+ * routine: its work list and partial sums live in one fixed device array of
+ * MBS_NOALLOC_SCRATCH_BYTES per device, which the library holds until the
+ * process ends, and the launch is sized to fit it (on large grids fewer,
+ * larger work items than mbs_diffract uses).  The allocations it relies on
+ * are made beforehand by mbs_ctx_create, mbs_set_grid and mbs_upload_beams.
+ * Flags: MBS_PREC_MIXED, MBS_NO_MFMA, MBS_NOSHADOW.  MBS_ERR_CAPACITY when
+ * one work item's Mueller grid (n_az*(n_zen+1)*128 bytes, twice that with
+ * MBS_NOSHADOW) exceeds the array.  Asynchronous on the context's stream.
+ * Every call on one device shares that device's array and leaves its
+ * contents behind: calls that overlap on one device (two contexts or host
+ * threads) race, and the caller must serialise them. */
+#define MBS_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int mbs_diffract_noalloc(mbs_ctx *ctx, uint32_t flags, double *d_m_full, double *d_m_noshadow);
+
 /* Device pointer of the result grid of the last mbs_diffract (FP64,
  * [n_az*(n_zen+1)*16]); valid until the next mbs_upload_beams/mbs_set_grid. */
 int mbs_device_result(mbs_ctx *ctx, const double **m_full,

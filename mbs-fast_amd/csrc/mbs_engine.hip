@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,55 @@ static int fail(mbs_ctx *ctx, int code, const char *fmt, ...)
                         hipGetErrorString(e_));                                   \
     } while (0)
 
+// Scratch memory of mbs_diffract_noalloc (work list and partial sums): one
+// fixed array per device, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, not device memory; the device
+// address of each device's copy comes from hipGetSymbolAddress, looked up
+// once per device when the first context on it is created (with that device
+// current), and only that address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[MBS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+struct ScratchSlot {
+    bool looked_up = false;
+    hipError_t err = hipSuccess;
+    unsigned char *ptr = nullptr;
+};
+ScratchSlot g_scratch[kMaxDevices];
+}  // namespace
+
+// With `device` current: records the address of its copy of `scratch`.
+static void scratch_lookup(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ScratchSlot &slot = g_scratch[device];
+    if (slot.looked_up)
+        return;
+    void *p = nullptr;
+    slot.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (slot.err != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();  // not left for a later launch check to report
+    }
+    slot.ptr = (unsigned char *)p;
+    slot.looked_up = true;
+}
+
+static unsigned char *scratch_address(int device, hipError_t *err)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices || !g_scratch[device].looked_up) {
+        *err = hipErrorInvalidDevice;
+        return nullptr;
+    }
+    *err = g_scratch[device].err;
+    return g_scratch[device].ptr;
+}
+
 extern "C" {
 
 int mbs_abi_version(void) { return MBS_ABI_VERSION; }
@@ -218,6 +268,7 @@ int mbs_ctx_create(int device, mbs_ctx **out)
         return fail(nullptr, MBS_ERR_DEVICE, "HIP device %d not available (%d visible)",
                     device, n);
     HIP_TRY(nullptr, hipSetDevice(device));
+    scratch_lookup(device);
     mbs_ctx *ctx = new mbs_ctx();
     ctx->device = device;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -831,6 +882,117 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     ctx->have_result = true;
     ctx->have_ns = ns;
     ctx->have_per = per;
+    return MBS_OK;
+}
+
+// Orientation reduction into caller-owned device memory, with the work list
+// and the partial sums in `scratch` (see scratch_lookup): the launch is the
+// one mbs_diffract makes, with the item count capped so that everything fits
+// the array.
+int mbs_diffract_noalloc(mbs_ctx *ctx, uint32_t flags, double *d_m_full, double *d_m_noshadow)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (!ctx->grid_ok)
+        return fail(ctx, MBS_ERR_NO_GRID, "mbs_set_grid has not been called");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_NO_MFMA))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "mbs_diffract_noalloc takes MBS_PREC_MIXED, MBS_NO_MFMA and MBS_NOSHADOW");
+    const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
+    if (!d_m_full || (ns != 0) != (d_m_noshadow != nullptr))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "mbs_diffract_noalloc: d_m_full is required, d_m_noshadow exactly with "
+                    "MBS_NOSHADOW");
+    hipError_t serr = hipSuccess;
+    unsigned char *base_ptr = scratch_address(ctx->device, &serr);
+    if (!base_ptr)
+        return fail(ctx, MBS_ERR_DEVICE, "no scratch address for device %d: %s", ctx->device,
+                    hipGetErrorString(serr));
+    const int base = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    const int prec = base ? ((flags & MBS_NO_MFMA) ? 1 : 2) : 0;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_tables(ctx);
+    if (rc)
+        return rc;
+    const int cells = ctx->cells;
+    const int n_tiles = (cells + 255) / 256;
+    const size_t mcount = (size_t)cells * 16;
+    hipStream_t s = ctx->stream;
+    // Items that fit: every item needs its partial grid(s) and a list entry
+    // (one alignment gap of at most 256 bytes between list and partials).
+    const size_t cap = MBS_NOALLOC_SCRATCH_BYTES - 256;
+    const size_t per_jones = (size_t)cells * 8 * sizeof(double) * (1 + ns) + sizeof(WorkItem);
+    const size_t per_mueller = (size_t)cells * 16 * sizeof(double) * (1 + ns) + sizeof(WorkItem);
+    if (per_mueller > cap)
+        return fail(ctx, MBS_ERR_CAPACITY,
+                    "mbs_diffract_noalloc: a grid of %d cells needs %zu bytes of scratch per "
+                    "work item, above the %zu of MBS_NOALLOC_SCRATCH_BYTES",
+                    cells, per_mueller, (size_t)MBS_NOALLOC_SCRATCH_BYTES);
+    const int64_t max_jones = (int64_t)(cap / per_jones);
+    const int64_t max_mueller = (int64_t)(cap / per_mueller);
+    std::vector<WorkItem> &items = ctx->h_items;
+    bool jones = false;
+    if (ctx->n_orient <= max_jones) {
+        // beam segments, halving the target until the segments fit
+        int64_t target = std::min<int64_t>(ctx->target_blocks, max_jones * n_tiles);
+        for (;;) {
+            jones = plan_items(ctx, n_tiles, (int)target, MBS_WORK_SEGMENTS, items);
+            if ((int64_t)items.size() <= max_jones || target <= n_tiles)
+                break;
+            target = std::max<int64_t>(n_tiles, target / 2);
+        }
+        if ((int64_t)items.size() > max_jones)
+            jones = false;
+    }
+    if (!jones) {
+        const int64_t target = std::min<int64_t>(ctx->target_blocks, max_mueller * n_tiles);
+        plan_items(ctx, n_tiles, (int)target, MBS_WORK_ORIENTATIONS, items);
+        if ((int64_t)items.size() > max_mueller)
+            return fail(ctx, MBS_ERR_CAPACITY, "mbs_diffract_noalloc: %zu work items exceed %lld",
+                        items.size(), (long long)max_mueller);
+    }
+    const int n_items = (int)items.size();
+    if (n_items == 0) {
+        const int zb = (int)((mcount + 255) / 256);
+        zero_kernel<<<zb, 256, 0, s>>>(d_m_full, mcount);
+        if (ns)
+            zero_kernel<<<zb, 256, 0, s>>>(d_m_noshadow, mcount);
+        HIP_TRY(ctx, hipGetLastError());
+        return MBS_OK;
+    }
+    const size_t list_bytes = ((sizeof(WorkItem) * n_items + 255) / 256) * 256;
+    const size_t pstride = (size_t)cells * (jones ? 8 : 16);
+    const size_t part_bytes = (size_t)n_items * pstride * sizeof(double);
+    if (list_bytes + part_bytes * (1 + ns) > MBS_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, MBS_ERR_CAPACITY, "mbs_diffract_noalloc: %zu bytes of scratch needed",
+                    list_bytes + part_bytes * (1 + ns));
+    WorkItem *d_items = (WorkItem *)base_ptr;
+    double *d_part = (double *)(base_ptr + list_bytes);
+    double *d_part_ns = ns ? (double *)(base_ptr + list_bytes + part_bytes) : nullptr;
+    HIP_TRY(ctx, hipMemcpyAsync(d_items, items.data(), sizeof(WorkItem) * n_items,
+                                hipMemcpyHostToDevice, s));
+    KernelArgs a;
+    batch_args(ctx, prec, n_items, n_tiles, a);
+    a.items = d_items;
+    a.out = d_part;
+    a.out_ns = d_part_ns;
+    a.per_orient = nullptr;
+    const int64_t total = (int64_t)n_items * n_tiles;
+    if (total > (int64_t)1 << 30)
+        return fail(ctx, MBS_ERR_CAPACITY, "launch of %lld workgroups", (long long)total);
+    const int blocks = (int)((total + 7) / 8 * 8);
+    ctx->last_items = n_items;
+    ctx->last_blocks = blocks;
+    rc = issue_diffraction(ctx, prec, ns, jones, 0, a, blocks, n_items, d_m_full, d_m_noshadow,
+                           nullptr, ctx->timing);
+    if (rc)
+        return rc;
+    if (ctx->timing) {
+        HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+        ctx->ev_diff = true;
+    }
     return MBS_OK;
 }
 

@@ -58,6 +58,8 @@ def lib():
         L.mbs_rescale_beams.restype = C.c_int
         L.mbs_diffract.argtypes = [vp, C.c_uint32]
         L.mbs_diffract.restype = C.c_int
+        L.mbs_diffract_noalloc.argtypes = [vp, C.c_uint32, vp, vp]
+        L.mbs_diffract_noalloc.restype = C.c_int
         L.mbs_accumulate_mueller.argtypes = [vp, dp, dp]
         L.mbs_accumulate_mueller.restype = C.c_int
         L.mbs_fetch_per_orient.argtypes = [vp, dp]
@@ -208,6 +210,13 @@ class Context:
 
     def diffract(self, flags=0):
         self._check(self._L.mbs_diffract(self._h, flags))
+
+    def diffract_noalloc(self, flags, d_m_full, d_m_noshadow=None):
+        """mbs_diffract_noalloc: writes the Mueller sum of the uploaded batch
+        into caller-owned device memory (device addresses as ints)."""
+        self._check(self._L.mbs_diffract_noalloc(self._h, flags, C.c_void_p(d_m_full),
+                                                 C.c_void_p(d_m_noshadow)
+                                                 if d_m_noshadow else None))
 
     def accumulate(self, m_full, m_noshadow=None):
         for a in (m_full, m_noshadow):
