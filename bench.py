@@ -278,24 +278,50 @@ def run(args):
     ctx.upload(batch, consts)  # H2D copy from pageable memory + pack, outside the timed region
     hip.sync()
     upload_ms = 1e3 * (time.perf_counter() - t_up)
+    # the same batch through the page-locked staging (mbs_stage_beams +
+    # mbs_upload_staged): host fill of the staging, asynchronous copies, pack
+    ctx.upload_staged(batch, consts)   # first use allocates the staging
+    hip.sync()
+    t_up = time.perf_counter()
+    ctx.upload_staged(batch, consts)
+    hip.sync()
+    staged_ms = 1e3 * (time.perf_counter() - t_up)
+    staged_device_ms = ctx.timing()["upload_ms"]   # copies + pack on the stream (HIP events)
+    upload_bytes = (batch.beam_scalars.nbytes + batch.beam_nv.nbytes + batch.beam_external.nbytes
+                    + batch.vert_offsets.nbytes + 4 * batch.vx.nbytes + batch.edge_flags.nbytes
+                    + batch.orient_offsets.nbytes + batch.weights.nbytes)
     results = {}
     traffic = measured_traffic()
-    legs = (("mixed", abi.PREC_MIXED, FP32_PEAK),
-            ("mixed_valu", abi.PREC_MIXED | abi.NO_MFMA, FP32_PEAK),
-            ("fp64", abi.PREC_FP64, FP64_PEAK))
-    for name, flags, peak in legs:
-        for _ in range(args.warmup):
-            ctx.diffract(flags)
-        hip.sync()
+
+    def leg(name, flags, peak, mode=None, graph=False):
+        """Times args.steps calls of mbs_diffract(flags): wall per step, and
+        the main kernel's mean duration from the engine's events (graph
+        replay brackets the whole sequence, so the kernel time comes from an
+        eager pass of the same shape first)."""
+        if mode is not None:
+            ctx.set_work_mode(mode)
         kernel_ms = []
+        for _ in range(max(1, args.warmup)):
+            ctx.diffract(flags)
+        for _ in range(min(args.steps, 5)):
+            ctx.diffract(flags)
+            kernel_ms.append(ctx.timing()["kernel_ms"])
+        tm = ctx.timing()
+        if graph:
+            ctx.set_graph(True)
+            for _ in range(max(1, args.warmup)):
+                ctx.diffract(flags)
+        hip.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             ctx.diffract(flags)
-            kernel_ms.append(ctx.timing()["kernel_ms"])  # waits for this step
+            if not graph:
+                kernel_ms.append(ctx.timing()["kernel_ms"])  # waits for this step
         hip.sync()
         wall = time.perf_counter() - t0
+        if graph:
+            ctx.set_graph(False)
         kms = float(np.mean(kernel_ms))
-        tm = ctx.timing()
         results[name] = {
             "value": units * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -303,12 +329,31 @@ def run(args):
             "roofline": {"bound": "valu", "achieved": flops_launch / (kms * 1e-3) / 1e12,
                          "peak": peak / 1e12, "unit": "TFLOP/s",
                          "frac": flops_launch / (kms * 1e-3) / peak,
-                         "traffic": traffic.get(name),
+                         "traffic": traffic.get(name.split("_")[0]),
                          "algorithmic_bytes": alg_bytes,
-                         "traffic_over_algorithmic": (traffic[name] / alg_bytes
-                                                      if name in traffic else None)},
+                         "traffic_over_algorithmic": (traffic[name.split("_")[0]] / alg_bytes
+                                                      if name.split("_")[0] in traffic
+                                                      else None)},
             "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
+            "graph": graph,
         }
+
+    # the default launch shapes (beam segments planned on the host; the
+    # device work queue), then the shape the autotuner picks, replayed as a
+    # graph: the headline
+    leg("mixed_segments", abi.PREC_MIXED, FP32_PEAK, mode=abi.WORK_SEGMENTS)
+    leg("mixed_queue", abi.PREC_MIXED, FP32_PEAK, mode=abi.WORK_QUEUE)
+    leg("mixed_valu", abi.PREC_MIXED | abi.NO_MFMA, FP32_PEAK, mode=abi.WORK_SEGMENTS)
+    leg("fp64", abi.PREC_FP64, FP64_PEAK, mode=abi.WORK_SEGMENTS)
+    ctx.set_work_mode(0)
+    t0 = time.perf_counter()
+    tune = ctx.autotune(abi.PREC_MIXED)
+    tune_s = time.perf_counter() - t0
+    best = min(tune, key=lambda e: e[3])
+    leg("mixed", abi.PREC_MIXED, FP32_PEAK, graph=True)
+    results["mixed"]["tuned"] = {"target_blocks": best[0], "work_mode": best[1],
+                                 "mfma": bool(best[2]), "candidates": len(tune),
+                                 "tuning_s": tune_s}
     fft = bench_aperture_fft(ctx, batch, args)
     ctx.close()
     hist = np.bincount(batch.beam_nv, minlength=32)
@@ -340,12 +385,29 @@ def run(args):
         },
         "kernel_ms": results["mixed"]["kernel_ms"],
         "upload_ms": upload_ms,
-        "pcie_inclusive_value": units / ((upload_ms + results["mixed"]["ms_per_step"]) * 1e-3),
+        "upload": {"bytes": upload_bytes, "pageable_ms": upload_ms,
+                   "pageable_GBps": upload_bytes / (upload_ms * 1e-3) / 1e9,
+                   "staged_ms": staged_ms,
+                   "staged_device_ms": staged_device_ms,
+                   "staged_GBps": upload_bytes / (staged_device_ms * 1e-3) / 1e9,
+                   "note": "pageable: wall of mbs_upload_beams (copies from pageable memory "
+                           "and the device pack); staged: the batch filled into page-locked "
+                           "staging by the host (staged_ms, wall) and its asynchronous copies "
+                           "plus the device pack on the stream (staged_device_ms, events)"},
+        "pcie_inclusive_value": units / ((staged_device_ms + results["mixed"]["ms_per_step"])
+                                         * 1e-3),
         "roofline": results["mixed"]["roofline"],
+        "launch": results["mixed"]["tuned"],
         "fp64": {k: results["fp64"][k] for k in ("value", "ms_per_step", "kernel_ms",
                                                  "roofline")},
         "mixed_valu": {k: results["mixed_valu"][k] for k in ("value", "ms_per_step", "kernel_ms",
                                                              "roofline")},
+        "mixed_segments": {k: results["mixed_segments"][k]
+                           for k in ("value", "ms_per_step", "kernel_ms", "work_items",
+                                     "workgroups")},
+        "mixed_queue": {k: results["mixed_queue"][k]
+                        for k in ("value", "ms_per_step", "kernel_ms", "work_items",
+                                  "workgroups")},
         "aperture_fft": fft,
     }
     if not args.no_cpu:

@@ -171,6 +171,37 @@ int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *grid);
 int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *batch,
                      const mbs_consts *consts);
 
+/* Page-locked staging of an upload (no reference counterpart; the reference
+ * copies from pageable packing buffers, src/cuda/GpuDiffraction.cu:4572-4606).
+ * mbs_stage_beams hands out host arrays in page-locked memory owned by the
+ * context, sized for a batch of n_orient orientations, n_beams beams and
+ * n_vert vertices, in the layout of mbs_beam_batch; the caller fills them
+ * and calls mbs_upload_staged, which checks and uploads them like
+ * mbs_upload_beams but returns as soon as the copies are queued: they run as
+ * DMA at the link's rate while the host goes on, and the next mbs_diffract
+ * on the context follows them on its stream.  Two staging areas are used in
+ * turn, so the caller can fill the next batch while the previous one is
+ * still in flight; mbs_stage_beams waits for the copies out of the area it
+ * hands back.  The arrays stay valid until the second mbs_stage_beams after
+ * this one. */
+typedef struct mbs_beam_staging {
+    int32_t n_orient;
+    int32_t n_beams;
+    int64_t n_vert;
+    int64_t *orient_offsets; /* [n_orient+1] */
+    double *weights;         /* [n_orient]   */
+    double *beam_scalars;    /* [n_beams * MBS_BEAM_SCALARS] */
+    int32_t *beam_nv;        /* [n_beams]    */
+    int32_t *beam_external;  /* [n_beams]    */
+    int64_t *vert_offsets;   /* [n_beams+1]  */
+    double *vx, *vy;         /* [n_vert]     */
+    double *slope_yx, *slope_xy;
+    uint8_t *edge_flags;     /* [n_vert]     */
+} mbs_beam_staging;
+int mbs_stage_beams(mbs_ctx *ctx, int32_t n_orient, int32_t n_beams, int64_t n_vert,
+                    mbs_beam_staging *out);
+int mbs_upload_staged(mbs_ctx *ctx, const mbs_consts *consts);
+
 /* Per-beam data from which the device re-derives a size-rescaled beam set
  * (the multi-size path: the reference packs every scale on the host,
  * src/cuda/GpuDiffraction.cu:273-358, applied when |scale - 1| > 1e-15,
@@ -387,24 +418,33 @@ int mbs_set_target_blocks(mbs_ctx *ctx, int blocks);
 
 /* Work decomposition of mbs_diffract: 0 = automatic (beam segments of equal
  * size unless their partial Jones grids exceed 8 GiB), 1 = runs of whole
- * orientations, 2 = beam segments.  Results agree to rounding. */
+ * orientations, 2 = beam segments, 3 = beam segments planned on the device
+ * and run by a persistent launch that takes (segment, direction tile) units
+ * from a device work queue (one atomic counter per XCD over a contiguous
+ * range of units, ranges of other XCDs taken when its own is empty); with
+ * graph replay the plan is part of the graph, so a replay follows a new
+ * upload with the same orientation count without any host work.
+ * MBS_PER_ORIENT and MBS_INCOHERENT run mode 3 as mode 2.  Results agree
+ * to rounding; modes 2 and 3 bit for bit. */
 #define MBS_WORK_AUTO 0
 #define MBS_WORK_ORIENTATIONS 1
 #define MBS_WORK_SEGMENTS 2
+#define MBS_WORK_QUEUE 3
 int mbs_set_work_mode(mbs_ctx *ctx, int mode);
 
 /* Autotuner of the launch shape (no counterpart in the reference, whose CUDA
  * path sizes its launch from fixed heuristics, GpuDiffraction.cu:2948-3029):
  * times mbs_diffract(flags) on the uploaded batch for every candidate
- * (target workgroups 4096..131072 x whole-orientation / beam-segment work
- * items x, for mixed precision, MFMA / vector-ALU coherent sum), keeps the
- * fastest for later mbs_diffract calls of the same precision, and reports
+ * (target workgroups 4096..131072 x whole-orientation / beam-segment /
+ * device-queue work items x, for mixed precision, MFMA / vector-ALU coherent
+ * sum), keeps the fastest for later mbs_diffract calls of the same
+ * precision, and reports
  * what it timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.
  * flags: MBS_PREC_MIXED and/or MBS_NOSHADOW only.  Leaves a result like
  * mbs_diffract(flags). */
 typedef struct mbs_tune_entry {
     int32_t target_blocks;
-    int32_t work_mode;  /* 1 whole orientations, 2 beam segments */
+    int32_t work_mode;  /* 1 whole orientations, 2 beam segments, 3 device queue */
     int32_t mfma;       /* 1 MFMA contraction, 0 vector ALU (mixed only) */
     float ms;           /* main kernel + reduction, device time */
 } mbs_tune_entry;
@@ -416,6 +456,10 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
  * replays it while launch shape, buffers and sizes are unchanged (the work
  * list is re-uploaded before each replay); re-captured otherwise. */
 int mbs_set_graph(mbs_ctx *ctx, int enable);
+
+/* Graph use of the context so far: captures made and graph launches (a
+ * launch that needed no capture is a replay). */
+int mbs_graph_stats(mbs_ctx *ctx, int64_t *captures, int64_t *launches);
 
 /* The HIP stream the context launches on (a hipStream_t, as void*), for
  * timing with events recorded on the same stream. */

@@ -23,6 +23,7 @@ STATUS_NAMES = {0: "MBS_OK", 1: "MBS_ERR_INVALID", 2: "MBS_ERR_BAD_BEAM",
 B_HOR, B_VER, B_DIR, B_CEN, B_AREA = 0, 3, 6, 9, 12
 B_NT, B_NP, B_NXDT, B_NXDP, B_JP, B_NORMAL = 13, 16, 19, 22, 25, 33
 BEAM_SCALARS = 36
+WORK_AUTO, WORK_ORIENTATIONS, WORK_SEGMENTS, WORK_QUEUE = 0, 1, 2, 3
 EDGE_VALID_X, EDGE_VALID_Y = 1, 2
 MIN_VERTICES, MAX_VERTICES = 3, 31
 
@@ -56,6 +57,16 @@ class mbs_beam_scaling(C.Structure):
     _fields_ = [("raw_jones", _dp), ("path", _dp), ("sign", _dp),
                 ("abs_offsets", C.POINTER(C.c_int64)), ("abs_paths", _dp),
                 ("c_abs", C.c_double), ("wave_index", C.c_double)]
+
+
+class mbs_beam_staging(C.Structure):
+    _fields_ = [("n_orient", C.c_int32), ("n_beams", C.c_int32), ("n_vert", C.c_int64),
+                ("orient_offsets", C.POINTER(C.c_int64)), ("weights", C.POINTER(C.c_double)),
+                ("beam_scalars", C.POINTER(C.c_double)), ("beam_nv", C.POINTER(C.c_int32)),
+                ("beam_external", C.POINTER(C.c_int32)),
+                ("vert_offsets", C.POINTER(C.c_int64)), ("vx", C.POINTER(C.c_double)),
+                ("vy", C.POINTER(C.c_double)), ("slope_yx", C.POINTER(C.c_double)),
+                ("slope_xy", C.POINTER(C.c_double)), ("edge_flags", C.POINTER(C.c_uint8))]
 
 
 class mbs_grid(C.Structure):

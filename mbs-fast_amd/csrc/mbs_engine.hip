@@ -106,10 +106,23 @@ struct mbs_ctx {
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
     size_t pinned_cap = 0;
+    // page-locked staging of uploads (mbs_stage_beams): two areas used in
+    // turn, each with the event that marks the end of its copies
+    struct Staging {
+        unsigned char *mem = nullptr;
+        size_t cap = 0;
+        hipEvent_t copied = nullptr;
+        bool in_flight = false;
+        mbs_beam_staging view{};
+    } stage[2];
+    int stage_next = 0, stage_cur = -1;
     // ~65 workgroups per slot (256 CUs x 4): items small enough that the
     // static per-XCD share of workgroups carries the same work on every XCD
     int target_blocks = 65536;
-    int work_mode = 0;  // 0 auto, 1 whole orientations, 2 beam segments
+    int work_mode = 0;  // 0 auto, 1 whole orientations, 2 beam segments, 3 device queue
+    // device work queue: header, and the item count its plan aims for
+    DevBuf<int> queue;
+    int queue_want = 0;
     // launch shape chosen by mbs_autotune, per precision (0 FP64, 1 mixed)
     struct Tuned {
         bool valid = false;
@@ -119,6 +132,7 @@ struct mbs_ctx {
     bool use_graph = false;
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
+    int64_t graph_captures = 0, graph_replays = 0;
     // stage timing
     bool timing = false;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -312,6 +326,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->wb.release();
     ctx->hdr32.release();
     ctx->items.release();
+    ctx->queue.release();
     ctx->part.release();
     ctx->part_ns.release();
     ctx->res.release();
@@ -337,6 +352,12 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
             (void)hipEventDestroy(e);
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
+    for (auto &st : ctx->stage) {
+        if (st.mem)
+            (void)hipHostFree(st.mem);
+        if (st.copied)
+            (void)hipEventDestroy(st.copied);
+    }
     for (hipEvent_t &e : ctx->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -414,7 +435,12 @@ static int launch_pack(mbs_ctx *ctx, int nb, const mbs_consts &c, double scale, 
     return MBS_OK;
 }
 
-extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts *c)
+// The upload of a batch: host checks, copies and the device pack on the
+// context's stream.  `copied`: when given, the copies end with that event
+// and the call returns without waiting (the source is page-locked staging);
+// otherwise it waits for the stream.
+static int upload_impl(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts *c,
+                       hipEvent_t copied)
 {
     if (!ctx || !bt || !c)
         return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: NULL argument");
@@ -503,6 +529,8 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
         HIP_TRY(ctx, hipMemcpyAsync(ctx->weights.p, bt->weights, sizeof(double) * no,
                                     hipMemcpyHostToDevice, s));
     }
+    if (copied)
+        HIP_TRY(ctx, hipEventRecord(copied, s));
     if (nb > 0) {
         const int rc = launch_pack(ctx, nb, *c, 1.0, false);
         if (rc)
@@ -512,7 +540,8 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
         HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
         ctx->ev_up = true;
     }
-    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (!copied)
+        HIP_TRY(ctx, hipStreamSynchronize(s));
     ctx->h_ooff.assign(bt->orient_offsets, bt->orient_offsets + no + 1);
     if (no == 0)
         ctx->h_ooff.assign(1, 0);
@@ -523,6 +552,109 @@ extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mb
     ctx->consts = *c;
     ctx->beams_ok = true;
     return MBS_OK;
+}
+
+extern "C" int mbs_upload_beams(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts *c)
+{
+    return upload_impl(ctx, bt, c, nullptr);
+}
+
+// Byte layout of one staging area: every array 256-byte aligned.
+static size_t stage_layout(int32_t no, int32_t nb, int64_t nv, size_t off[11])
+{
+    const size_t sizes[11] = {sizeof(int64_t) * (size_t)(no + 1), sizeof(double) * (size_t)no,
+                              sizeof(double) * (size_t)nb * MBS_BEAM_SCALARS,
+                              sizeof(int32_t) * (size_t)nb, sizeof(int32_t) * (size_t)nb,
+                              sizeof(int64_t) * (size_t)(nb + 1), sizeof(double) * (size_t)nv,
+                              sizeof(double) * (size_t)nv, sizeof(double) * (size_t)nv,
+                              sizeof(double) * (size_t)nv, (size_t)nv};
+    size_t total = 0;
+    for (int i = 0; i < 11; ++i) {
+        off[i] = total;
+        total += (sizes[i] + 255) / 256 * 256;
+    }
+    return total;
+}
+
+extern "C" int mbs_stage_beams(mbs_ctx *ctx, int32_t n_orient, int32_t n_beams, int64_t n_vert,
+                               mbs_beam_staging *out)
+{
+    if (!ctx || !out)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_stage_beams: NULL argument");
+    if (n_orient < 0 || n_beams < 0 || n_vert < 0)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_stage_beams: negative size");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const int k = ctx->stage_next;
+    mbs_ctx::Staging &st = ctx->stage[k];
+    if (st.in_flight) {
+        HIP_TRY(ctx, hipEventSynchronize(st.copied));
+        st.in_flight = false;
+    }
+    if (!st.copied)
+        HIP_TRY(ctx, hipEventCreateWithFlags(&st.copied, hipEventDisableTiming));
+    size_t off[11];
+    const size_t need = stage_layout(n_orient, n_beams, n_vert, off);
+    if (st.cap < need) {
+        if (st.mem)
+            (void)hipHostFree(st.mem);
+        st.mem = nullptr;
+        st.cap = 0;
+        HIP_TRY(ctx, hipHostMalloc((void **)&st.mem, std::max<size_t>(need, 256)));
+        st.cap = std::max<size_t>(need, 256);
+    }
+    unsigned char *m = st.mem;
+    mbs_beam_staging &v = st.view;
+    v.n_orient = n_orient;
+    v.n_beams = n_beams;
+    v.n_vert = n_vert;
+    v.orient_offsets = (int64_t *)(m + off[0]);
+    v.weights = (double *)(m + off[1]);
+    v.beam_scalars = (double *)(m + off[2]);
+    v.beam_nv = (int32_t *)(m + off[3]);
+    v.beam_external = (int32_t *)(m + off[4]);
+    v.vert_offsets = (int64_t *)(m + off[5]);
+    v.vx = (double *)(m + off[6]);
+    v.vy = (double *)(m + off[7]);
+    v.slope_yx = (double *)(m + off[8]);
+    v.slope_xy = (double *)(m + off[9]);
+    v.edge_flags = (uint8_t *)(m + off[10]);
+    *out = v;
+    ctx->stage_cur = k;
+    ctx->stage_next = 1 - k;
+    return MBS_OK;
+}
+
+extern "C" int mbs_upload_staged(mbs_ctx *ctx, const mbs_consts *consts)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (ctx->stage_cur < 0)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_upload_staged: nothing staged (mbs_stage_beams)");
+    mbs_ctx::Staging &st = ctx->stage[ctx->stage_cur];
+    const mbs_beam_staging &v = st.view;
+    mbs_beam_batch bt;
+    bt.n_orient = v.n_orient;
+    bt.n_beams = v.n_beams;
+    bt.orient_offsets = v.orient_offsets;
+    bt.weights = v.weights;
+    bt.beam_scalars = v.beam_scalars;
+    bt.beam_nv = v.beam_nv;
+    bt.beam_external = v.beam_external;
+    bt.vert_offsets = v.vert_offsets;
+    bt.vx = v.vx;
+    bt.vy = v.vy;
+    bt.slope_yx = v.slope_yx;
+    bt.slope_xy = v.slope_xy;
+    bt.edge_flags = v.edge_flags;
+    if ((int64_t)(v.n_beams > 0 ? v.vert_offsets[v.n_beams] : 0) != v.n_vert)
+        return fail(ctx, MBS_ERR_INVALID, "staged vert_offsets end at %lld, staged for %lld",
+                    (long long)(v.n_beams > 0 ? v.vert_offsets[v.n_beams] : 0),
+                    (long long)v.n_vert);
+    ctx->stage_cur = -1;
+    const int rc = upload_impl(ctx, &bt, consts, st.copied);
+    if (rc == MBS_OK)
+        st.in_flight = true;
+    return rc;
 }
 
 extern "C" int mbs_upload_beam_scaling(mbs_ctx *ctx, const mbs_beam_scaling *sc)
@@ -602,6 +734,8 @@ extern "C" int mbs_rescale_beams(mbs_ctx *ctx, double scale)
 //    totals; each item keeps the weighted Mueller sum of its run.  Used when
 //    the partial Jones grids of segment mode would exceed the memory budget.
 static const size_t kPartialBudget = (size_t)8 << 30;  // bytes
+// Workgroups of a persistent (work-queue) launch: 256 CUs x 8 slots.
+static const int kQueueBlocks = 2048;
 
 static bool plan_items(mbs_ctx *ctx, int n_tiles, int target, int work_mode,
                        std::vector<WorkItem> &items)
@@ -648,7 +782,7 @@ static bool plan_items(mbs_ctx *ctx, int n_tiles, int target, int work_mode,
     return false;
 }
 
-template <int PREC, int NS, int JONES, int PER>
+template <int PREC, int NS, int JONES, int PER, int Q>
 static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
     if constexpr (PREC == 3)
@@ -656,33 +790,37 @@ static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
             a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.vtx, a.vdd, a.vflag, a.nv,
             a.voff, a.ooff, a.weights, a.k, a.eps1, a.out);
     else if constexpr (PREC == 2)
-        diffract_mfma_kernel<NS, JONES, PER><<<blocks, 256, 0, s>>>(
+        diffract_mfma_kernel<NS, JONES, PER, Q><<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
             a.vflag,
             a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
-            a.per_orient);
+            a.per_orient, a.queue);
     else
-        diffract_kernel<PREC, NS, JONES, PER><<<blocks, 256, 0, s>>>(
+        diffract_kernel<PREC, NS, JONES, PER, Q><<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd,
             a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
-            a.per_orient);
+            a.per_orient, a.queue);
 }
 
-static void launch_diffract(int prec, int ns, int jones, int per, const KernelArgs &a,
-                            int blocks, hipStream_t s)
+// queue: the persistent work-queue kernels (beam segments, no per-orientation
+// grids), which take their units from a.queue.
+static void launch_diffract(int prec, int ns, int jones, int per, bool queue,
+                            const KernelArgs &a, int blocks, hipStream_t s)
 {
-#define MBS_L(P, N, J, R)                                          \
-    if (prec == P && ns == N && jones == J && per == R) {         \
-        launch_one<P, N, J, R>(a, blocks, s);                      \
-        return;                                                    \
+#define MBS_L(P, N, J, R, Q)                                                 \
+    if (prec == P && ns == N && jones == J && per == R && (int)queue == Q) { \
+        launch_one<P, N, J, R, Q>(a, blocks, s);                             \
+        return;                                                              \
     }
-    MBS_L(0, 0, 0, 0) MBS_L(0, 1, 0, 0) MBS_L(0, 0, 1, 0) MBS_L(0, 1, 1, 0)
-    MBS_L(0, 0, 0, 1) MBS_L(0, 1, 0, 1)
-    MBS_L(1, 0, 0, 0) MBS_L(1, 1, 0, 0) MBS_L(1, 0, 1, 0) MBS_L(1, 1, 1, 0)
-    MBS_L(1, 0, 0, 1) MBS_L(1, 1, 0, 1)
-    MBS_L(2, 0, 0, 0) MBS_L(2, 1, 0, 0) MBS_L(2, 0, 1, 0) MBS_L(2, 1, 1, 0)
-    MBS_L(2, 0, 0, 1) MBS_L(2, 1, 0, 1)
-    MBS_L(3, 0, 0, 0)
+    MBS_L(0, 0, 0, 0, 0) MBS_L(0, 1, 0, 0, 0) MBS_L(0, 0, 1, 0, 0) MBS_L(0, 1, 1, 0, 0)
+    MBS_L(0, 0, 0, 1, 0) MBS_L(0, 1, 0, 1, 0)
+    MBS_L(1, 0, 0, 0, 0) MBS_L(1, 1, 0, 0, 0) MBS_L(1, 0, 1, 0, 0) MBS_L(1, 1, 1, 0, 0)
+    MBS_L(1, 0, 0, 1, 0) MBS_L(1, 1, 0, 1, 0)
+    MBS_L(2, 0, 0, 0, 0) MBS_L(2, 1, 0, 0, 0) MBS_L(2, 0, 1, 0, 0) MBS_L(2, 1, 1, 0, 0)
+    MBS_L(2, 0, 0, 1, 0) MBS_L(2, 1, 0, 1, 0)
+    MBS_L(3, 0, 0, 0, 0)
+    MBS_L(0, 0, 1, 0, 1) MBS_L(0, 1, 1, 0, 1) MBS_L(1, 0, 1, 0, 1) MBS_L(1, 1, 1, 0, 1)
+    MBS_L(2, 0, 1, 0, 1) MBS_L(2, 1, 1, 0, 1)
 #undef MBS_L
 }
 
@@ -699,20 +837,30 @@ static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, bool jones, int per
     hipStream_t s = ctx->stream;
     const int cells = ctx->cells;
     const size_t mcount = (size_t)cells * 16;
+    const bool queue = a.queue != nullptr;
     if (events)
         HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-    launch_diffract(prec, ns, jones ? 1 : 0, per_k, a, blocks, s);
+    if (queue) {
+        // the segment plan and the queue header, made on the device (timed
+        // with the main kernel)
+        plan_segments_kernel<<<1, 1024, 0, s>>>(a.ooff, ctx->n_orient, ctx->queue_want,
+                                                a.n_tiles, const_cast<WorkItem *>(a.items),
+                                                a.queue);
+        HIP_TRY(ctx, hipGetLastError());
+    }
+    launch_diffract(prec, ns, jones ? 1 : 0, per_k, queue, a, blocks, s);
     HIP_TRY(ctx, hipGetLastError());
     if (events)
         HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     const int rt = 256;
     if (jones) {
         jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-            a.out, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res, per_res);
+            a.out, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res, per_res, a.queue);
         HIP_TRY(ctx, hipGetLastError());
         if (ns) {
             jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-                a.out_ns, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res_ns, nullptr);
+                a.out_ns, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res_ns, nullptr,
+                a.queue);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {
@@ -787,12 +935,25 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
         return rc;
     const int cells = ctx->cells;
     const int n_tiles = (cells + 255) / 256;
-    // the incoherent sum keeps per-item Mueller sums (orientation items)
-    const bool jones = plan_items(ctx, n_tiles, tn.valid ? tn.target_blocks : ctx->target_blocks,
-                                  incoh ? MBS_WORK_ORIENTATIONS
-                                        : tn.valid ? tn.work_mode : ctx->work_mode,
-                                  ctx->h_items);
-    const int n_items = (int)ctx->h_items.size();
+    const int target = tn.valid ? tn.target_blocks : ctx->target_blocks;
+    const int mode = tn.valid ? tn.work_mode : ctx->work_mode;
+    // the device queue plans beam segments on the device; the incoherent sum
+    // and per-orientation grids keep the host plan
+    const bool queue = mode == MBS_WORK_QUEUE && !incoh && !per && ctx->n_beams > 0;
+    bool jones = true;
+    int n_items;
+    if (queue) {
+        ctx->queue_want = (int)std::max<int64_t>(1, (target + n_tiles - 1) / n_tiles);
+        n_items = ctx->queue_want + ctx->n_orient;  // the most the device plan makes
+        ctx->h_items.clear();
+    } else {
+        // the incoherent sum keeps per-item Mueller sums (orientation items)
+        jones = plan_items(ctx, n_tiles, target,
+                           incoh ? MBS_WORK_ORIENTATIONS
+                                 : mode == MBS_WORK_QUEUE ? MBS_WORK_SEGMENTS : mode,
+                           ctx->h_items);
+        n_items = (int)ctx->h_items.size();
+    }
     const size_t mcount = (size_t)cells * 16;
     HIP_TRY(ctx, ctx->res.ensure(mcount));
     if (ns)
@@ -816,18 +977,25 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     if (ns)
         HIP_TRY(ctx, ctx->part_ns.ensure((size_t)n_items * pstride));
     HIP_TRY(ctx, ctx->items.ensure(n_items));
-    HIP_TRY(ctx, hipMemcpyAsync(ctx->items.p, ctx->h_items.data(), sizeof(WorkItem) * n_items,
-                                hipMemcpyHostToDevice, s));
+    if (queue)
+        HIP_TRY(ctx, ctx->queue.ensure(Q_WORDS));
+    else
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->items.p, ctx->h_items.data(),
+                                    sizeof(WorkItem) * n_items, hipMemcpyHostToDevice, s));
     KernelArgs a;
     batch_args(ctx, prec, n_items, n_tiles, a);
     a.items = ctx->items.p;
     a.out = ctx->part.p;
     a.out_ns = ns ? ctx->part_ns.p : nullptr;
     a.per_orient = (per && !jones) ? ctx->per_orient.p : nullptr;
+    a.queue = queue ? ctx->queue.p : nullptr;
     const int64_t total = (int64_t)n_items * n_tiles;
     if (total > (int64_t)1 << 30)
         return fail(ctx, MBS_ERR_CAPACITY, "launch of %lld workgroups", (long long)total);
-    const int blocks = (int)((total + 7) / 8 * 8);
+    // the queue kernels are persistent: workgroups enough to fill every
+    // CU's slots, each taking units until the queue is empty
+    const int blocks = queue ? (int)std::min<int64_t>((total + 7) / 8 * 8, kQueueBlocks)
+                             : (int)((total + 7) / 8 * 8);
     ctx->last_items = n_items;
     ctx->last_blocks = blocks;
     const int per_k = (per && !jones) ? 1 : 0;
@@ -862,7 +1030,9 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
                 return fail(ctx, MBS_ERR_DEVICE, "graph instantiate: %s", hipGetErrorString(ce));
             }
             ctx->graph_key.swap(key);
+            ++ctx->graph_captures;
         }
+        ++ctx->graph_replays;
         if (ctx->timing)
             HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
         HIP_TRY(ctx, hipGraphLaunch(ctx->graph_exec, s));
@@ -1445,8 +1615,8 @@ int mbs_batch_stats(mbs_ctx *ctx, int64_t *n_beams, double *units, int64_t *nv_h
 
 int mbs_set_work_mode(mbs_ctx *ctx, int mode)
 {
-    if (!ctx || mode < 0 || mode > 2)
-        return fail(ctx, MBS_ERR_INVALID, "work mode must be 0, 1 or 2");
+    if (!ctx || mode < 0 || mode > 3)
+        return fail(ctx, MBS_ERR_INVALID, "work mode must be 0, 1, 2 or 3");
     ctx->work_mode = mode;
     ctx->tuned[0].valid = ctx->tuned[1].valid = false;
     return MBS_OK;
@@ -1476,6 +1646,17 @@ int mbs_set_graph(mbs_ctx *ctx, int enable)
     return MBS_OK;
 }
 
+int mbs_graph_stats(mbs_ctx *ctx, int64_t *captures, int64_t *launches)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    if (captures)
+        *captures = ctx->graph_captures;
+    if (launches)
+        *launches = ctx->graph_replays;
+    return MBS_OK;
+}
+
 int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t max_entries,
                  int32_t *n_entries)
 {
@@ -1497,11 +1678,16 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
     mbs_ctx::Tuned best;
     float best_ms = 0.0f;
     for (int mfma = (base ? 1 : 0); mfma >= 0; --mfma)
-        for (int mode = 2; mode >= 1; --mode)
+        for (int mode = 3; mode >= 1; --mode)
             for (int tb : kBlocks) {
                 std::vector<WorkItem> items;
-                const bool jones = plan_items(ctx, n_tiles, tb, mode, items);
-                if (jones && items.size() * (size_t)ctx->cells * 8 * sizeof(double) > kPartialBudget)
+                const bool jones = plan_items(ctx, n_tiles, tb,
+                                              mode == MBS_WORK_QUEUE ? MBS_WORK_SEGMENTS : mode,
+                                              items);
+                const size_t n_it = mode == MBS_WORK_QUEUE
+                                        ? (size_t)((tb + n_tiles - 1) / n_tiles + ctx->n_orient)
+                                        : items.size();
+                if (jones && n_it * (size_t)ctx->cells * 8 * sizeof(double) > kPartialBudget)
                     continue;
                 mbs_ctx::Tuned t;
                 t.valid = true;

@@ -939,6 +939,103 @@ __device__ __forceinline__ bool map_block(int n_items, int n_tiles, int &item,
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Device work queue (work mode MBS_WORK_QUEUE).  plan_segments_kernel cuts
+// every orientation into beam segments on the device (the host plan's rule:
+// segments of max(32, ceil(n_beams / want)) beams, at least one per
+// orientation) and splits the (item, tile) units into 8 contiguous ranges,
+// one per XCD, so the units of one item stay in one XCD's L2.  A persistent
+// workgroup takes units from its XCD's range with an atomic counter and,
+// when that range is empty, from the next XCDs' in turn; it exits when all
+// eight are empty.  Each unit still writes its own partial sums, so results
+// do not depend on which workgroup ran it.
+// ---------------------------------------------------------------------------
+struct QueueCursor {
+    int range = -1;   // XCD range being drained (thread 0 only)
+    int tried = 0;    // ranges found empty so far
+};
+
+__device__ __forceinline__ bool next_unit(int *__restrict__ queue, int n_tiles, QueueCursor &qc,
+                                          int &s_unit, int &item, int &tile)
+{
+    if (threadIdx.x == 0) {
+        if (qc.range < 0)
+            qc.range = blockIdx.x & 7;
+        int u = -1;
+        while (qc.tried < 8) {
+            const int t = atomicAdd(queue + Q_NEXT + qc.range, 1);
+            if (t < queue[Q_END + qc.range]) {
+                u = t;
+                break;
+            }
+            qc.range = (qc.range + 1) & 7;
+            ++qc.tried;
+        }
+        s_unit = u;
+    }
+    __syncthreads();
+    const int u = s_unit;
+    __syncthreads();
+    if (u < 0)
+        return false;
+    item = u / n_tiles;
+    tile = u - item * n_tiles;
+    return true;
+}
+
+// One workgroup: the segment plan of the resident batch, and the queue
+// header.  ooff [n_orient + 1] device orientation offsets; want: the target
+// item count; items has room for want + n_orient items (the plan never
+// needs more: sum over o of ceil(n_o / seg) <= n_beams / seg + n_orient).
+__global__ void __launch_bounds__(1024) plan_segments_kernel(
+    const int64_t *__restrict__ ooff, int n_orient, int want, int n_tiles,
+    WorkItem *__restrict__ items, int *__restrict__ queue)
+{
+    __shared__ int s_scan[1024];
+    __shared__ int s_base;
+    const int64_t nb = ooff[n_orient];
+    const int64_t seg =
+        max((int64_t)32, (nb + (int64_t)want - 1) / max((int64_t)want, (int64_t)1));
+    if (threadIdx.x == 0)
+        s_base = 0;
+    __syncthreads();
+    for (int o0 = 0; o0 < n_orient; o0 += blockDim.x) {
+        const int o = o0 + threadIdx.x;
+        int64_t b0 = 0, n = 0;
+        int kseg = 0;
+        if (o < n_orient) {
+            b0 = ooff[o];
+            n = ooff[o + 1] - b0;
+            kseg = (int)max((int64_t)1, (n + seg - 1) / seg);
+        }
+        // inclusive scan of the segment counts over this chunk
+        s_scan[threadIdx.x] = kseg;
+        __syncthreads();
+        for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+            const int v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int first = s_base + s_scan[threadIdx.x] - kseg;
+        for (int i = 0; i < kseg; ++i)
+            items[first + i] = WorkItem{o, o + 1, b0 + n * i / kseg, b0 + n * (i + 1) / kseg};
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1)
+            s_base += s_scan[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int total = s_base;
+        const int64_t units = (int64_t)total * n_tiles;
+        queue[Q_ITEMS] = total;
+        for (int x = 0; x < 8; ++x) {
+            queue[Q_NEXT + x] = (int)(units * x / 8);
+            queue[Q_END + x] = (int)(units * (x + 1) / 8);
+        }
+    }
+}
+
 struct KernelArgs {
     const WorkItem *items;
     int n_items, n_tiles, cells, n_az;
@@ -958,6 +1055,7 @@ struct KernelArgs {
     double *out;             // Mueller mode: [n_items][cells][16]; Jones: [n_items][cells][8]
     double *out_ns;          // same shape, no-shadow (NS only)
     double *per_orient;      // [n_orient][cells][16] (PER only, Mueller mode)
+    int *queue;              // work-queue header (queue mode), else null
 };
 
 // Mueller of one orientation's Jones sum, weighted, stored to (first) or
@@ -987,7 +1085,7 @@ __device__ __forceinline__ void flush_mueller(const T *j, double w, double *dst,
 // All pointers are separate __restrict__ kernel arguments: the per-orientation
 // stores into `out` must be provably disjoint from the beam records for the
 // compiler to keep reading those through the scalar cache.
-template <int PREC, int NS, int JONES, int PER>
+template <int PREC, int NS, int JONES, int PER, int QUEUE>
 __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,
@@ -997,17 +1095,8 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     const int64_t *__restrict__ voff, const int32_t *__restrict__ ext,
     const int64_t *__restrict__ ooff, const double *__restrict__ weights, double k,
     double eps1, double *__restrict__ out, double *__restrict__ out_ns,
-    double *__restrict__ per_orient)
+    double *__restrict__ per_orient, int *__restrict__ queue)
 {
-    int item, tile;
-    if (!map_block(n_items, n_tiles, item, tile))
-        return;
-    const int cell = tile * 256 + threadIdx.x;
-    const bool live = cell < cells;
-    const int c = live ? cell : cells - 1;  // dead lanes compute, never store
-    double d[3];
-    for (int q = 0; q < 3; ++q)
-        d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
     using U = typename std::conditional<PREC == 1, float, double>::type;
     using U2 = typename std::conditional<PREC == 1, float2, double2>::type;
     // LDS: this lane's rotation vectors u0, u1 ([12][256], read once per beam
@@ -1016,8 +1105,12 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     constexpr int TN = PREC == 1 ? kTabN32 : kTabN;
     __shared__ U s_u[12 * 256];
     __shared__ U2 s_tab[TN];
-    for (int q = 0; q < 12; ++q)
-        s_u[q * 256 + threadIdx.x] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
+    __shared__ int s_unit;
+    int item = 0, tile = 0;
+    if constexpr (!QUEUE) {
+        if (!map_block(n_items, n_tiles, item, tile))
+            return;
+    }
     for (int i = threadIdx.x; i < TN; i += blockDim.x) {
         double ts, tc;
         sincospi((double)i / (TN / 2), &ts, &tc);
@@ -1026,6 +1119,20 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
         else
             s_tab[i] = make_double2(tc, ts);
     }
+    QueueCursor qc;
+    for (;;) {
+    if constexpr (QUEUE) {
+        if (!next_unit(queue, n_tiles, qc, s_unit, item, tile))
+            return;
+    }
+    const int cell = tile * 256 + threadIdx.x;
+    const bool live = cell < cells;
+    const int c = live ? cell : cells - 1;  // dead lanes compute, never store
+    double d[3];
+    for (int q = 0; q < 3; ++q)
+        d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
+    for (int q = 0; q < 12; ++q)
+        s_u[q * 256 + threadIdx.x] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
     __syncthreads();
     const WorkItem it = items[item];
     for (int o = it.o_begin; o < it.o_end; ++o) {
@@ -1087,6 +1194,10 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
                 flush_mueller(jn, w, out_ns + ((size_t)item * cells + cell) * 16, first,
                               (double *)nullptr);
         }
+    }
+    if constexpr (!QUEUE)
+        return;
+    __syncthreads();  // the next unit overwrites s_u
     }
 }
 
@@ -1201,7 +1312,7 @@ __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1
     }
 }
 
-template <int NS, int JONES, int PER>
+template <int NS, int JONES, int PER, int QUEUE>
 __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,
@@ -1210,27 +1321,36 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const int32_t *__restrict__ nv, const int64_t *__restrict__ voff,
     const int32_t *__restrict__ ext, const int64_t *__restrict__ ooff,
     const double *__restrict__ weights, double k, double eps1, double *__restrict__ out,
-    double *__restrict__ out_ns, double *__restrict__ per_orient)
+    double *__restrict__ out_ns, double *__restrict__ per_orient, int *__restrict__ queue)
 {
-    int item, tile;
-    if (!map_block(n_items, n_tiles, item, tile))
-        return;
+    __shared__ float2 s_tab[kTabN32];
+    __shared__ float s_g[4 * 32 * G_LD];
+    __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
+    __shared__ int s_unit;
+    int item = 0, tile = 0;
+    if constexpr (!QUEUE) {
+        if (!map_block(n_items, n_tiles, item, tile))
+            return;
+    }
+    for (int i = threadIdx.x; i < kTabN32; i += blockDim.x) {
+        double ts, tc;
+        sincospi((double)i / (kTabN32 / 2), &ts, &tc);
+        s_tab[i] = make_float2((float)tc, (float)ts);
+    }
+    QueueCursor qc;
+    for (;;) {
+    if constexpr (QUEUE) {
+        if (!next_unit(queue, n_tiles, qc, s_unit, item, tile))
+            return;
+    }
     const int cell = tile * 256 + threadIdx.x;
     const bool live = cell < cells;
     const int c = live ? cell : cells - 1;  // dead lanes compute, never store
     double d[3];
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
-    __shared__ float2 s_tab[kTabN32];
-    __shared__ float s_g[4 * 32 * G_LD];
-    __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
     for (int q = 0; q < 12; ++q)
         s_u[q * 256 + threadIdx.x] = (float)tab[(size_t)(DT_U0 + q) * cells + c];
-    for (int i = threadIdx.x; i < kTabN32; i += blockDim.x) {
-        double ts, tc;
-        sincospi((double)i / (kTabN32 / 2), &ts, &tc);
-        s_tab[i] = make_float2((float)tc, (float)ts);
-    }
     __syncthreads();
     float *sg = s_g + (threadIdx.x >> 6) * 32 * G_LD;
     const int lane = threadIdx.x & 63;
@@ -1296,6 +1416,10 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
                               (double *)nullptr);
         }
     }
+    if constexpr (!QUEUE)
+        return;
+    __syncthreads();  // the next unit overwrites s_u and s_g
+    }
 }
 
 // Zero fill (used instead of a memset so that no memset node is ever recorded
@@ -1311,6 +1435,7 @@ __global__ void zero_kernel(double *__restrict__ p, size_t n)
 __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_items,
                                       int cells, int n_az, double *__restrict__ res)
 {
+    // (Mueller-mode items are always planned on the host)
     const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = (size_t)cells * 16;
     if (idx >= n)
@@ -1329,11 +1454,14 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
                                         int n_items, int cells, int n_az,
                                         const double *__restrict__ weights,
                                         double *__restrict__ res,
-                                        double *__restrict__ per_orient)
+                                        double *__restrict__ per_orient,
+                                        const int *__restrict__ queue)
 {
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= cells)
         return;
+    if (queue)  // planned on the device
+        n_items = queue[Q_ITEMS];
     const size_t out_cell = abi_cell(cell, n_az, cells);
     double m[16];
     for (int q = 0; q < 16; ++q)

@@ -62,4 +62,10 @@ struct WorkItem {
     int64_t b_end;
 };
 
+// Header of the device work queue (work mode MBS_WORK_QUEUE), written by
+// plan_segments_kernel: the item count, then for each of the 8 XCDs the next
+// unit to hand out and the end of its range of units (unit = item * n_tiles
+// + tile).
+enum { Q_ITEMS = 0, Q_NEXT = 1, Q_END = 9, Q_WORDS = 17 };
+
 }  // namespace mbs

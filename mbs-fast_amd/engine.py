@@ -52,6 +52,11 @@ def lib():
         L.mbs_upload_beams.argtypes = [vp, C.POINTER(abi.mbs_beam_batch),
                                        C.POINTER(abi.mbs_consts)]
         L.mbs_upload_beams.restype = C.c_int
+        L.mbs_stage_beams.argtypes = [vp, C.c_int32, C.c_int32, C.c_int64,
+                                      C.POINTER(abi.mbs_beam_staging)]
+        L.mbs_stage_beams.restype = C.c_int
+        L.mbs_upload_staged.argtypes = [vp, C.POINTER(abi.mbs_consts)]
+        L.mbs_upload_staged.restype = C.c_int
         L.mbs_upload_beam_scaling.argtypes = [vp, C.POINTER(abi.mbs_beam_scaling)]
         L.mbs_upload_beam_scaling.restype = C.c_int
         L.mbs_rescale_beams.argtypes = [vp, C.c_double]
@@ -82,6 +87,8 @@ def lib():
         L.mbs_autotune.argtypes = [vp, C.c_uint32, C.POINTER(abi.mbs_tune_entry), C.c_int32,
                                    C.POINTER(C.c_int32)]
         L.mbs_autotune.restype = C.c_int
+        L.mbs_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.mbs_graph_stats.restype = C.c_int
         L.mbs_set_graph.argtypes = [vp, C.c_int]
         L.mbs_set_graph.restype = C.c_int
         fp = C.POINTER(C.c_float)
@@ -191,6 +198,30 @@ class Context:
 
     def upload(self, batch, consts):
         self._check(self._L.mbs_upload_beams(self._h, C.byref(batch.struct()), C.byref(consts)))
+        self.batch = batch
+
+    def upload_staged(self, batch, consts):
+        """The batch through the context's page-locked staging
+        (mbs_stage_beams, filled here, then mbs_upload_staged): the copies
+        are asynchronous and the call returns once they are queued."""
+        st = abi.mbs_beam_staging()
+        nv = int(batch.vert_offsets[-1])
+        self._check(self._L.mbs_stage_beams(self._h, batch.n_orient, batch.n_beams, nv,
+                                            C.byref(st)))
+        for name, a, n in (("orient_offsets", batch.orient_offsets, batch.n_orient + 1),
+                           ("weights", batch.weights, batch.n_orient),
+                           ("beam_scalars", batch.beam_scalars.reshape(-1),
+                            batch.n_beams * abi.BEAM_SCALARS),
+                           ("beam_nv", batch.beam_nv, batch.n_beams),
+                           ("beam_external", batch.beam_external, batch.n_beams),
+                           ("vert_offsets", batch.vert_offsets, batch.n_beams + 1),
+                           ("vx", batch.vx, nv), ("vy", batch.vy, nv),
+                           ("slope_yx", batch.slope_yx, nv), ("slope_xy", batch.slope_xy, nv),
+                           ("edge_flags", batch.edge_flags, nv)):
+            if n:
+                dst = np.ctypeslib.as_array(getattr(st, name), shape=(n,))
+                dst[:] = a
+        self._check(self._L.mbs_upload_staged(self._h, C.byref(consts)))
         self.batch = batch
 
     def upload_scaling(self, raw_jones, path, sign, abs_offsets, abs_paths, c_abs, wave_index):
@@ -317,7 +348,8 @@ class Context:
         self._check(self._L.mbs_set_target_blocks(self._h, int(n)))
 
     def set_work_mode(self, mode):
-        """0 automatic, 1 whole orientations per item, 2 beam segments."""
+        """0 automatic, 1 whole orientations per item, 2 beam segments, 3 beam
+        segments planned on the device and run from a device work queue."""
         self._check(self._L.mbs_set_work_mode(self._h, int(mode)))
 
     def autotune(self, flags=0):
@@ -330,6 +362,12 @@ class Context:
 
     def set_graph(self, enable=True):
         self._check(self._L.mbs_set_graph(self._h, 1 if enable else 0))
+
+    def graph_stats(self):
+        """(captures, launches) of graph replay on this context."""
+        c, l = C.c_int64(), C.c_int64()
+        self._check(self._L.mbs_graph_stats(self._h, C.byref(c), C.byref(l)))
+        return c.value, l.value
 
     def set_timing(self, enable=True):
         self._check(self._L.mbs_set_timing(self._h, 1 if enable else 0))

@@ -209,16 +209,34 @@ void ScaledJones(const PreparedBeam &pb, const PackScale &sc, double jp[8])
 }
 
 // The job of pack_prepared_gpu_beam (src/cuda/GpuDiffraction.cu:273-358):
-// PreparedBeam (src/handler/HandlerPO.h:14-46) -> structure of arrays.
-// Two passes over the orientations, both OpenMP-parallel as the reference's
-// packing is (:4407-4428): beam and vertex counts (which also validate every
-// beam), then, after a prefix sum, every orientation fills its own slice.
-// Returns false for a beam the device path cannot take.  With `scaling`
-// the per-beam data of a device rescale (mbs_beam_scaling) are packed too:
-// the raw Jones matrix, the path and sign of ScaledJones, the absorption
-// paths.
-bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
-          const PackScale &sc, PackedBatch &pk, bool scaling = false)
+// PreparedBeam (src/handler/HandlerPO.h:14-46) -> structure of arrays, in
+// two OpenMP-parallel passes as the reference's packing is (:4407-4428):
+// Count (beam, vertex and absorption-path counts per orientation, which
+// also validates every beam), then, after prefix sums, Fill (every
+// orientation writes its own slice of the output arrays).
+struct PackCounts
+{
+    std::vector<int64_t> beamBase, vertBase, absBase;  // [nOrient + 1] prefix sums
+    int64_t Beams() const { return beamBase.back(); }
+    int64_t Verts() const { return vertBase.back(); }
+};
+
+// Where Fill writes: the arrays of an mbs_beam_batch (and, for a device
+// rescale, of mbs_beam_scaling), for orientations [0, nOrient).
+struct PackTarget
+{
+    int64_t *orientOffsets, *vertOffsets;
+    double *weights, *scalars, *vx, *vy, *syx, *sxy;
+    int32_t *nv, *external;
+    uint8_t *flags;
+    // optional (null when not wanted): the per-beam data of a device rescale
+    double *rawJones, *path, *sign, *absPaths;
+    int64_t *absOffsets;
+};
+
+// Returns false for a beam the device path cannot take.
+bool Count(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
+           PackCounts &pc)
 {
     std::vector<int64_t> nb(nOrient), nvert(nOrient), nabs(nOrient);
     int bad = 0;
@@ -241,46 +259,37 @@ bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrie
     }
     if (bad)
         return false;
-    pk.orientOffsets.assign(nOrient + 1, 0);
-    std::vector<int64_t> vbase(nOrient + 1, 0), abase(nOrient + 1, 0);
+    pc.beamBase.assign(nOrient + 1, 0);
+    pc.vertBase.assign(nOrient + 1, 0);
+    pc.absBase.assign(nOrient + 1, 0);
     for (int i = 0; i < nOrient; ++i)
     {
-        pk.orientOffsets[i + 1] = pk.orientOffsets[i] + nb[i];
-        vbase[i + 1] = vbase[i] + nvert[i];
-        abase[i + 1] = abase[i] + nabs[i];
+        pc.beamBase[i + 1] = pc.beamBase[i] + nb[i];
+        pc.vertBase[i + 1] = pc.vertBase[i] + nvert[i];
+        pc.absBase[i + 1] = pc.absBase[i] + nabs[i];
     }
+    return true;
+}
+
+// Fills t for prepared[start, start + nOrient), whose counts are pc.
+void Fill(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
+          const PackCounts &pc, const PackScale &sc, const PackTarget &t)
+{
+    const bool scaling = t.rawJones != nullptr;
+    for (int i = 0; i <= nOrient; ++i)
+        t.orientOffsets[i] = pc.beamBase[i];
+    t.vertOffsets[pc.Beams()] = pc.Verts();
     if (scaling)
-    {
-        const size_t n = (size_t)pk.orientOffsets[nOrient];
-        pk.rawJones.resize(n * 8);
-        pk.path.resize(n);
-        pk.sign.resize(n);
-        pk.absOffsets.resize(n + 1);
-        pk.absOffsets[n] = abase[nOrient];
-        pk.absPaths.resize((size_t)abase[nOrient]);
-    }
-    const size_t nBeams = (size_t)pk.orientOffsets[nOrient];
-    const size_t nVerts = (size_t)vbase[nOrient];
-    pk.weights.resize(nOrient);
-    pk.scalars.resize(nBeams * MBS_BEAM_SCALARS);
-    pk.nv.resize(nBeams);
-    pk.external.resize(nBeams);
-    pk.vertOffsets.resize(nBeams + 1);
-    pk.vx.resize(nVerts);
-    pk.vy.resize(nVerts);
-    pk.syx.resize(nVerts);
-    pk.sxy.resize(nVerts);
-    pk.flags.resize(nVerts);
-    pk.vertOffsets[nBeams] = (int64_t)nVerts;
+        t.absOffsets[pc.Beams()] = pc.absBase[nOrient];
     const double g = sc.on ? sc.scale : 1.0, g2 = g * g;
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < nOrient; ++i)
     {
         const PreparedOrientation &po = prepared[start + i];
-        pk.weights[i] = po.sinZenith;
-        size_t bi = (size_t)pk.orientOffsets[i];
-        size_t vi = (size_t)vbase[i];
-        size_t ai = (size_t)abase[i];
+        t.weights[i] = po.sinZenith;
+        size_t bi = (size_t)pc.beamBase[i];
+        size_t vi = (size_t)pc.vertBase[i];
+        size_t ai = (size_t)pc.absBase[i];
         for (const PreparedBeam &pb : po.beams)
         {
             const BeamEdgeData &ed = pb.edgeData;
@@ -290,17 +299,17 @@ bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrie
                                           pb.origBeam.J.m21, pb.origBeam.J.m22};
                 for (int k = 0; k < 4; ++k)
                 {
-                    pk.rawJones[bi * 8 + 2 * k] = real(raw[k]);
-                    pk.rawJones[bi * 8 + 2 * k + 1] = imag(raw[k]);
+                    t.rawJones[bi * 8 + 2 * k] = real(raw[k]);
+                    t.rawJones[bi * 8 + 2 * k + 1] = imag(raw[k]);
                 }
-                pk.path[bi] = pb.isExternal ? pb.origBeam.opticalPath : pb.info.projLenght;
+                t.path[bi] = pb.isExternal ? pb.origBeam.opticalPath : pb.info.projLenght;
                 double sg = pb.isExternal ? -1.0 : 1.0;
                 if (!pb.isExternal && (pb.origBeam.nActs & 1))
                     sg = -sg;
-                pk.sign[bi] = sg;
-                pk.absOffsets[bi] = (int64_t)ai;
+                t.sign[bi] = sg;
+                t.absOffsets[bi] = (int64_t)ai;
                 for (double p : pb.absorptionPaths)
-                    pk.absPaths[ai++] = p;
+                    t.absPaths[ai++] = p;
             }
             double jp[8] = {pb.jp00r, pb.jp00i, pb.jp01r, pb.jp01i,
                             pb.jp10r, pb.jp10i, pb.jp11r, pb.jp11i};
@@ -314,22 +323,62 @@ bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrie
                 pb.pnxDTx, pb.pnxDTy, pb.pnxDTz, pb.pnxDPx, pb.pnxDPy, pb.pnxDPz,
                 jp[0], jp[1], jp[2], jp[3], jp[4], jp[5], jp[6], jp[7],
                 pb.normx, pb.normy, pb.normz};
-            std::memcpy(&pk.scalars[bi * MBS_BEAM_SCALARS], s, sizeof s);
-            pk.nv[bi] = ed.nVertices;
-            pk.external[bi] = pb.isExternal ? 1 : 0;
-            pk.vertOffsets[bi] = (int64_t)vi;
+            std::memcpy(&t.scalars[bi * MBS_BEAM_SCALARS], s, sizeof s);
+            t.nv[bi] = ed.nVertices;
+            t.external[bi] = pb.isExternal ? 1 : 0;
+            t.vertOffsets[bi] = (int64_t)vi;
             for (int v = 0; v < ed.nVertices; ++v, ++vi)
             {
-                pk.vx[vi] = ed.x[v] * g;
-                pk.vy[vi] = ed.y[v] * g;
-                pk.syx[vi] = ed.edge_valid_x[v] ? ed.slope_yx[v] : 0.0;
-                pk.sxy[vi] = ed.edge_valid_y[v] ? ed.slope_xy[v] : 0.0;
-                pk.flags[vi] = (uint8_t)((ed.edge_valid_x[v] ? MBS_EDGE_VALID_X : 0)
-                                         | (ed.edge_valid_y[v] ? MBS_EDGE_VALID_Y : 0));
+                t.vx[vi] = ed.x[v] * g;
+                t.vy[vi] = ed.y[v] * g;
+                t.syx[vi] = ed.edge_valid_x[v] ? ed.slope_yx[v] : 0.0;
+                t.sxy[vi] = ed.edge_valid_y[v] ? ed.slope_xy[v] : 0.0;
+                t.flags[vi] = (uint8_t)((ed.edge_valid_x[v] ? MBS_EDGE_VALID_X : 0)
+                                        | (ed.edge_valid_y[v] ? MBS_EDGE_VALID_Y : 0));
             }
             ++bi;
         }
     }
+}
+
+// Count and Fill into pk's host vectors.  With `scaling` the per-beam data
+// of a device rescale (mbs_beam_scaling) are packed too.
+bool Pack(const std::vector<PreparedOrientation> &prepared, int start, int nOrient,
+          const PackScale &sc, PackedBatch &pk, bool scaling = false)
+{
+    PackCounts pc;
+    if (!Count(prepared, start, nOrient, pc))
+        return false;
+    const size_t nBeams = (size_t)pc.Beams(), nVerts = (size_t)pc.Verts();
+    pk.orientOffsets.resize(nOrient + 1);
+    pk.weights.resize(nOrient);
+    pk.scalars.resize(nBeams * MBS_BEAM_SCALARS);
+    pk.nv.resize(nBeams);
+    pk.external.resize(nBeams);
+    pk.vertOffsets.resize(nBeams + 1);
+    pk.vx.resize(nVerts);
+    pk.vy.resize(nVerts);
+    pk.syx.resize(nVerts);
+    pk.sxy.resize(nVerts);
+    pk.flags.resize(nVerts);
+    PackTarget t = {pk.orientOffsets.data(), pk.vertOffsets.data(), pk.weights.data(),
+                    pk.scalars.data(), pk.vx.data(), pk.vy.data(), pk.syx.data(),
+                    pk.sxy.data(), pk.nv.data(), pk.external.data(), pk.flags.data(),
+                    nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (scaling)
+    {
+        pk.rawJones.resize(nBeams * 8);
+        pk.path.resize(nBeams);
+        pk.sign.resize(nBeams);
+        pk.absOffsets.resize(nBeams + 1);
+        pk.absPaths.resize((size_t)pc.absBase[nOrient]);
+        t.rawJones = pk.rawJones.data();
+        t.path = pk.path.data();
+        t.sign = pk.sign.data();
+        t.absOffsets = pk.absOffsets.data();
+        t.absPaths = pk.absPaths.data();
+    }
+    Fill(prepared, start, nOrient, pc, sc, t);
     return true;
 }
 }  // namespace
@@ -547,6 +596,70 @@ bool RunBatch(ThreadContexts::Entry &e, const PackedBatch &pk, const mbs_consts 
     return DiffractBatch(e, pk, c, localMns != nullptr, packMs, upload)
            && AddResult(e, localM, localMns);
 }
+// Beams per part of a pipelined batch: a batch of more is cut into parts
+// of whole orientations, so the host fills part i + 1 into page-locked
+// staging while the device copies and diffracts part i.
+const int64_t kPartBeams = 65536;
+
+// Diffracts prepared[start, start + nOrient) (counts pc) in parts through
+// the engine's page-locked staging (mbs_stage_beams / mbs_upload_staged:
+// asynchronous DMA) and adds every part's result into localM (*localMns).
+bool RunStaged(ThreadContexts::Entry &e, const std::vector<PreparedOrientation> &prepared,
+               int start, int nOrient, const PackCounts &pc, const PackScale &sc,
+               const mbs_consts &c, Arr2D &localM, Arr2D *localMns)
+{
+    const bool timing = EnvIs("MBS_GPU_TIMING", "1");
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<int> cuts(1, 0);
+    for (int o = 1; o <= nOrient; ++o)
+        if (o == nOrient || pc.beamBase[o] - pc.beamBase[cuts.back()] >= kPartBeams)
+            cuts.push_back(o);
+    double fillMs = 0.0;
+    for (size_t part = 0; part + 1 < cuts.size(); ++part)
+    {
+        const int o0 = cuts[part], n = cuts[part + 1] - o0;
+        PackCounts sub;
+        sub.beamBase.resize(n + 1);
+        sub.vertBase.resize(n + 1);
+        sub.absBase.resize(n + 1);
+        for (int i = 0; i <= n; ++i)
+        {
+            sub.beamBase[i] = pc.beamBase[o0 + i] - pc.beamBase[o0];
+            sub.vertBase[i] = pc.vertBase[o0 + i] - pc.vertBase[o0];
+            sub.absBase[i] = pc.absBase[o0 + i] - pc.absBase[o0];
+        }
+        mbs_beam_staging st;
+        if (!Status(mbs_stage_beams(e.ctx, n, (int32_t)sub.Beams(), sub.Verts(), &st), "staging",
+                    e.ctx))
+            return false;
+        const auto tf = std::chrono::steady_clock::now();
+        const PackTarget t = {st.orient_offsets, st.vert_offsets, st.weights, st.beam_scalars,
+                              st.vx, st.vy, st.slope_yx, st.slope_xy, st.beam_nv,
+                              st.beam_external, st.edge_flags,
+                              nullptr, nullptr, nullptr, nullptr, nullptr};
+        Fill(prepared, start + o0, n, sub, sc, t);
+        fillMs += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tf)
+                      .count();
+        // the previous part's result, while this part waits to be copied
+        if (part > 0 && !AddResult(e, localM, localMns))
+            return false;
+        int rc = mbs_upload_staged(e.ctx, &c);
+        if (rc == MBS_OK)
+            rc = mbs_diffract(e.ctx, PrecisionFlag() | (localMns ? MBS_NOSHADOW : 0u));
+        if (!Status(rc, "diffraction", e.ctx))
+            return false;
+    }
+    if (!AddResult(e, localM, localMns))
+        return false;
+    if (timing)
+        std::fprintf(stderr,
+                     "GPU timing: %d orientations, %lld beams in %zu part(s): fill %.3f ms, call "
+                     "%.3f ms (page-locked staging, asynchronous upload)\n",
+                     nOrient, (long long)pc.Beams(), cuts.size() - 1, fillMs,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                         .count());
+    return true;
+}
 }  // namespace
 
 // Handler constants (src/handler/Handler.cpp:102-126).
@@ -600,22 +713,24 @@ bool HandlerPO::HandleOrientationsToLocalGpuOnDevice(
     if (cacheToken != 0 && e->resident == key)
         return pk.nv.empty() || RunBatch(*e, pk, c, localM, ns, 0.0, false);
     e->resident = ThreadContexts::Entry::CacheKey();
+    const PackScale sc(scale, waveIndex, AbsorptionCoefficient());
+    if (cacheToken == 0)
+    {
+        // no reuse asked for: pipelined through page-locked staging
+        PackCounts pc;
+        if (!Count(prepared, start, nOrient, pc))
+            return false;
+        return pc.Beams() == 0 || RunStaged(*e, prepared, start, nOrient, pc, sc, c, localM, ns);
+    }
+    // a batch to keep: packed whole on the host (the cache holds pk)
     const auto tp = std::chrono::steady_clock::now();
-    if (!Pack(prepared, start, nOrient, PackScale(scale, waveIndex, AbsorptionCoefficient()),
-              pk))
+    if (!Pack(prepared, start, nOrient, sc, pk))
         return false;
     const double packMs =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp).count();
-    if (pk.nv.empty())
-    {
-        if (cacheToken != 0)
-            e->resident = key;
-        return true;
-    }
-    if (!RunBatch(*e, pk, c, localM, ns, packMs))
+    if (!pk.nv.empty() && !RunBatch(*e, pk, c, localM, ns, packMs))
         return false;
-    if (cacheToken != 0)
-        e->resident = key;
+    e->resident = key;
     return true;
 }
 

@@ -40,7 +40,7 @@ SLOT_OBJS := $(patsubst $(REF)/%,$(OUT)/objslot/%.o,$(SLOT_SRCS))
 BASE_OBJS := $(filter-out $(patsubst $(REF)/%,$(OUT)/obj/%.o,$(SLOT_SRCS) $(REF)/cpu/GpuSupportStub.cpp),$(OBJS))
 
 all: $(OUT)/mbs_po $(OUT)/libmbsref.a $(OUT)/mbs_po_dump $(OUT)/ref_fixed_harness \
-     $(OUT)/mbs_po_hip $(OUT)/ref_sobol_dump
+     $(OUT)/mbs_po_hip $(OUT)/ref_sobol_dump $(OUT)/dropin_harness
 
 # The reference's Sobol2D points, for the orientation-set fixture.
 $(OUT)/ref_sobol_dump: tests/golden/ref_sobol_dump.cpp $(OUT)/obj/src/math/Sobol.cpp.o
@@ -55,6 +55,14 @@ $(OUT)/hip_shim.o: mbs-fast_amd/host/HandlerPOHip.cpp include/mbs_hip.h
 
 $(OUT)/mbs_po_hip: $(BASE_OBJS) $(SLOT_OBJS) $(OUT)/hip_shim.o $(ENGINE_SO)
 	$(CXX) $(CXXFLAGS) $(BASE_OBJS) $(SLOT_OBJS) $(OUT)/hip_shim.o -o $@ \
+	    -L$(dir $(ENGINE_SO)) -lmbs_hip -Wl,-rpath,'$$ORIGIN/../../mbs-fast_amd' -lm -lgomp
+
+# Harness of the drop-in members the CLI does not reach on one GPU (the
+# pack-cache token, the theta probe): the reference library, the GPU-slot
+# HandlerPO and the product's drop-in over the engine library.
+$(OUT)/dropin_harness: tests/dropin_harness.cpp $(filter-out $(OUT)/obj/src/main.cpp.o,$(BASE_OBJS)) $(OUT)/objslot/src/handler/HandlerPO.cpp.o $(OUT)/hip_shim.o $(ENGINE_SO)
+	$(CXX) $(CXXFLAGS) $(INC) tests/dropin_harness.cpp $(filter-out $(OUT)/obj/src/main.cpp.o,$(BASE_OBJS)) \
+	    $(OUT)/objslot/src/handler/HandlerPO.cpp.o $(OUT)/hip_shim.o -o $@ \
 	    -L$(dir $(ENGINE_SO)) -lmbs_hip -Wl,-rpath,'$$ORIGIN/../../mbs-fast_amd' -lm -lgomp
 
 # Fixed-orientation fixture harness: the reference library plus the dump shim.

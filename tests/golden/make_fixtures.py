@@ -73,8 +73,15 @@ class _Reader:
 
 
 def read_dump(path):
-    """Parses the stream written by ref_dump_shim.cpp into numpy arrays."""
-    r = _Reader(open(path, "rb").read())
+    """Parses the stream written by ref_dump_shim.cpp into numpy arrays.
+
+    Beam records are variable-length (a 12-byte header, 36 doubles, then 33
+    bytes per vertex: x, y, slope_yx, slope_xy and a flag byte); a scan
+    over the record headers finds every record, and the fields are then
+    gathered column by column in blocks."""
+    buf = np.fromfile(path, dtype=np.uint8)
+    data = buf.tobytes()
+    r = _Reader(data)
     assert r.take(8) == b"MBSFIX01"
     n_az = r.i32()
     n_zen = r.i32()
@@ -86,51 +93,99 @@ def read_dump(path):
     out["inv_compl_wave"] = r.f64(2)
     out["legacy_sign"] = r.i32()
     r.i32()  # no-shadow flag of the handler (informational)
-    weights, orient_beams = [], []
-    scalars, nv, ext, valid = [], [], [], []
-    vx, vy, syx, sxy, flags = [], [], [], [], []
+    weights, orient_beams, beam_off = [], [], []
+    scal_rec, scal_cabs = [], None
     m_full = np.zeros((n_az, n_zen + 1, 16))
     m_ns = np.zeros((n_az, n_zen + 1, 16))
-    while not r.done():
-        tag = r.tag()
-        assert tag == "BTCH", tag
-        n_or = r.i32()
+    i32 = struct.Struct("<i")
+    o = r.o
+    while o < len(data):
+        assert data[o:o + 4] == b"BTCH", data[o:o + 4]
+        n_or = i32.unpack_from(data, o + 4)[0]
+        o += 8
         for _ in range(n_or):
-            assert r.tag() == "ORNT"
-            weights.append(r.f64())
-            nb = r.i32()
+            assert data[o:o + 4] == b"ORNT"
+            weights.append(struct.unpack_from("<d", data, o + 4)[0])
+            nb = i32.unpack_from(data, o + 12)[0]
             orient_beams.append(nb)
+            o += 16
             for _ in range(nb):
-                n = r.i32()
-                nv.append(n)
-                ext.append(r.i32())
-                valid.append(r.i32())
-                scalars.append(r.f64(36))
-                for _ in range(n):
-                    x, y, a, b = r.f64(4)
-                    vx.append(x)
-                    vy.append(y)
-                    syx.append(a)
-                    sxy.append(b)
-                    flags.append(r.take(1)[0])
+                beam_off.append(o)
+                o += 300 + 33 * i32.unpack_from(data, o)[0]
+        r.o = o
+        if r.done():
+            break
+        if data[r.o:r.o + 4] == b"SCAL":
+            r.tag()
+            scal_cabs = r.f64()
+            o = r.o
+            nb_batch = sum(orient_beams[-n_or:]) if n_or else 0
+            for _ in range(nb_batch):
+                scal_rec.append(o)
+                o += 80 + 4 + 8 * i32.unpack_from(data, o + 80)[0]
+            r.o = o
         assert r.tag() == "MFUL"
         m_full += r.f64(n_az * (n_zen + 1) * 16).reshape(n_az, n_zen + 1, 16)
         assert r.tag() == "MNSH"
         m_ns += r.f64(n_az * (n_zen + 1) * 16).reshape(n_az, n_zen + 1, 16)
+        o = r.o
+    boff = np.array(beam_off, dtype=np.int64)
+    nb = len(boff)
+
+    def gather(starts, width):
+        idx = starts[:, None] + np.arange(width, dtype=np.int64)[None, :]
+        return buf[idx]
+
+    hdr = gather(boff, 12).view("<i4").reshape(nb, 3) if nb else np.zeros((0, 3), np.int32)
+    nv = hdr[:, 0].astype(np.int32)
+    scalars = np.empty((nb, 36))
+    for s0 in range(0, nb, 65536):
+        blk = boff[s0:s0 + 65536]
+        scalars[s0:s0 + len(blk)] = gather(blk + 12, 288).view("<f8").reshape(-1, 36)
+    voff = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
+    n_vert = int(voff[-1])
+    # byte offset of every vertex record: its beam's block plus 33 j
+    vbeam = np.repeat(np.arange(nb), nv)
+    vstart = boff[vbeam] + 300 + 33 * (np.arange(n_vert) - voff[vbeam])
+    vrec = np.empty((n_vert, 4))
+    flags = np.empty(n_vert, dtype=np.uint8)
+    for s0 in range(0, n_vert, 262144):
+        blk = vstart[s0:s0 + 262144]
+        raw = gather(blk, 33)
+        vrec[s0:s0 + len(blk)] = np.ascontiguousarray(raw[:, :32]).view("<f8").reshape(-1, 4)
+        flags[s0:s0 + len(blk)] = raw[:, 32]
     out["orient_offsets"] = np.concatenate([[0], np.cumsum(orient_beams)]).astype(np.int64)
     out["weights"] = np.array(weights)
-    out["beam_scalars"] = np.array(scalars).reshape(-1, 36)
-    out["beam_nv"] = np.array(nv, dtype=np.int32)
-    out["beam_external"] = np.array(ext, dtype=np.int32)
-    out["beam_valid"] = np.array(valid, dtype=np.int32)
-    out["vert_offsets"] = np.concatenate([[0], np.cumsum(nv)]).astype(np.int64)
-    out["vx"] = np.array(vx)
-    out["vy"] = np.array(vy)
-    out["slope_yx"] = np.array(syx)
-    out["slope_xy"] = np.array(sxy)
-    out["edge_flags"] = np.array(flags, dtype=np.uint8)
+    out["beam_scalars"] = scalars
+    out["beam_nv"] = nv
+    out["beam_external"] = hdr[:, 1].astype(np.int32)
+    out["beam_valid"] = hdr[:, 2].astype(np.int32)
+    out["vert_offsets"] = voff
+    out["vx"] = vrec[:, 0].copy()
+    out["vy"] = vrec[:, 1].copy()
+    out["slope_yx"] = vrec[:, 2].copy()
+    out["slope_xy"] = vrec[:, 3].copy()
+    out["edge_flags"] = flags
     out["mueller_full"] = m_full
     out["mueller_noshadow"] = m_ns
+    if scal_rec:
+        # the multi-size data of every beam (MBS_FIXTURE_SCALING=1)
+        so = np.array(scal_rec, dtype=np.int64)
+        rec = np.empty((len(so), 10))
+        for s0 in range(0, len(so), 65536):
+            blk = so[s0:s0 + 65536]
+            rec[s0:s0 + len(blk)] = gather(blk, 80).view("<f8").reshape(-1, 10)
+        n_abs = gather(so + 80, 4).view("<i4").reshape(-1).astype(np.int64)
+        aoff = np.concatenate([[0], np.cumsum(n_abs)]).astype(np.int64)
+        abeam = np.repeat(np.arange(len(so)), n_abs)
+        astart = so[abeam] + 84 + 8 * (np.arange(int(aoff[-1])) - aoff[abeam])
+        out["raw_jones"] = rec[:, :8].copy()
+        out["path"] = rec[:, 8].copy()
+        out["sign"] = rec[:, 9].copy()
+        out["abs_offsets"] = aoff
+        out["abs_paths"] = (gather(astart, 8).view("<f8").reshape(-1).copy()
+                            if len(astart) else np.zeros(0))
+        out["c_abs"] = scal_cabs
     return out
 
 

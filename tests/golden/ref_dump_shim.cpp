@@ -150,6 +150,39 @@ static void WriteOrientation(const PreparedOrientation &po)
     }
 }
 
+// MBS_FIXTURE_SCALING=1: after a batch's orientations, the per-beam data
+// from which a size-rescaled beam set is re-derived (the multi-size pack
+// rule, src/cuda/GpuDiffraction.cu:273-358): the absorption coefficient,
+// then per beam the raw Jones matrix (8 doubles), the path (optical path of
+// an external beam, projected length of an internal one), the sign (-1 for
+// external beams and internal ones with an odd number of interactions) and
+// the absorption path lengths (count, then values).
+static void WriteScaling(const std::vector<PreparedOrientation> &prepared, int start, int end,
+                         double cAbs)
+{
+    PutTag("SCAL");
+    PutF64(cAbs);
+    for (int i = start; i < end; ++i)
+        for (const PreparedBeam &pb : prepared[i].beams)
+        {
+            const complex raw[4] = {pb.origBeam.J.m11, pb.origBeam.J.m12, pb.origBeam.J.m21,
+                                    pb.origBeam.J.m22};
+            for (int k = 0; k < 4; ++k)
+            {
+                PutF64(real(raw[k]));
+                PutF64(imag(raw[k]));
+            }
+            PutF64(pb.isExternal ? pb.origBeam.opticalPath : pb.info.projLenght);
+            double sign = pb.isExternal ? -1.0 : 1.0;
+            if (!pb.isExternal && (pb.origBeam.nActs & 1))
+                sign = -sign;
+            PutF64(sign);
+            PutI32((int)pb.absorptionPaths.size());
+            for (double p : pb.absorptionPaths)
+                PutF64(p);
+        }
+}
+
 static void WriteMueller(const char *tag, const Arr2D &m, int nAz, int nZen)
 {
     PutTag(tag);
@@ -207,6 +240,9 @@ bool HandlerPO::HandleOrientationsToLocalGpu(
         localJ[0].ClearArr();
         localJns[0].ClearArr();
     }
+    const char *scaling = std::getenv("MBS_FIXTURE_SCALING");
+    if (scaling && scaling[0] == '1')
+        WriteScaling(prepared, start, end, AbsorptionCoefficient());
     WriteMueller("MFUL", batchM, nAz, nZen);
     WriteMueller("MNSH", batchNs, nAz, nZen);
     std::fflush(DumpFile());

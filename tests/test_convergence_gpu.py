@@ -1,37 +1,41 @@
-"""Time to 1 % Mueller error for the 50 um hexagonal column, end to end.
+"""Time to 1 % Mueller error, and the learned control variate inside the
+orientation average, end to end on the reference's own traced beams.
 
-A measurement rather than a parity test; it runs only with
-MBS_CONVERGENCE=1 (about ten minutes on one GPU and 16 host cores).  The reference's
-orientation-convergence method (scripts/run_po_orientation_sampling_convergence.sh,
-scripts/plot_po_orientation_sampling_convergence.py:43-90): independent runs
-of the CLI at growing orientation counts N, each compared with a converged
-high-N run on the same theta grid, and the wall time of the first N whose
-error is at most 1 % on every element P11 ... P44.
+Method: the reference's orientation-convergence measurement
+(scripts/run_po_orientation_sampling_convergence.sh, scripts/
+plot_po_orientation_sampling_convergence.py:43-90): estimates from N = 256,
+512, ... Owen-Sobol orientations, each compared with a converged FP64
+reference on the same theta grid; the wall time of the first N whose error
+is at most 1 % on every element P11 ... P44 (metrics.convergence_errors:
+2 pi dcos-weighted, M11 as a relative L2, the other 15 as the RMS of
+P_ij / P11 differences).
 
-Workload: D = 50 um, L = 100 um, m = 1.3116 + 1e-4 i, lambda = 0.532 um, 10
-reflections, theta rows 0:0.05:10, 10.5:0.5:180 (541) x 96 phi, Owen-Sobol
-orientations (--sobol-seed N 1).  Converged reference: FP64 engine, the mean
-of REF_RUNS independent scrambles (seeds 7, 8, ...) of REF_N orientations
-each (4 x 32768 = 131072 by default; every run reports when it ends).
+Default study (runs in the GPU suite; about three minutes with 16 host
+cores): the hexagonal column D = 10 um, L = 20 um, lambda = 0.532 um,
+m = 1.3116 + 1e-4 i, 6 reflections, theta 0-180 deg at 1 deg x 32 phi.
+  * converged reference: the FP64 engine through the reference CLI
+    (oracle/_ref/mbs_po_hip), the mean of 4 independent scrambles x 16384
+    orientations (seeds 7-10), 8x the largest N studied;
+  * estimation set: Sobol seed 1, 8192 orientations traced once by the
+    reference tracer (oracle/_ref/mbs_po_dump); the N-orientation estimate
+    is its first N orientations (the reference's --sobol-seed N 1 set, as a
+    CLI run at one N confirms), the engine's per-orientation Mueller rows
+    averaged -- FP64 and mixed precision;
+  * control variate: the device MLP (5 -> 4 x 256 GELU -> 16 x 181) trained
+    on 2 x 2048 other orientations (seeds 2, 3), E[g] over 65536 device
+    Sobol orientations, then M_cv = mean(M - c g) + c E[g];
+  * wall times: the GPU sides' tracing at the dump run's measured rate per
+    orientation plus the engine time measured at that N (and the CLI run
+    end to end at the N it needs), the CPU side the reference CLI
+    (oracle/_ref/mbs_po --backend cpu) run at the N it needs, on the host's
+    cores.
 
-Sides:
-  gpu fp64   oracle/_ref/mbs_po_hip --backend gpu, MBS_HIP_PRECISION=fp64
-  gpu mixed  the same with MBS_HIP_PRECISION=fp32 (FP32 aperture sums, FP64
-             phase, MFMA coherent sum)
-  cpu        oracle/_ref/mbs_po --backend cpu on the host's cores: run at the
-             smallest N and scaled linearly in N (tracing and diffraction are
-             both per-orientation loops) to the N the GPU sides need
-Both CLIs trace with the reference's own tracer; the diffraction phase is the
-only part that differs.  Wall time and the CLI's own phase-1 / phase-2 times
-are printed for every run.
-
-Errors: metrics.convergence_errors (2 pi dcos-weighted: M11 relative L2 and
-P_ij / P11 RMS) decides the 1 % bar; metrics.plot_element_errors (the
-reference plot's unweighted form) is printed beside it.
-"""
+The configuration-2 measurement (D = 50 um, 541 x 96, N up to 32768, about
+ten minutes) runs with MBS_CONVERGENCE=1."""
 import os
 import re
 import subprocess
+import sys
 import tempfile
 import time
 
@@ -44,109 +48,419 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF = os.path.join(ROOT, "oracle", "_ref")
 CPU_BIN = os.path.join(REF, "mbs_po")
 HIP_BIN = os.path.join(REF, "mbs_po_hip")
+DUMP_BIN = os.path.join(REF, "mbs_po_dump")
 
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(os.environ.get("MBS_CONVERGENCE") != "1",
-                                 reason="long measurement; set MBS_CONVERGENCE=1 to run"),
-              pytest.mark.skipif(not (os.path.exists(CPU_BIN) and os.path.exists(HIP_BIN)),
+              pytest.mark.skipif(not all(os.path.exists(b) for b in (CPU_BIN, HIP_BIN, DUMP_BIN)),
                                  reason="reference CLI builds (oracle/_ref) are not present")]
 
-THETA = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
-REF_N = int(os.environ.get("MBS_CONVERGENCE_REF_N", "32768"))
-REF_RUNS = int(os.environ.get("MBS_CONVERGENCE_REF_RUNS", "4"))
-N_MAX = int(os.environ.get("MBS_CONVERGENCE_NMAX", "32768"))
-NPHI = 96
 BAR = 0.01
+PARTICLE = ["--particle", "1", "20", "10", "--refractive-index", "1.3116", "1e-4",
+            "--wavelength-um", "0.532", "--max-reflections", "6"]
+GRID = ["--scattering-grid", "0", "180", "32", "180"]
+N_MAX = 8192
+TRUTH_N, TRUTH_SEEDS = 16384, (7, 8, 9, 10)
+TRAIN_N, TRAIN_SEEDS = 2048, (2, 3)
+BETA_SYM, GAMMA_SYM = np.pi / 2, np.pi / 3   # the hexagonal prism's domain
 
 
 def _threads():
     return max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count() or 1))
 
 
-def _run(wd, binary, backend, n, seed, out, prec=None):
+def _cli(binary, backend, n, seed, wd, out, args, prec=None, env_extra=None):
     env = dict(os.environ)
     if prec:
         env["MBS_HIP_PRECISION"] = prec
-    args = [binary, "--method", "po", "--particle", "1", "100", "50",
-            "--refractive-index", "1.3116", "1e-4", "--wavelength-um", "0.532",
-            "--max-reflections", "10", "--sobol-seed", str(n), str(seed),
-            "--theta-grid-file", "theta.grid", "--phi-points", str(NPHI),
-            "--cutoff-profile", "off", "--threads", str(_threads()), "--close",
-            "--backend", backend, "--output", out]
+    env.update(env_extra or {})
+    cmd = [binary, "--method", "po"] + args + ["--sobol-seed", str(n), str(seed),
+                                              "--cutoff-profile", "off", "--threads",
+                                              str(_threads()), "--close", "--backend", backend,
+                                              "--output", out]
     t0 = time.perf_counter()
-    # The CLI's per-chunk lines are echoed as it prints them, so a long run
-    # shows its progress.
-    p = subprocess.Popen(args, cwd=wd, env=env, stdin=subprocess.DEVNULL, stdout=subprocess.PIPE,
-                         stderr=subprocess.STDOUT, text=True)
-    lines = []
-    for line in p.stdout:
-        lines.append(line)
-        if "adaptive memory" in line:
-            print("  [%s] %s" % (out, line.strip()[:160]), flush=True)
-    rc = p.wait(timeout=1200)
+    p = subprocess.run(cmd, cwd=wd, env=env, stdin=subprocess.DEVNULL, capture_output=True,
+                       text=True, timeout=1800)
     wall = time.perf_counter() - t0
-    text = "".join(lines)
-    assert rc == 0, "%s rc %d:\n%s" % (binary, rc, text[-3000:])
-    ph = [float(x) for x in re.findall(r"Phase [12] \([^)]*\): ([0-9.]+) s", text)[:2]]
-    rows = np.loadtxt(os.path.join(wd, out, out + ".dat"), skiprows=1)
-    return rows, wall, ph
+    assert p.returncode == 0, "%s rc %d:\n%s\n%s" % (binary, p.returncode, p.stdout[-3000:],
+                                                     p.stderr[-2000:])
+    ph = [float(x) for x in re.findall(r"Phase [12] \([^)]*\): ([0-9.]+) s", p.stdout)[:2]]
+    return wall, ph
 
 
-def _sweep(wd, name, prec, ref):
-    """Runs N = 64, 128, ... until the bar is met (or N_MAX); returns the
-    first N that meets it with its wall and phase times (None if none does)
-    and the largest element error of every run."""
-    n, hit, worst = 64, None, []
-    while n <= N_MAX:
-        rows, wall, ph = _run(wd, HIP_BIN, "gpu", n, 1, "%s_%d" % (name, n), prec)
-        e, ep = M.convergence_errors(rows, ref), M.plot_element_errors(rows, ref)
-        print("%-9s N=%5d wall %7.2f s (trace %.2f s, diffraction %.2f s)  max element %.4f "
-              "(M%d)  P11 %.4f | plot form: max %.4f" % (name, n, wall, ph[0], ph[1], e.max(),
-                                                          11 + 10 * (e.argmax() // 4) + e.argmax() % 4,
-                                                          e[0], ep.max()), flush=True)
-        worst.append(e.max())
-        if e.max() <= BAR:
-            hit = (n, wall, ph)
-            break
-        n *= 2
-    return hit, worst
+def _rows(wd, out):
+    return np.loadtxt(os.path.join(wd, out, out + ".dat"), skiprows=1)
 
 
-def test_time_to_one_percent():
+def _trace(wd, n, seed):
+    """The reference tracer's prepared beams of --sobol-seed n seed, and the
+    tracing (phase 1) seconds of that run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fixtures
+    d = os.path.join(wd, "trace_%d_%d" % (n, seed))
+    os.makedirs(d)
+    dump = os.path.join(d, "dump.bin")
+    _, ph = _cli(DUMP_BIN, "gpu", n, seed, d, "dmp", PARTICLE + GRID,
+                 env_extra={"MBS_FIXTURE_DUMP": dump, "MBS_FIXTURE_NO_EXPECTED": "1"})
+    fx = make_fixtures.read_dump(dump)
+    os.remove(dump)
+    return fx, ph[0]
+
+
+def _orientation_rows(pkg, ctx, fx, flags, chunk=2048):
+    """Per-orientation azimuth-averaged Mueller rows [n, 181, 16] (weights
+    divided out) from the engine, the 2 pi dcos column, and the engine's
+    wall time for the whole set."""
+    abi = pkg.abi
+    grid, batch = abi.Grid.from_fixture(fx), abi.BeamBatch.from_fixture(fx)
+    consts = abi.consts_from_fixture(fx)
+    ctx.set_grid(grid)
+    out, t = [], 0.0
+    for o0 in range(0, batch.n_orient, chunk):
+        sub = batch.subset(list(range(o0, min(batch.n_orient, o0 + chunk))))
+        t0 = time.perf_counter()
+        ctx.upload(sub, consts)
+        ctx.diffract(flags | abi.PER_ORIENT)
+        rows = ctx.azimuth_average_per_orient()
+        t += time.perf_counter() - t0
+        out.append(pkg.cv.unweighted_rows(rows, sub.weights))
+    return np.concatenate(out), rows[0, :, :2], t
+
+
+def _engine_time(pkg, ctx, fx, n, flags):
+    """Engine wall time (upload, diffraction, Mueller grid back to the host)
+    of the first n orientations, as the drop-in spends it per batch."""
+    abi = pkg.abi
+    grid, batch = abi.Grid.from_fixture(fx), abi.BeamBatch.from_fixture(fx)
+    sub = batch.subset(list(range(n)))
+    ctx.set_grid(grid)
+    consts = abi.consts_from_fixture(fx)
+    ctx.upload(sub, consts)       # first touch of this size's buffers, untimed
+    ctx.mueller(flags)
+    t0 = time.perf_counter()
+    ctx.upload(sub, consts)
+    ctx.mueller(flags)
+    return time.perf_counter() - t0
+
+
+def _estimate_rows(head, m):
+    return np.concatenate([head, m], axis=1)
+
+
+@pytest.fixture(scope="module")
+def study(pkg):
     wd = tempfile.mkdtemp(prefix="mbs_conv_")
-    with open(os.path.join(wd, "theta.grid"), "w") as f:
-        f.write("\n".join("%.10g" % t for t in THETA) + "\n")
-    print("converged reference: fp64 engine, %d x %d orientations (seeds 7..%d)"
-          % (REF_RUNS, REF_N, 6 + REF_RUNS), flush=True)
+    s = {"wd": wd}
+    t0 = time.perf_counter()
     parts = []
-    for r in range(REF_RUNS):
-        rows, wall, ph = _run(wd, HIP_BIN, "gpu", REF_N, 7 + r, "ref%d" % r, "fp64")
-        parts.append(rows)
+    for seed in TRUTH_SEEDS:
+        _cli(HIP_BIN, "gpu", TRUTH_N, seed, wd, "truth%d" % seed, PARTICLE + GRID, prec="fp64")
+        parts.append(_rows(wd, "truth%d" % seed))
+    s["truth"] = np.mean(parts, axis=0)
+    spread = max(M.convergence_errors(p, s["truth"]).max() for p in parts)
+    print("converged reference: %d x %d orientations in %.1f s; one scramble's largest element "
+          "error against the mean %.4f (the mean's own error ~%.4f)"
+          % (len(TRUTH_SEEDS), TRUTH_N, time.perf_counter() - t0, spread,
+             spread / np.sqrt(len(TRUTH_SEEDS))), flush=True)
+    s["est_fx"], t_trace = _trace(wd, N_MAX, 1)
+    s["trace_per_orient"] = t_trace / N_MAX
+    print("estimation set: %d orientations traced in %.2f s (%d beams)"
+          % (N_MAX, t_trace, len(s["est_fx"]["beam_nv"])), flush=True)
+    s["train_fx"] = [_trace(wd, TRAIN_N, seed)[0] for seed in TRAIN_SEEDS]
+    with pkg.engine.Context(0) as ctx:
+        abi = pkg.abi
+        s["M64"], s["head"], _ = _orientation_rows(pkg, ctx, s["est_fx"], abi.PREC_FP64)
+        s["M32"], _, _ = _orientation_rows(pkg, ctx, s["est_fx"], abi.PREC_MIXED)
+        train = [_orientation_rows(pkg, ctx, fx, abi.PREC_FP64)[0] for fx in s["train_fx"]]
+        X = [pkg.cv.features(ctx.orientation_set(abi.ORIENT_SOBOL, TRAIN_N, seed=sd,
+                                                 beta_sym=BETA_SYM, gamma_sym=GAMMA_SYM),
+                             np.log(10.0)) for sd in TRAIN_SEEDS]
+        s["X"] = pkg.cv.features(ctx.orientation_set(abi.ORIENT_SOBOL, N_MAX, seed=1,
+                                                     beta_sym=BETA_SYM, gamma_sym=GAMMA_SYM),
+                                 np.log(10.0))
+        cv = pkg.cv.ControlVariate(ctx, s["M64"].shape[1], width=256, depth=4, seed=11)
+        t0 = time.perf_counter()
+        s["cv_loss"] = cv.fit(np.concatenate(X), np.concatenate(train), epochs=200, batch=128,
+                              lr=1e-3, seed=5)
+        s["Eg"] = cv.expectation(abi.ORIENT_SOBOL, 65536, 7, BETA_SYM, GAMMA_SYM, np.log(10.0))
+        s["t_train"] = time.perf_counter() - t0
+        s["G"] = cv.predict(s["X"])
+        s["c"] = cv.c.copy()
+        t0 = time.perf_counter()
+        cv.predict(s["X"])
+        s["predict_per_orient"] = (time.perf_counter() - t0) / N_MAX
+        s["t_engine"] = {}
+        for n in (256, 512, 1024, 2048, 4096, 8192):
+            for name, flags in (("fp64", abi.PREC_FP64), ("mixed", abi.PREC_MIXED)):
+                s["t_engine"][name, n] = _engine_time(pkg, ctx, s["est_fx"], n, flags)
+        cv.close()
+    return s
+
+
+def _plain(s, key, n):
+    return _estimate_rows(s["head"], s[key][:n].mean(0))
+
+
+def _with_cv(s, n, c=None):
+    c = s["c"] if c is None else c
+    return _estimate_rows(s["head"], (s["M64"][:n] - c * s["G"][:n]).mean(0) + c * s["Eg"])
+
+
+def test_prefix_is_the_n_point_run(study):
+    """The first N orientations of the 8192-orientation set are the
+    reference's --sobol-seed N 1 set: the CLI's own result at N = 1024
+    equals the prefix estimate (both FP64, same beams)."""
+    wd = study["wd"]
+    _cli(HIP_BIN, "gpu", 1024, 1, wd, "prefix", PARTICLE + GRID, prec="fp64")
+    cli = _rows(wd, "prefix")
+    est = _plain(study, "M64", 1024)
+    np.testing.assert_allclose(est[:, 0], cli[:, 0], rtol=0, atol=1e-9)
+    g = M.gate_l2(est, cli)
+    print("prefix estimate vs the CLI's N=1024 run: gate_l2 %.2e" % g)
+    assert g < 1e-9
+
+
+def test_control_variate_is_unbiased_and_gains(study):
+    """c = 0 is the plain average bit for bit; with the fitted c the error
+    against the converged reference is printed per element for every N,
+    with the variance reduction per element measured over 64 disjoint
+    blocks of 128 orientations; the P11 error must drop."""
+    truth = study["truth"]
+    n = 1024
+    np.testing.assert_array_equal(_with_cv(study, n, c=0.0), _plain(study, "M64", n))
+    rows = []
+    for n in (256, 512, 1024, 2048, 4096, 8192):
+        ep = M.convergence_errors(_plain(study, "M64", n), truth)
+        ec = M.convergence_errors(_with_cv(study, n), truth)
+        rows.append((n, ep, ec))
+        print("N=%5d  plain: max %.4f P11 %.4f | control variate: max %.4f P11 %.4f"
+              % (n, ep.max(), ep[0], ec.max(), ec[0]))
+    # variance of block means, plain vs control variate, per element
+    blocks = study["M64"].reshape(64, 128, *study["M64"].shape[1:])
+    gblocks = study["G"].reshape(64, 128, *study["G"].shape[1:])
+    w = truth[:, 1] / truth[:, 1].sum()
+    plain_means = blocks.mean(1)
+    cv_means = (blocks - study["c"] * gblocks).mean(1)
+    vp = np.einsum("t,tk->k", w, plain_means.var(0))
+    vc = np.einsum("t,tk->k", w, cv_means.var(0))
+    ratio = vp / np.maximum(vc, 1e-300)
+    print("variance reduction per element (plain / control variate, 2 pi dcos-weighted "
+          "block variance): " + " ".join("%.2f" % r for r in ratio))
+    print("network: final loss %.3f, training + E[g] %.1f s" % (study["cv_loss"],
+                                                                 study["t_train"]))
+    gain11 = np.mean([ep[0] / ec[0] for n, ep, ec in rows if n >= 1024])
+    print("P11 error ratio plain / control variate, N >= 1024: %.2f" % gain11)
+    assert ratio[0] > 1.5
+    assert gain11 > 1.2
+
+
+def test_time_to_one_percent(study):
+    truth, wd = study["truth"], study["wd"]
+    hit = {}
+    ns = (256, 512, 1024, 2048, 4096, 8192)
+    for name in ("fp64", "mixed", "cv"):
+        for n in ns:
+            est = _with_cv(study, n) if name == "cv" else \
+                _plain(study, "M64" if name == "fp64" else "M32", n)
+            e = M.convergence_errors(est, truth)
+            if e.max() <= BAR:
+                hit[name] = (n, e.max())
+                break
+        print("%-5s 1 %% on every element at N = %s" % (name, hit.get(name, ("not reached",))[0]))
+    assert "fp64" in hit, "the plain FP64 average does not reach 1 % by N = %d" % N_MAX
+    n64 = hit["fp64"][0]
+    t_tr = study["trace_per_orient"]
+    lines = []
+
+    def gpu_time(name, n, extra=0.0):
+        key = "mixed" if name == "mixed" else "fp64"
+        return n * t_tr + study["t_engine"][key, n] + extra
+    # end to end through the CLI at the N each GPU side needs
+    wall = {}
+    for name, prec in (("fp64", "fp64"), ("mixed", "fp32")):
+        if name in hit:
+            wall[name], ph = _cli(HIP_BIN, "gpu", hit[name][0], 1, wd, "e2e_" + name,
+                                  PARTICLE + GRID, prec=prec)
+            lines.append("GPU %-5s N=%5d  CLI wall %.2f s (trace %.2f s, diffraction %.2f s); "
+                         "trace rate + engine %.2f s" % (name, hit[name][0], wall[name], ph[0],
+                                                          ph[1], gpu_time(name, hit[name][0])))
+    wall_cpu, ph_cpu = _cli(CPU_BIN, "cpu", n64, 1, wd, "cpu", PARTICLE + GRID)
+    lines.append("CPU ref N=%5d  CLI wall %.2f s (trace %.2f s, diffraction %.2f s) on %d "
+                 "threads" % (n64, wall_cpu, ph_cpu[0], ph_cpu[1], _threads()))
+    if "cv" in hit:
+        n = hit["cv"][0]
+        t_cv = gpu_time("cv", n, n * study["predict_per_orient"])
+        lines.append("GPU CV    N=%5d  trace rate + engine + network %.2f s, plus training "
+                     "%.1f s once" % (n, t_cv, study["t_train"]))
+    for ln in lines:
+        print(ln)
+    print("time to 1 %%: CPU %.2f s, GPU FP64 %.2f s (%.1fx), GPU mixed %s, control variate %s"
+          % (wall_cpu, wall["fp64"], wall_cpu / wall["fp64"],
+             "%.2f s" % wall["mixed"] if "mixed" in wall else "not reached",
+             "%.2f s" % gpu_time("cv", hit["cv"][0], hit["cv"][0] * study["predict_per_orient"])
+             if "cv" in hit else "not reached"))
+    # mixed precision reaches the bar with the same orientations as FP64
+    assert hit.get("mixed", (None,))[0] == n64
+    assert wall["fp64"] < wall_cpu
+
+
+# ---- the control variate at the learned-control-variate configuration's scale
+# (opt-in) ---------------------------------------------------------------------
+
+C4_SIZES = np.geomspace(10.0, 80.0, 8)
+C4_TRAIN_N, C4_CHUNK = 65536, 8192
+C4_PARTICLE_D10 = ["--particle", "1", "20", "10", "--refractive-index", "1.3116", "1e-4",
+                   "--wavelength-um", "0.532", "--max-reflections", "10"]
+
+
+def _c4_particle(d):
+    return ["--particle", "1", "%.10g" % (2 * d), "%.10g" % d, "--refractive-index", "1.3116",
+            "1e-4", "--wavelength-um", "0.532", "--max-reflections", "10"]
+
+
+@pytest.mark.skipif(os.environ.get("MBS_CV_CONFIG4") != "1",
+                    reason="long measurement; set MBS_CV_CONFIG4=1 to run")
+def test_control_variate_config4(pkg):
+    """The network at its stated scale: 65536 orientations x 8 sizes (D = 10
+    ... 80 um log-spaced, L = 2D, m = 1.3116 + 1e-4 i, 10 reflections) on
+    181 theta rows x 32 phi.  Beam splitting does not depend on size, so
+    every orientation is traced once (D = 10 um, with the per-beam rescale
+    data; 8 Sobol scrambles of 8192) and each size is re-derived on the
+    device (mbs_rescale_beams, the multi-size rule); the engine's
+    mixed-precision per-orientation rows are the targets (524288 samples).
+    Then, at D = 50 um (not a training size): variance reduction per element
+    on 8192 held-out orientations, and the error against a converged FP64
+    reference (4 x 16384 orientations) of the plain and the control-variate
+    averages for N = 256 ... 8192."""
+    abi, cvm = pkg.abi, pkg.cv
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_fixtures
+    wd = tempfile.mkdtemp(prefix="mbs_cv4_")
+    t_start = time.perf_counter()
+    X, T = [], []
+    with pkg.engine.Context(0) as ctx:
+        t_eng = 0.0
+        for k in range(C4_TRAIN_N // C4_CHUNK):
+            seed = 100 + k
+            d = os.path.join(wd, "s%d" % seed)
+            os.makedirs(d)
+            dump = os.path.join(d, "dump.bin")
+            _cli(DUMP_BIN, "gpu", C4_CHUNK, seed, d, "dmp", C4_PARTICLE_D10 + GRID,
+                 env_extra={"MBS_FIXTURE_DUMP": dump, "MBS_FIXTURE_NO_EXPECTED": "1",
+                            "MBS_FIXTURE_SCALING": "1"})
+            fx = make_fixtures.read_dump(dump)
+            os.remove(dump)
+            batch = abi.BeamBatch.from_fixture(fx)
+            grid, consts = abi.Grid.from_fixture(fx), abi.consts_from_fixture(fx)
+            q = ctx.orientation_set(abi.ORIENT_SOBOL, C4_CHUNK, seed=seed, beta_sym=BETA_SYM,
+                                    gamma_sym=GAMMA_SYM)
+            t0 = time.perf_counter()
+            ctx.set_grid(grid)
+            ctx.upload(batch, consts)
+            ctx.upload_scaling(fx["raw_jones"], fx["path"], fx["sign"], fx["abs_offsets"],
+                               fx["abs_paths"], fx["c_abs"], consts.wave_index)
+            for dsz in C4_SIZES:
+                ctx.rescale(dsz / 10.0)
+                ctx.diffract(abi.PREC_MIXED | abi.PER_ORIENT)
+                rows = ctx.azimuth_average_per_orient()
+                T.append(cvm.unweighted_rows(rows, batch.weights).astype(np.float32))
+                X.append(cvm.features(q, np.log(dsz)))
+            t_eng += time.perf_counter() - t0
+            print("scramble %d: %d orientations, %d beams; %.0f s so far"
+                  % (seed, batch.n_orient, batch.n_beams, time.perf_counter() - t_start),
+                  flush=True)
+        X = np.concatenate(X)
+        T = np.concatenate(T).astype(np.float64)
+        print("training samples: %d, engine %.1f s" % (len(X), t_eng), flush=True)
+        cv = cvm.ControlVariate(ctx, T.shape[1], width=256, depth=4, seed=11)
+        try:
+            t0 = time.perf_counter()
+            loss = cv.fit(X, T, epochs=20, batch=512, lr=1e-3, seed=5)
+            print("network: final loss %.3f, training %.1f s" % (loss, time.perf_counter() - t0),
+                  flush=True)
+            del T
+            d = os.path.join(wd, "d50")
+            os.makedirs(d)
+            dump = os.path.join(d, "dump.bin")
+            _cli(DUMP_BIN, "gpu", N_MAX, 1, d, "dmp", _c4_particle(50.0) + GRID,
+                 env_extra={"MBS_FIXTURE_DUMP": dump, "MBS_FIXTURE_NO_EXPECTED": "1"})
+            fx50 = make_fixtures.read_dump(dump)
+            os.remove(dump)
+            M50, head, _ = _orientation_rows(pkg, ctx, fx50, abi.PREC_MIXED)
+            X50 = cvm.features(ctx.orientation_set(abi.ORIENT_SOBOL, N_MAX, seed=1,
+                                                   beta_sym=BETA_SYM, gamma_sym=GAMMA_SYM),
+                               np.log(50.0))
+            G50 = cv.predict(X50)
+            Eg = cv.expectation(abi.ORIENT_SOBOL, 65536, 7, BETA_SYM, GAMMA_SYM, np.log(50.0))
+            c = cv.c
+        finally:
+            cv.close()
+    parts = []
+    for seed in TRUTH_SEEDS:
+        _cli(HIP_BIN, "gpu", TRUTH_N, seed, wd, "t50_%d" % seed, _c4_particle(50.0) + GRID,
+             prec="fp64")
+        parts.append(_rows(wd, "t50_%d" % seed))
+    truth = np.mean(parts, axis=0)
+    w = truth[:, 1] / truth[:, 1].sum()
+    blocks = M50.reshape(64, -1, *M50.shape[1:])
+    gb = G50.reshape(64, -1, *G50.shape[1:])
+    vp = np.einsum("t,tk->k", w, blocks.mean(1).var(0))
+    vc = np.einsum("t,tk->k", w, (blocks - c * gb).mean(1).var(0))
+    print("D = 50 um, variance reduction per element (plain / control variate): "
+          + " ".join("%.2f" % r for r in vp / np.maximum(vc, 1e-300)))
+    hit = {}
+    for n in (256, 512, 1024, 2048, 4096, 8192):
+        ep = M.convergence_errors(_estimate_rows(head, M50[:n].mean(0)), truth)
+        ec = M.convergence_errors(_estimate_rows(head, (M50[:n] - c * G50[:n]).mean(0) + c * Eg),
+                                  truth)
+        for name, e in (("plain", ep), ("cv", ec)):
+            if e.max() <= BAR and name not in hit:
+                hit[name] = n
+        print("N=%5d  plain: max %.4f P11 %.4f | control variate: max %.4f P11 %.4f"
+              % (n, ep.max(), ep[0], ec.max(), ec[0]))
+    print("1 %% on every element: plain at N = %s, control variate at N = %s; total %.0f s"
+          % (hit.get("plain"), hit.get("cv"), time.perf_counter() - t_start))
+
+
+# ---- the configuration-2 measurement (opt-in) -------------------------------
+
+THETA_C2 = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
+C2_PARTICLE = ["--particle", "1", "100", "50", "--refractive-index", "1.3116", "1e-4",
+               "--wavelength-um", "0.532", "--max-reflections", "10"]
+
+
+@pytest.mark.skipif(os.environ.get("MBS_CONVERGENCE") != "1",
+                    reason="long measurement; set MBS_CONVERGENCE=1 to run")
+def test_time_to_one_percent_config2():
+    """D = 50 um, L = 100 um, 541 x 96, N = 1024 ... 32768 through the CLI on
+    both GPU precisions, the converged reference 4 x 32768 (seeds 7-10), the
+    CPU reference CLI run at the N the GPU needs."""
+    wd = tempfile.mkdtemp(prefix="mbs_conv2_")
+    with open(os.path.join(wd, "theta.grid"), "w") as f:
+        f.write("\n".join("%.10g" % t for t in THETA_C2) + "\n")
+    grid = ["--theta-grid-file", "theta.grid", "--phi-points", "96"]
+    parts = []
+    for seed in (7, 8, 9, 10):
+        w, ph = _cli(HIP_BIN, "gpu", 32768, seed, wd, "ref%d" % seed, C2_PARTICLE + grid,
+                     prec="fp64")
+        parts.append(_rows(wd, "ref%d" % seed))
         print("reference run %d: wall %.1f s (trace %.1f s, diffraction %.1f s)"
-              % (r, wall, *ph), flush=True)
+              % (seed, w, *ph), flush=True)
     ref = np.mean(parts, axis=0)
-    if REF_RUNS > 1:
-        # spread of the independent scrambles: the reference's own error is
-        # about this divided by sqrt(REF_RUNS)
-        spread = max(M.convergence_errors(p, ref).max() for p in parts)
-        print("largest element error of one reference run against their mean: %.4f" % spread,
-              flush=True)
-    sweeps = {name: _sweep(wd, name, prec, ref) for name, prec in (("gpu fp64", "fp64"),
-                                                                   ("gpu mixed", "fp32"))}
-    n_cpu = 64
-    _, wall_cpu, ph_cpu = _run(wd, CPU_BIN, "cpu", n_cpu, 1, "cpu_%d" % n_cpu)
-    print("cpu       N=%5d wall %7.2f s (trace %.2f s, diffraction %.2f s) on %d threads"
-          % (n_cpu, wall_cpu, ph_cpu[0], ph_cpu[1], _threads()), flush=True)
-    for name, (hit, worst) in sweeps.items():
-        # Sobol sets converge: the worst element shrinks from the first N to the last.
-        assert len(worst) == 1 or worst[-1] < worst[0], (name, worst)
-        if hit is None:
-            print("%s: 1 %% on every element not reached by N = %d" % (name, N_MAX))
-            continue
-        n, wall, ph = hit
-        cpu_wall = wall_cpu * n / n_cpu
-        cpu_diff = ph_cpu[1] * n / n_cpu
-        print("%s: 1 %% at N = %d after %.2f s wall (diffraction %.2f s); cpu reference at the "
-              "same N (scaled from N=%d): %.1f s wall (diffraction %.1f s) -> %.1fx wall, "
-              "%.0fx diffraction" % (name, n, wall, ph[1], n_cpu, cpu_wall, cpu_diff,
-                                     cpu_wall / wall, cpu_diff / max(ph[1], 1e-9)))
+    hit = {}
+    for name, prec in (("gpu fp64", "fp64"), ("gpu mixed", "fp32")):
+        n = 1024
+        while n <= 32768:
+            w, ph = _cli(HIP_BIN, "gpu", n, 1, wd, "%s_%d" % (prec, n), C2_PARTICLE + grid,
+                         prec=prec)
+            e = M.convergence_errors(_rows(wd, "%s_%d" % (prec, n)), ref)
+            print("%-9s N=%5d wall %7.2f s (trace %.2f s, diffraction %.2f s) max element %.4f"
+                  % (name, n, w, ph[0], ph[1], e.max()), flush=True)
+            if e.max() <= BAR:
+                hit[name] = (n, w)
+                break
+            n *= 2
+    if "gpu fp64" in hit:
+        n = hit["gpu fp64"][0]
+        w, ph = _cli(CPU_BIN, "cpu", n, 1, wd, "cpu", C2_PARTICLE + grid)
+        print("cpu N=%d wall %.1f s (diffraction %.1f s) -> GPU FP64 %.1fx sooner"
+              % (n, w, ph[1], w / hit["gpu fp64"][1]))

@@ -657,3 +657,106 @@ def test_mlp_gradient_against_oracle(pkg, ctx):
         l1 = net2.train(xs, ts, batch=256, epochs=30, lr=3e-3, seed=1)
         print("training loss %.4e -> %.4e" % (l0, l1))
         assert l1 < 0.1 * l0
+
+
+def test_staged_upload_matches_plain_upload(pkg, ctx):
+    """mbs_stage_beams / mbs_upload_staged (page-locked staging, asynchronous
+    copies) give the same device batch as mbs_upload_beams: results bit for
+    bit, for a staged batch diffracted at once and for one whose staging
+    area is reused two stagings later while the other area is in flight."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, full, consts = _setup(pkg, fx)
+    parts = [full.subset([0, 1, 2]), full.subset([3, 4]), full.subset([5, 6, 7])]
+    ctx.set_grid(grid)
+    want = []
+    for b in parts:
+        ctx.upload(b, consts)
+        want.append(ctx.mueller(abi.PREC_FP64, noshadow=True))
+    for i, b in enumerate(parts):
+        ctx.upload_staged(b, consts)
+        m, mn = ctx.mueller(abi.PREC_FP64, noshadow=True)
+        np.testing.assert_array_equal(m, want[i][0])
+        np.testing.assert_array_equal(mn, want[i][1])
+    # a staged batch that is empty of beams, then a plain upload again
+    empty = abi.BeamBatch([0, 0], [1.0], np.zeros((0, 36)), np.zeros(0, np.int32),
+                          np.zeros(0, np.int32), [0], np.zeros(0), np.zeros(0), np.zeros(0),
+                          np.zeros(0), np.zeros(0, np.uint8))
+    ctx.upload_staged(empty, consts)
+    assert not ctx.mueller(0)[0].any()
+    ctx.upload(parts[0], consts)
+    np.testing.assert_array_equal(ctx.mueller(abi.PREC_FP64, noshadow=True)[0], want[0][0])
+
+
+def test_work_queue_matches_segments(pkg, oracle, ctx):
+    """The device-planned, persistent work-queue launch (MBS_WORK_QUEUE)
+    makes the host plan's segments and writes one partial per segment, so it
+    equals the host-planned segment launch bit for bit -- FP64, mixed with
+    MFMA and with the vector-ALU sum, full and no-shadow -- and the oracle."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    for flags in (abi.PREC_FP64, abi.PREC_MIXED, abi.PREC_MIXED | abi.NO_MFMA):
+        out = {}
+        for mode in (abi.WORK_SEGMENTS, abi.WORK_QUEUE):
+            ctx.set_work_mode(mode)
+            out[mode] = ctx.mueller(flags, noshadow=True)
+        ctx.set_work_mode(0)
+        for k in range(2):
+            np.testing.assert_array_equal(out[abi.WORK_QUEUE][k], out[abi.WORK_SEGMENTS][k])
+    ctx.set_work_mode(abi.WORK_QUEUE)
+    m, _ = ctx.mueller(abi.PREC_FP64)
+    ctx.set_work_mode(0)
+    o, _, _ = oracle.diffract(grid, batch, consts)
+    check_fp64(m, o, grid.theta, "work queue vs oracle")
+
+
+def test_queue_graph_replays_new_uploads(pkg, ctx):
+    """16384 orientations with beam counts spread log-uniformly over 5..400
+    in one persistent launch under graph replay.  The segment plan is made on
+    the device inside the graph, so after a new upload with other beam counts
+    (same orientation count, within the buffers) the same graph replays --
+    one capture for every launch -- and follows the new batch: every replay
+    equals the eager host-planned launch of its batch bit for bit, with eager
+    work and synchronisations between the replays."""
+    abi, syn = pkg.abi, pkg.synthetic
+    fx = load_fixture("sobol_average")
+    src, consts = abi.BeamBatch.from_fixture(fx), abi.consts_from_fixture(fx)
+    grid = syn.uniform_grid(36, 8)
+    n_orient = 16384
+    rng = np.random.default_rng(3)
+    batches = []
+    for seed, hi in ((1, 400), (2, 300)):
+        counts = np.exp(rng.uniform(np.log(5), np.log(hi), n_orient)).astype(np.int64)
+        batches.append(syn.resample(src, counts, seed=seed))
+    print("batches: %d orientations, %d and %d beams"
+          % (n_orient, batches[0].n_beams, batches[1].n_beams), flush=True)
+    ctx.set_grid(grid)
+    want = []
+    for b in batches:
+        ctx.upload(b, consts)
+        ctx.set_work_mode(abi.WORK_SEGMENTS)
+        want.append(ctx.mueller(abi.PREC_MIXED)[0])
+    ctx.set_work_mode(abi.WORK_QUEUE)
+    ctx.set_graph(True)
+    ctx.upload(batches[0], consts)   # the larger batch first: later uploads fit its buffers
+    ctx.mueller(abi.PREC_MIXED)
+    c0, l0 = ctx.graph_stats()
+    try:
+        for rep in range(3):
+            for i, b in enumerate(batches):
+                ctx.upload(b, consts)
+                got = ctx.mueller(abi.PREC_MIXED)[0]
+                np.testing.assert_array_equal(got, want[i])
+                # eager work of the program between the replays: the
+                # azimuth average of the result, with a synchronisation
+                rows = ctx.azimuth_average()
+                assert np.all(np.isfinite(rows))
+        c1, l1 = ctx.graph_stats()
+        print("graph: %d capture(s) for %d launches" % (c1 - c0 + 1, l1 - l0 + 1))
+        assert c1 == c0 and l1 - l0 == 6   # six replays of the one graph
+    finally:
+        ctx.set_graph(False)
+        ctx.set_work_mode(0)
