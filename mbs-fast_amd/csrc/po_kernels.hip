@@ -351,8 +351,8 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
         const uint8_t f = ef[v0 + i];
         v[V_KX] = k * X(i);
         v[V_KY] = k * Y(i);
-        v[V_SYX] = (f & 1) ? syx[v0 + i] : 0.0;
-        v[V_SXY] = (f & 2) ? sxy[v0 + i] : 0.0;
+        v[V_SYX] = (f & 1) ? syx[v0 + i] : kInvalidSlope;
+        v[V_SXY] = (f & 2) ? sxy[v0 + i] : kInvalidSlope;
         const int j = (i + 1 < nv) ? i + 1 : 0;
         double *w = vdd + (size_t)(v0 + i) * VDD_STRIDE;
         w[D_DX] = X(j) - X(i);
@@ -673,15 +673,15 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
     }
     // the edge coefficient is cancellation-prone: FP64
     const double cc = useB ? __builtin_fma(syx, B, A) : __builtin_fma(sxy, A, B);
-    const bool valid = (f & vmask) != 0;
     const bool big = __builtin_fabs(cc) > eps1;
     // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
-    // carry more rounding than that
+    // carry more rounding than that; an invalid edge's stored slope makes
+    // it exactly zero
     const float r = __builtin_amdgcn_rcpf((float)cc);
-    const float inv = (valid && big) ? r : 0.0f;
+    const float inv = big ? r : 0.0f;
     sr = __builtin_fmaf(cn - cp, inv, sr);
     si = __builtin_fmaf(sn - sp, inv, si);
-    if (__builtin_expect(valid && !big && !small, 0)) {
+    if (__builtin_expect(!big && !small && (f & vmask) != 0, 0)) {
         const double *w = vdd + (size_t)e * VDD_STRIDE;
         const double dd = useB ? uniform_f64(w[D_DX]) : uniform_f64(w[D_DY]);
         const double2 q =
@@ -729,12 +729,14 @@ __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__
 
 // The edge sum of aperture_mixed_nv.  Both coefficients come from scalar
 // slopes, then a per-lane select of the result (selecting the slope first
-// costs two moves and a select per half of the FP64 operand).  A separate
-// path for waves whose lanes all take one branch of the reference's sum --
-// one coefficient per edge, invalid edges skipped by a scalar branch --
-// measured slower here and no faster in the FP64 sum (the code triples).  Near-singular edges (|C| <= eps1,
-// their term below zero) are evaluated in one pass after the sum, taken
-// only when some lane has one.
+// costs two moves and a select per half of the FP64 operand).  An edge that
+// is invalid for the lane's branch carries kInvalidSlope, so its reciprocal
+// is zero and the sum needs no validity test.  A separate path for waves
+// whose lanes all take one branch of the reference's sum -- one coefficient
+// per edge, invalid edges skipped by a scalar branch -- measured slower here
+// and no faster in the FP64 sum (the code triples).  Near-singular edges
+// (|C| <= eps1, their term below zero) are evaluated in one pass after the
+// sum, taken only when some lane has one; the validity test lives there.
 template <int NV>
 __device__ __forceinline__ void edge_sum_mixed_nv(
     const double *syx, const double *sxy, const int *fl, const float *sv, const float *cv,
@@ -749,24 +751,22 @@ __device__ __forceinline__ void edge_sum_mixed_nv(
         const double cx = __builtin_fma(syx[e], B, A);
         const double cy = __builtin_fma(sxy[e], A, B);
         const double cc = useB ? cx : cy;
-        const bool valid = (fl[e] & vmask) != 0;
         const bool big = __builtin_fabs(cc) > eps1;
         // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
         // carry more rounding than that
         const float r = __builtin_amdgcn_rcpf((float)cc);
-        const float inv = (valid && big) ? r : 0.0f;
+        const float inv = big ? r : 0.0f;
         sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
         si = __builtin_fmaf(sv[f] - sv[e], inv, si);
         // near-singular lanes as scalar mask arithmetic (a second vector
         // compare for !big otherwise)
-        rare[e] = __builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(valid) &
-                                                      ~__builtin_amdgcn_ballot_w64(big));
+        rare[e] = __builtin_amdgcn_inverse_ballot_w64(~__builtin_amdgcn_ballot_w64(big));
         any_rare = any_rare || rare[e];
     }
     if (__builtin_expect(any_rare && !small, 0)) {
 #pragma unroll
         for (int e = 0; e < NV; ++e) {
-            if (rare[e]) {
+            if (rare[e] && (fl[e] & vmask) != 0) {
                 const int f = e + 1 < NV ? e + 1 : 0;
                 const double cc =
                     useB ? __builtin_fma(syx[e], B, A) : __builtin_fma(sxy[e], A, B);
@@ -797,6 +797,8 @@ __device__ __forceinline__ void aperture_mixed_nv(
     const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
     float &out_i)
 {
+    // a product sum, then one subtraction: an FMA chain ending on A0 would
+    // pair two scalar operands in one instruction and cost two moves
     const double A = h[H_A0] - (d[0] * h[H_HOR] + d[1] * h[H_HOR + 1] + d[2] * h[H_HOR + 2]);
     const double B = h[H_B0] - (d[0] * h[H_VER] + d[1] * h[H_VER + 1] + d[2] * h[H_VER + 2]);
     const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];

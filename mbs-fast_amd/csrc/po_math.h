@@ -26,6 +26,16 @@ constexpr double kSmallPhase = 1e-3;
 // fifth order (truncation below max|phase|^6 / 6! = 3.4e-7 there), where the
 // edge sum's error has fallen to ~1e-5.
 constexpr double kSmallPhaseMixed = 0.25;
+
+// Slope stored for an edge that is not valid in that direction (|dx| or
+// |dy| below the reference's geometric tolerance).  Its coefficient
+// A + s B (or s A + B) is then at least 1e300 times the lane's larger
+// projection, so the mixed kernels' FP32 reciprocal of it is exactly zero
+// and the edge drops out of the sum with no per-lane validity test; only a
+// lane with both projections below 1e-300 (every vertex phase tiny, so it
+// takes the moment form) sees a small coefficient.  The FP64 kernels mask
+// invalid edges by their flags and never use the value.
+constexpr double kInvalidSlope = 1e300;
 // One Horner step p*z + c as a single three-operand v_fma_f64 with the
 // coefficient in an SGPR pair.  Written as inline assembly because the
 // compiler otherwise keeps loop-invariant FP64 coefficients in VGPRs and
