@@ -659,7 +659,7 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
                                            const double *__restrict__ vdd,
                                            const int32_t *__restrict__ vflag, double A,
                                            double B, double psi, bool useB, int vmask,
-                                           const float2 *__restrict__ tab, double k,
+                                           const float4 *__restrict__ tab, double k,
                                            double eps1, float s0, float c0, float &sp,
                                            float &cp, float &sr, float &si, bool small)
 {
@@ -699,7 +699,7 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
 __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__ h, double A,
                                                       double B, double psi, bool useB,
                                                       bool small, float sr, float si,
-                                                      const float2 *__restrict__ tab,
+                                                      const float4 *__restrict__ tab,
                                                       float &out_r, float &out_i)
 {
     const float nrm = rcp_f32((float)(useB ? B : -A));
@@ -794,7 +794,7 @@ template <int NV>
 __device__ __forceinline__ void aperture_mixed_nv(
     const double *__restrict__ h, const double *__restrict__ vtx,
     const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
-    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
+    const double *d, const float4 *__restrict__ tab, double k, double eps1, float &out_r,
     float &out_i)
 {
     // a product sum, then one subtraction: an FMA chain ending on A0 would
@@ -816,7 +816,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
             ky[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KY]);
         }
         float dv[NV];
-        float2 ev[NV];
+        float4 ev[NV];
         // every vertex tested here: the per-beam bound that spares the FP64
         // kernel this test measured slower in this one
         small = true;
@@ -861,7 +861,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
 __device__ __forceinline__ void aperture_mixed(
     const double *__restrict__ h, const double *__restrict__ vtx,
     const double *__restrict__ vdd, const int32_t *__restrict__ vflag, int nv,
-    const double *d, const float2 *__restrict__ tab, double k, double eps1, float &out_r,
+    const double *d, const float4 *__restrict__ tab, double k, double eps1, float &out_r,
     float &out_i)
 {
     if (nv == 4)
@@ -878,7 +878,7 @@ __device__ __forceinline__ void beam_dir_mixed(
     const double *__restrict__ h, const float *__restrict__ h32,
     const double *__restrict__ vtx, const double *__restrict__ vdd,
     const int32_t *__restrict__ vflag, int nv, const double *d,
-    const float *__restrict__ su, const float2 *__restrict__ tab, double k, double eps1,
+    const float *__restrict__ su, const float4 *__restrict__ tab, double k, double eps1,
     float *j, float *jn)
 {
     float fr, fi;
@@ -1100,7 +1100,7 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     double *__restrict__ per_orient, int *__restrict__ queue)
 {
     using U = typename std::conditional<PREC == 1, float, double>::type;
-    using U2 = typename std::conditional<PREC == 1, float2, double2>::type;
+    using U2 = typename std::conditional<PREC == 1, float4, double2>::type;
     // LDS: this lane's rotation vectors u0, u1 ([12][256], read once per beam
     // after the vertex loop instead of occupying 12-24 VGPRs through it) and
     // the phasor table e^{i 2 pi j/TN} (512 FP32 entries, 256 FP64 ones)
@@ -1114,12 +1114,13 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
             return;
     }
     for (int i = threadIdx.x; i < TN; i += blockDim.x) {
-        double ts, tc;
-        sincospi((double)i / (TN / 2), &ts, &tc);
-        if constexpr (PREC == 1)
-            s_tab[i] = make_float2((float)tc, (float)ts);
-        else
+        if constexpr (PREC == 1) {
+            s_tab[i] = tab32_entry(i);
+        } else {
+            double ts, tc;
+            sincospi((double)i / (TN / 2), &ts, &tc);
             s_tab[i] = make_double2(tc, ts);
+        }
     }
     QueueCursor qc;
     for (;;) {
@@ -1283,7 +1284,8 @@ enum { G_LD = 25 };  // LDS row stride (floats) of the per-wave G transpose
 // five workgroups share a CU).  Called by every thread of the block.
 __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1,
                                               float *__restrict__ sg,
-                                              const float *__restrict__ su, float *j)
+                                              const double *__restrict__ u, int cells,
+                                              float *j)
 {
     const int lane = threadIdx.x & 63;
     const int n = lane & 31;
@@ -1302,7 +1304,7 @@ __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1
                 const int i = t >> 1, c = t & 1;
                 float re = 0.0f, im = 0.0f;
                 for (int m = 0; m < 6; ++m) {
-                    const float um = su[(6 * i + m) * 256 + threadIdx.x];
+                    const float um = (float)u[(size_t)(6 * i + m) * cells];
                     re = __builtin_fmaf(um, gr[4 * m + 2 * c], re);
                     im = __builtin_fmaf(um, gr[4 * m + 2 * c + 1], im);
                 }
@@ -1325,20 +1327,16 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const double *__restrict__ weights, double k, double eps1, double *__restrict__ out,
     double *__restrict__ out_ns, double *__restrict__ per_orient, int *__restrict__ queue)
 {
-    __shared__ float2 s_tab[kTabN32];
+    __shared__ float4 s_tab[kTabN32];
     __shared__ float s_g[4 * 32 * G_LD];
-    __shared__ float s_u[12 * 256];  // u0, u1 of each lane's direction
     __shared__ int s_unit;
     int item = 0, tile = 0;
     if constexpr (!QUEUE) {
         if (!map_block(n_items, n_tiles, item, tile))
             return;
     }
-    for (int i = threadIdx.x; i < kTabN32; i += blockDim.x) {
-        double ts, tc;
-        sincospi((double)i / (kTabN32 / 2), &ts, &tc);
-        s_tab[i] = make_float2((float)tc, (float)ts);
-    }
+    for (int i = threadIdx.x; i < kTabN32; i += blockDim.x)
+        s_tab[i] = tab32_entry(i);
     QueueCursor qc;
     for (;;) {
     if constexpr (QUEUE) {
@@ -1351,9 +1349,10 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     double d[3];
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
-    for (int q = 0; q < 12; ++q)
-        s_u[q * 256 + threadIdx.x] = (float)tab[(size_t)(DT_U0 + q) * cells + c];
-    __syncthreads();
+    // this lane's u0, u1, read by the epilogue straight from the grid tables
+    // (once per work item; staging them in LDS cost 12 KB per workgroup)
+    const double *u = tab + (size_t)DT_U0 * cells + c;
+    __syncthreads();  // the phasor table
     float *sg = s_g + (threadIdx.x >> 6) * 32 * G_LD;
     const int lane = threadIdx.x & 63;
     const WorkItem it = items[item];
@@ -1393,9 +1392,9 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
             }
         }
         float j[8], jn[8];
-        mfma_epilogue(g0, g1, sg, s_u, j);
+        mfma_epilogue(g0, g1, sg, u, cells, j);
         if constexpr (NS)
-            mfma_epilogue(n0, n1, sg, s_u, jn);
+            mfma_epilogue(n0, n1, sg, u, cells, jn);
         if (!live)
             continue;
         if constexpr (JONES) {
@@ -1420,7 +1419,7 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     }
     if constexpr (!QUEUE)
         return;
-    __syncthreads();  // the next unit overwrites s_u and s_g
+    __syncthreads();  // the next unit overwrites s_g
     }
 }
 

@@ -112,13 +112,25 @@ __device__ __forceinline__ void sincos_table(double x, const double2 *__restrict
 // again an exact scaling, and |x - j kTab32Step1| < 2^-6).  Against the
 // 256-entry table the index costs one more instruction (a mask and a shift
 // instead of one shift of the low byte) and the correction two fewer.
+//
+// An entry holds (cos, sin, cos/2, sin/2) of its angle (16 bytes, one LDS
+// read): with the halves at hand the correction and the product are four
+// FMAs (see sincos_finish_f32).
 constexpr int kTabN32 = 512;
 constexpr double kTab32InvStep = 81.487330863050411912;  // 512 / (2 pi)
 constexpr double kTab32Step1 = 3.141592653589793116 / 256.0;
 constexpr double kTab32Step2 = 1.2246467991473532e-16 / 256.0;
 
-__device__ __forceinline__ void sincos_reduce_f32(double x, const float2 *__restrict__ tab,
-                                                  float &d, float2 &e)
+// Entry j of the FP32 table.
+__device__ __forceinline__ float4 tab32_entry(int j)
+{
+    double ts, tc;
+    sincospi((double)j / (kTabN32 / 2), &ts, &tc);
+    return make_float4((float)tc, (float)ts, (float)(0.5 * tc), (float)(0.5 * ts));
+}
+
+__device__ __forceinline__ void sincos_reduce_f32(double x, const float4 *__restrict__ tab,
+                                                  float &d, float4 &e)
 {
     const double t = __builtin_fma(x, kTab32InvStep, kRoundMagic);
     const double j = t - kRoundMagic;
@@ -131,19 +143,20 @@ __device__ __forceinline__ void sincos_reduce_f32(double x, const float2 *__rest
 // |d| <= pi/512 = 6.14e-3, so e^{i d} = (1 - d^2/2) + i d to FP32 accuracy:
 // the dropped terms are below 3.9e-8 (d^3/6) and 6e-11 (d^4/24), within an
 // ulp of FP32 near 1 (6e-8) -- the phasor rounding the mixed kernels carry
-// anyway.
-__device__ __forceinline__ void sincos_finish_f32(float d, float2 e, float &s, float &c)
+// anyway.  With T = (c, s) the entry,
+//   Im T e^{id} = s + d (c - d s/2),   Re T e^{id} = c - d (s + d c/2),
+// two FMAs each on the stored halves.
+__device__ __forceinline__ void sincos_finish_f32(float d, float4 e, float &s, float &c)
 {
-    const float cd = __builtin_fmaf(d * d, -0.5f, 1.0f);
-    s = __builtin_fmaf(e.y, cd, e.x * d);
-    c = __builtin_fmaf(e.x, cd, -(e.y * d));
+    s = __builtin_fmaf(d, __builtin_fmaf(-d, e.w, e.x), e.y);
+    c = __builtin_fmaf(-d, __builtin_fmaf(d, e.z, e.y), e.x);
 }
 
-__device__ __forceinline__ void sincos_table_f32(double x, const float2 *__restrict__ tab,
+__device__ __forceinline__ void sincos_table_f32(double x, const float4 *__restrict__ tab,
                                                  float &s, float &c)
 {
     float d;
-    float2 e;
+    float4 e;
     sincos_reduce_f32(x, tab, d, e);
     sincos_finish_f32(d, e, s, c);
 }
