@@ -255,6 +255,66 @@ def bench_aperture_fft(ctx, batch, args):
                          "traffic": measured_traffic().get("fft")}}
 
 
+def bench_near_forward(eng, abi, args):
+    """The large-particle near-forward workload (hexagonal column D=200 um,
+    L=400 um, m=1.3116+1e-3i, 12 reflections; the reference tracer's beams of
+    4 Sobol orientations, tests/golden/config5_near_forward.npz; theta 0-5 deg
+    at 0.005 deg x 48 phi) through both F sources: the direct integral for
+    every beam, and the spectra (mbs_spectrum_source) for the near-forward
+    beams.  The spectrum path's time per call is the source's (scan, raster +
+    FFT, sampling; it depends on batch and grid only) plus its diffraction
+    kernel.  The crossover is the number of grid cells per selected beam at
+    which the source's fixed cost per beam equals what it saves per cell."""
+    fx = dict(np.load(os.path.join(ROOT, "tests", "golden", "config5_near_forward.npz")))
+    batch = abi.BeamBatch.from_fixture(fx)
+    consts = abi.consts_from_fixture(fx)
+    theta = np.radians(np.round(np.arange(0, 1001) * 0.005, 10))
+    grid = abi.Grid.build(theta, 48)
+    units = float(batch.n_beams) * grid.cells
+    ctx = eng.Context(0)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_timing(True)
+
+    def timed(flags):
+        for _ in range(max(1, args.warmup)):
+            ctx.diffract(flags)
+        ks = []
+        for _ in range(max(3, min(args.steps, 10))):
+            ctx.diffract(flags)
+            ks.append(ctx.timing()["kernel_ms"])
+        return float(np.median(ks))
+
+    direct_ms = timed(abi.PREC_MIXED)
+    info = ctx.spectrum_source(a_max=0.1, oversample=8.0)
+    source_ms = info["scan_ms"] + info["fft_ms"] + info["sample_ms"]
+    spec_ms = timed(abi.PREC_MIXED | abi.SPECTRUM)
+    ctx.close()
+    n_sel = max(info["n_selected"], 1)
+    saved_per_cell = (direct_ms - spec_ms) / (n_sel * grid.cells)    # ms per (beam, cell)
+    sample_per_cell = info["sample_ms"] / (n_sel * grid.cells)
+    fixed_per_beam = (info["scan_ms"] + info["fft_ms"]) / n_sel
+    gain = saved_per_cell - sample_per_cell
+    return {
+        "workload": "hexagonal column D=200 um L=400 um, m=1.3116+1e-3i, 12 reflections, "
+                    "4 orientations (%d traced beams), theta 0-5 deg at 0.005 deg x 48 phi"
+                    % batch.n_beams,
+        "direct": {"value": units / (direct_ms * 1e-3), "kernel_ms": direct_ms},
+        "spectrum": {"value": units / ((source_ms + spec_ms) * 1e-3),
+                     "kernel_ms": spec_ms, "source_ms": source_ms,
+                     "selected_beams": info["n_selected"],
+                     "selected_area_share": info["selected_area"] / info["total_area"],
+                     "fft_sides": {str(k): v for k, v in info["n_by_size"].items() if v},
+                     "spectrum_MB": info["spectrum_bytes"] / 1e6,
+                     "scan_ms": info["scan_ms"], "fft_ms": info["fft_ms"],
+                     "sample_ms": info["sample_ms"]},
+        "unit": "beam.dir evals/s",
+        "crossover_cells_per_beam": (fixed_per_beam / gain) if gain > 0 else None,
+        "note": "crossover: cells per selected beam above which the spectrum source would "
+                "pay for itself (null: sampling a cell costs more than the direct integral)",
+    }
+
+
 def run(args):
     import mbs_loader
     pkg = mbs_loader.load()
@@ -356,6 +416,7 @@ def run(args):
                                  "tuning_s": tune_s}
     fft = bench_aperture_fft(ctx, batch, args)
     ctx.close()
+    near = bench_near_forward(eng, abi, args)
     hist = np.bincount(batch.beam_nv, minlength=32)
     out = {
         "metric": "beam.dir evals/s",
@@ -409,6 +470,7 @@ def run(args):
                         for k in ("value", "ms_per_step", "kernel_ms", "work_items",
                                   "workgroups")},
         "aperture_fft": fft,
+        "near_forward": near,
     }
     if not args.no_cpu:
         port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)

@@ -127,6 +127,12 @@ typedef struct mbs_consts {
                                  (theta_m11_kernel, GpuDiffraction.cu:1909-1957).
                                  Not with MBS_PREC_MIXED, MBS_NOSHADOW or
                                  MBS_PER_ORIENT (MBS_ERR_INVALID)               */
+#define MBS_SPECTRUM 64u      /* mixed precision (MFMA contraction): the beams
+                                 mbs_spectrum_source selected take F from
+                                 their FFT spectra, the others the direct
+                                 integral.  Not with MBS_PER_ORIENT or
+                                 MBS_INCOHERENT; needs a current
+                                 mbs_spectrum_source (MBS_ERR_INVALID)         */
 
 typedef struct mbs_ctx mbs_ctx;
 
@@ -341,6 +347,54 @@ int mbs_aperture_fft_spectrum(mbs_ctx *ctx, int32_t index, float *out);
 /* With timing enabled: device times of the last mbs_aperture_fft's raster+row
  * pass, column pass and sampling. */
 int mbs_aperture_fft_timing(mbs_ctx *ctx, float *rows_ms, float *cols_ms, float *sample_ms);
+
+/* ---- near-forward F source ----------------------------------------------
+ * mbs_spectrum_source selects, for the uploaded batch on the current grid,
+ * the beams whose scattering directions all lie close to their own
+ * (max over the grid of max(|A|, |B|) at most a_max), whose polygon is
+ * convex with at most 16 vertices, and whose spectrum fits an FFT of at most
+ * max_n points per side; for each it rasterises the aperture (tent-weighted:
+ * every pixel the polygon integral of a tent of half-width h around it,
+ * exact), transforms it, and samples F at every cell of the grid by
+ * non-uniform FFT interpolation (exponential-of-semicircle kernel, 6 taps
+ * per axis, oversampling 2).  mbs_diffract with MBS_SPECTRUM then uses those
+ * F values for the selected beams.  The pixel size is h = min(pi / (k a_b
+ * oversample), extent / 32, width / 16), a_b the beam's own max(|A|, |B|)
+ * (so |f h| <= 1/(2 oversample) on the grid), extent the larger side of its
+ * bounding box and width its least width, with halved oversampling (down
+ * to 2) when the FFT would exceed max_n.  The grid side is the power of two
+ * >= 2 (extent + 4 h) / h, at least min_n.  This restates the reference's
+ * aperture probe
+ * (tools/fft_aperture_probe.cu: 0/1 raster :130-169, 2-D FFT :549-551,
+ * Lanczos sampling :50-102) with an anti-aliased raster and a non-uniform
+ * FFT interpolation, composed into the coherent sum.  A new upload, rescale
+ * or grid drops the selection.  p = NULL drops it too.
+ * Device memory: sum over selected beams of n^2 * 8 bytes (spectra) plus
+ * selected beams * cells * 8 bytes (F). */
+typedef struct mbs_spectrum_params {
+    double a_max;       /* selection bound on max(|A|, |B|) over the grid     */
+    double oversample;  /* pixels per half period of the highest frequency    */
+    int32_t min_n;      /* smallest FFT side (>= MBS_FFT_MIN_N)               */
+    int32_t max_n;      /* largest FFT side (<= MBS_FFT_MAX_N)                */
+} mbs_spectrum_params;
+typedef struct mbs_spectrum_info {
+    int32_t n_selected;       /* beams on the spectrum path                    */
+    int32_t n_by_size[6];     /* of which with FFT side 64, 128, ..., 2048     */
+    int32_t reserved;
+    int64_t spectrum_bytes;   /* device memory of the spectra                  */
+    double selected_area;     /* sum of the selected beams' aperture areas     */
+    double total_area;        /* sum over all beams                            */
+    float scan_ms, fft_ms, sample_ms;  /* device times (timing enabled)        */
+    float reserved2;
+} mbs_spectrum_info;
+int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum_info *info);
+/* The current selection: beams[n_selected] (indices into the uploaded batch,
+ * in slot order) and, if f is not NULL, f[n_selected][cells][2] the F values
+ * (re, im) the spectrum variant uses, cells in the device's theta-major order
+ * (cell t * n_az + p), in the kernels' convention
+ * s k^2 I(A, B) e^{i psi} (that is, F / complWave of mbs_diffract's aperture
+ * factor). */
+int mbs_spectrum_values(mbs_ctx *ctx, int32_t *beams, float *f);
 
 /* Learned control variate: a multilayer perceptron trained on the device
  * (no counterpart in the reference).  Layers: n_in -> depth x width (GELU,

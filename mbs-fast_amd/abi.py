@@ -33,6 +33,7 @@ NOSHADOW = 4
 PER_ORIENT = 8
 NO_MFMA = 16
 INCOHERENT = 32
+SPECTRUM = 64
 
 ORIENT_SOBOL, ORIENT_HAMMERSLEY, ORIENT_SHOEMAKE = 0, 1, 2
 ORIENT_STRIDE = 16
@@ -91,6 +92,18 @@ class mbs_device_info(C.Structure):
 class mbs_timing(C.Structure):
     _fields_ = [("upload_ms", C.c_float), ("kernel_ms", C.c_float),
                 ("reduce_ms", C.c_float), ("n_items", C.c_int32), ("n_blocks", C.c_int32)]
+
+
+class mbs_spectrum_params(C.Structure):
+    _fields_ = [("a_max", C.c_double), ("oversample", C.c_double), ("min_n", C.c_int32),
+                ("max_n", C.c_int32)]
+
+
+class mbs_spectrum_info(C.Structure):
+    _fields_ = [("n_selected", C.c_int32), ("n_by_size", C.c_int32 * 6), ("reserved", C.c_int32),
+                ("spectrum_bytes", C.c_int64), ("selected_area", C.c_double),
+                ("total_area", C.c_double), ("scan_ms", C.c_float), ("fft_ms", C.c_float),
+                ("sample_ms", C.c_float), ("reserved2", C.c_float)]
 
 
 class mbs_tune_entry(C.Structure):

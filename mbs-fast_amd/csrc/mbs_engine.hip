@@ -24,6 +24,7 @@
 #include "fft_kernels.hip"
 #include "mlp_kernels.hip"
 #include "upsample_kernels.hip"
+#include "spectrum_kernels.hip"
 
 using namespace mbs;
 
@@ -102,6 +103,15 @@ struct mbs_ctx {
     int fft_n = 0, fft_batch = 0;
     float fft_ms[3] = {0, 0, 0};
     hipEvent_t fft_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // near-forward spectrum source (mbs_spectrum_source): the selected
+    // beams' slots, records, spectra and F at every cell
+    bool spec_ok = false;
+    int spec_n = 0;
+    double pack_scale = 1.0;  // size scale of the packed beams (mbs_rescale_beams)
+    DevBuf<int32_t> spec_sidx, spec_beam, spec_logn;
+    DevBuf<int64_t> spec_off;
+    DevBuf<double> spec_scan, spec_h, spec_rec, spec_deap;
+    DevBuf<float2> spec_grid, spec_f;
     std::vector<WorkItem> h_items;
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
@@ -385,6 +395,7 @@ int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *g)
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     ctx->grid_ok = false;
     ctx->have_result = false;
+    ctx->spec_ok = false;
     ctx->n_az = g->n_az;
     ctx->n_zen1 = g->n_zen + 1;
     ctx->cells = (int)cells;
@@ -445,6 +456,8 @@ static int upload_impl(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts 
     if (!ctx || !bt || !c)
         return fail(ctx, MBS_ERR_INVALID, "mbs_upload_beams: NULL argument");
     ctx->beams_ok = false;
+    ctx->spec_ok = false;
+    ctx->pack_scale = 1.0;
     ctx->scaling_ok = false;
     ctx->have_result = false;
     if (bt->n_orient < 0 || bt->n_beams < 0 || (bt->n_orient > 0 && !bt->orient_offsets)
@@ -713,11 +726,13 @@ extern "C" int mbs_rescale_beams(mbs_ctx *ctx, double scale)
     if (!(scale > 0) || !std::isfinite(scale))
         return fail(ctx, MBS_ERR_INVALID, "scale must be positive");
     ctx->have_result = false;
+    ctx->spec_ok = false;
     if (ctx->n_beams == 0)
         return MBS_OK;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     const int rc = launch_pack(ctx, ctx->n_beams, ctx->consts, scale,
                                std::fabs(scale - 1.0) > 1e-15);
+    ctx->pack_scale = std::fabs(scale - 1.0) > 1e-15 ? scale : 1.0;
     if (rc)
         return rc;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -785,7 +800,12 @@ static bool plan_items(mbs_ctx *ctx, int n_tiles, int target, int work_mode,
 template <int PREC, int NS, int JONES, int PER, int Q>
 static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 {
-    if constexpr (PREC == 3)
+    if constexpr (PREC == 4)
+        diffract_mfma_kernel<NS, JONES, PER, Q, 1><<<blocks, 256, 0, s>>>(
+            a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
+            a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
+            a.per_orient, a.queue, a.sidx, a.fbuf);
+    else if constexpr (PREC == 3)
         diffract_incoherent_kernel<<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.vtx, a.vdd, a.vflag, a.nv,
             a.voff, a.ooff, a.weights, a.k, a.eps1, a.out);
@@ -821,6 +841,7 @@ static void launch_diffract(int prec, int ns, int jones, int per, bool queue,
     MBS_L(3, 0, 0, 0, 0)
     MBS_L(0, 0, 1, 0, 1) MBS_L(0, 1, 1, 0, 1) MBS_L(1, 0, 1, 0, 1) MBS_L(1, 1, 1, 0, 1)
     MBS_L(2, 0, 1, 0, 1) MBS_L(2, 1, 1, 0, 1)
+    MBS_L(4, 0, 0, 0, 0) MBS_L(4, 1, 0, 0, 0) MBS_L(4, 0, 1, 0, 0) MBS_L(4, 1, 1, 0, 0)
 #undef MBS_L
 }
 
@@ -900,8 +921,14 @@ static void batch_args(const mbs_ctx *ctx, int prec, int n_items, int n_tiles, K
     // switch to the exact half-angle edge form happens at |C| <= 3e-5, the
     // floor the reference applies to its FP32 build (GpuDiffraction.cu:498-512).
     // The FP64 kernels (prec 0, and 3 = the FP64 incoherent sum) keep the
-    // handler's eps1.
-    a.eps1 = (prec == 1 || prec == 2) ? std::max(ctx->consts.eps1, 3e-5) : ctx->consts.eps1;
+    // handler's eps1.  (prec 4: the mixed MFMA kernel with the spectrum
+    // source.)
+    a.eps1 = (prec == 1 || prec == 2 || prec == 4) ? std::max(ctx->consts.eps1, 3e-5)
+                                                   : ctx->consts.eps1;
+    if (prec == 4) {
+        a.sidx = ctx->spec_sidx.p;
+        a.fbuf = ctx->spec_f.p;
+    }
 }
 
 extern "C" {
@@ -920,15 +947,24 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     const mbs_ctx::Tuned &tn = ctx->tuned[base];
     const bool mfma = !(flags & MBS_NO_MFMA) && !(tn.valid && !tn.mfma);
     const bool incoh = (flags & MBS_INCOHERENT) != 0;
-    const int prec = incoh ? 3 : base ? (mfma ? 2 : 1) : 0;
+    const bool spec = (flags & MBS_SPECTRUM) != 0;
+    const int prec = incoh ? 3 : spec ? 4 : base ? (mfma ? 2 : 1) : 0;
     const int ns = (flags & MBS_NOSHADOW) ? 1 : 0;
     const int per = (flags & MBS_PER_ORIENT) ? 1 : 0;
     if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW | MBS_PER_ORIENT | MBS_NO_MFMA |
-                            MBS_INCOHERENT))
+                            MBS_INCOHERENT | MBS_SPECTRUM))
         return fail(ctx, MBS_ERR_INVALID, "unknown flag bits 0x%x", flags);
     if (incoh && (base || ns || per))
         return fail(ctx, MBS_ERR_INVALID,
                     "MBS_INCOHERENT runs in FP64 without MBS_NOSHADOW or MBS_PER_ORIENT");
+    if (spec && (!base || per || incoh || (flags & MBS_NO_MFMA)))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "MBS_SPECTRUM runs in mixed precision on the MFMA contraction, without "
+                    "MBS_PER_ORIENT");
+    if (spec && !ctx->spec_ok)
+        return fail(ctx, MBS_ERR_INVALID,
+                    "MBS_SPECTRUM: no spectrum source for this batch and grid "
+                    "(mbs_spectrum_source)");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     int rc = ensure_tables(ctx);
     if (rc)
@@ -939,7 +975,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
     const int mode = tn.valid ? tn.work_mode : ctx->work_mode;
     // the device queue plans beam segments on the device; the incoherent sum
     // and per-orientation grids keep the host plan
-    const bool queue = mode == MBS_WORK_QUEUE && !incoh && !per && ctx->n_beams > 0;
+    const bool queue = mode == MBS_WORK_QUEUE && !incoh && !per && !spec && ctx->n_beams > 0;
     bool jones = true;
     int n_items;
     if (queue) {
@@ -1553,6 +1589,229 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
     if (n_points > 0)
         return fetch(ctx, ctx->fft_out.p, (size_t)n_beams * n_points * 2, out, false);
     HIP_TRY(ctx, hipStreamSynchronize(s));
+    return MBS_OK;
+}
+
+}  // extern "C"
+
+// Raster + row pass and column pass of one group of spectrum beams (all of
+// grid side 2^LOGN).
+template <int LOGN>
+static int launch_spectrum_fft(mbs_ctx *ctx, int nb, const double *rec, float2 *grid)
+{
+    constexpr int N = 1 << LOGN;
+    const size_t lds_rows = (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2);
+    const size_t lds_cols = (size_t)FFT_COLS * fft_ld(N) * sizeof(float2);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)spectrum_rows_kernel<LOGN>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<LOGN>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
+    hipStream_t s = ctx->stream;
+    const int rb = (nb * (N / FFT_ROWS) + 7) / 8 * 8;
+    const int cb = (nb * (N / FFT_COLS) + 7) / 8 * 8;
+    spectrum_rows_kernel<LOGN><<<rb, FFT_ROW_THREADS, lds_rows, s>>>(nb, rec, ctx->spec_deap.p,
+                                                                    ctx->fft_tw.p, grid);
+    HIP_TRY(ctx, hipGetLastError());
+    fft_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(nb, ctx->fft_tw.p, grid);
+    HIP_TRY(ctx, hipGetLastError());
+    return MBS_OK;
+}
+
+extern "C" {
+
+int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum_info *info)
+{
+    if (!ctx)
+        return fail(nullptr, MBS_ERR_INVALID, "ctx is NULL");
+    ctx->spec_ok = false;
+    ctx->have_result = false;
+    if (!p)
+        return MBS_OK;
+    if (!ctx->grid_ok)
+        return fail(ctx, MBS_ERR_NO_GRID, "mbs_set_grid has not been called");
+    if (!ctx->beams_ok)
+        return fail(ctx, MBS_ERR_NO_BEAMS, "mbs_upload_beams has not been called");
+    const auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (!(p->a_max > 0.0) || !(p->oversample >= 2.0) || !pow2(p->min_n) || !pow2(p->max_n) ||
+        p->min_n < MBS_FFT_MIN_N || p->max_n > MBS_FFT_MAX_N || p->min_n > p->max_n)
+        return fail(ctx, MBS_ERR_INVALID,
+                    "spectrum source: a_max > 0, oversample >= 2, min_n <= max_n powers of two "
+                    "in %d..%d", MBS_FFT_MIN_N, MBS_FFT_MAX_N);
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    int limit = 0;
+    HIP_TRY(ctx, hipDeviceGetAttribute(&limit, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                       ctx->device));
+    if ((size_t)limit < (size_t)FFT_COLS * fft_ld(p->max_n) * sizeof(float2))
+        return fail(ctx, MBS_ERR_CAPACITY, "FFT side %d needs more LDS than the device has",
+                    p->max_n);
+    int rc = ensure_tables(ctx);
+    if (rc)
+        return rc;
+    mbs_spectrum_info inf{};
+    const int nb = ctx->n_beams, cells = ctx->cells;
+    const double k = ctx->consts.wave_index;
+    hipStream_t s = ctx->stream;
+    for (hipEvent_t &e : ctx->fft_ev)
+        if (!e)
+            HIP_TRY(ctx, hipEventCreate(&e));
+    HIP_TRY(ctx, ctx->spec_sidx.ensure(nb));
+    std::vector<int32_t> sidx(nb, -1);
+    std::vector<double> scan((size_t)nb * 5);
+    std::vector<int32_t> nv(nb);
+    std::vector<double> area(nb);
+    if (nb > 0) {
+        HIP_TRY(ctx, ctx->spec_scan.ensure((size_t)nb * 5));
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[0], s));
+        spectrum_scan_kernel<<<nb, 256, 0, s>>>(nb, ctx->hdr.p, ctx->tab.p, cells, ctx->nv.p,
+                                                ctx->voff.p, ctx->vx_in.p, ctx->vy_in.p,
+                                                ctx->pack_scale, ctx->spec_scan.p);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
+        HIP_TRY(ctx, hipMemcpyAsync(scan.data(), ctx->spec_scan.p, scan.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(nv.data(), ctx->nv.p, nb * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, s));
+        // the aperture areas (field 12 of the reference's beam scalars)
+        HIP_TRY(ctx, hipMemcpy2DAsync(area.data(), sizeof(double), ctx->bs_in.p + 12,
+                                      MBS_BEAM_SCALARS * sizeof(double), sizeof(double), nb,
+                                      hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        HIP_TRY(ctx, hipEventElapsedTime(&inf.scan_ms, ctx->fft_ev[0], ctx->fft_ev[1]));
+    }
+    // selection, grouped by FFT side (slots in ascending side order)
+    struct Pick {
+        int beam, logn;
+        double h;
+    };
+    std::vector<Pick> picks;
+    for (int b = 0; b < nb; ++b) {
+        const double *sc = &scan[(size_t)b * 5];
+        const double amax = sc[0], ext = sc[1], width = sc[4];
+        area[b] *= ctx->pack_scale * ctx->pack_scale;
+        inf.total_area += std::fabs(area[b]);
+        if (sc[2] == 0.0 || nv[b] > kSpecMaxNv || !(amax <= p->a_max) || !(amax > 0.0) ||
+            !(width > 0.0))
+            continue;
+        for (double os = p->oversample; os >= 2.0; os *= 0.5) {
+            // enough pixels across the polygon in every direction (see
+            // kSpecPixelsAcross)
+            const double h = std::min({kPi / (k * amax * os), ext / kSpecPixelsAcross,
+                                       width / kSpecPixelsWidth});
+            const double need = 2.0 * (ext + 4.0 * h) / h;
+            int logn = 0;
+            while ((1 << logn) < need || (1 << logn) < p->min_n)
+                ++logn;
+            if ((1 << logn) <= p->max_n) {
+                picks.push_back(Pick{b, logn, h});
+                inf.selected_area += std::fabs(area[b]);
+                break;
+            }
+        }
+    }
+    std::stable_sort(picks.begin(), picks.end(),
+                     [](const Pick &x, const Pick &y) { return x.logn < y.logn; });
+    const int ns = (int)picks.size();
+    std::vector<int32_t> sb(ns), sl(ns);
+    std::vector<double> sh(ns);
+    std::vector<int64_t> so(ns);
+    int64_t total = 0;
+    int max_n = 0;
+    for (int i = 0; i < ns; ++i) {
+        sb[i] = picks[i].beam;
+        sl[i] = picks[i].logn;
+        sh[i] = picks[i].h;
+        so[i] = total;
+        sidx[picks[i].beam] = i;
+        total += (int64_t)1 << (2 * picks[i].logn);
+        max_n = std::max(max_n, 1 << picks[i].logn);
+        ++inf.n_by_size[picks[i].logn - 6];
+    }
+    inf.n_selected = ns;
+    inf.spectrum_bytes = total * (int64_t)sizeof(float2);
+    HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_sidx.p, sidx.data(), nb * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
+    if (ns > 0) {
+        HIP_TRY(ctx, ctx->spec_beam.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_logn.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_h.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_off.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_rec.ensure((size_t)ns * SP_STRIDE));
+        HIP_TRY(ctx, ctx->spec_grid.ensure((size_t)total));
+        HIP_TRY(ctx, ctx->spec_f.ensure((size_t)ns * cells));
+        HIP_TRY(ctx, ctx->fft_tw.ensure(max_n));
+        HIP_TRY(ctx, ctx->spec_deap.ensure(max_n));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_beam.p, sb.data(), ns * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_logn.p, sl.data(), ns * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_h.p, sh.data(), ns * sizeof(double),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_off.p, so.data(), ns * sizeof(int64_t),
+                                    hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
+        spectrum_setup_kernel<<<(ns + 63) / 64, 64, 0, s>>>(ns, ctx->spec_beam.p, ctx->spec_h.p,
+                                                            ctx->nv.p, ctx->voff.p,
+                                                            ctx->vx_in.p, ctx->vy_in.p,
+                                                            ctx->pack_scale, ctx->spec_rec.p);
+        HIP_TRY(ctx, hipGetLastError());
+        for (int i0 = 0; i0 < ns;) {
+            int i1 = i0;
+            while (i1 < ns && sl[i1] == sl[i0])
+                ++i1;
+            const int n = 1 << sl[i0];
+            fft_twiddle_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, ctx->fft_tw.p);
+            spectrum_deap_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, ctx->spec_deap.p);
+            HIP_TRY(ctx, hipGetLastError());
+            const double *rec = ctx->spec_rec.p + (size_t)i0 * SP_STRIDE;
+            float2 *grid = ctx->spec_grid.p + so[i0];
+            switch (sl[i0]) {
+            case 6: rc = launch_spectrum_fft<6>(ctx, i1 - i0, rec, grid); break;
+            case 7: rc = launch_spectrum_fft<7>(ctx, i1 - i0, rec, grid); break;
+            case 8: rc = launch_spectrum_fft<8>(ctx, i1 - i0, rec, grid); break;
+            case 9: rc = launch_spectrum_fft<9>(ctx, i1 - i0, rec, grid); break;
+            case 10: rc = launch_spectrum_fft<10>(ctx, i1 - i0, rec, grid); break;
+            case 11: rc = launch_spectrum_fft<11>(ctx, i1 - i0, rec, grid); break;
+            default: rc = fail(ctx, MBS_ERR_INVALID, "FFT side %d", n);
+            }
+            if (rc)
+                return rc;
+            i0 = i1;
+        }
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[2], s));
+        const size_t units = (size_t)ns * cells;
+        spectrum_sample_kernel<<<(unsigned)((units + 255) / 256), 256, 0, s>>>(
+            ns, cells, ctx->spec_beam.p, ctx->spec_off.p, ctx->spec_logn.p, ctx->spec_rec.p,
+            ctx->spec_grid.p, ctx->hdr.p, ctx->tab.p, k, ctx->spec_f.p);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[3], s));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->fft_ev[3]));
+        HIP_TRY(ctx, hipEventElapsedTime(&inf.fft_ms, ctx->fft_ev[1], ctx->fft_ev[2]));
+        HIP_TRY(ctx, hipEventElapsedTime(&inf.sample_ms, ctx->fft_ev[2], ctx->fft_ev[3]));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    ctx->spec_n = ns;
+    ctx->spec_ok = true;
+    if (info)
+        *info = inf;
+    return MBS_OK;
+}
+
+int mbs_spectrum_values(mbs_ctx *ctx, int32_t *beams, float *f)
+{
+    if (!ctx || !beams)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_spectrum_values: NULL argument");
+    if (!ctx->spec_ok)
+        return fail(ctx, MBS_ERR_INVALID, "no spectrum source (mbs_spectrum_source)");
+    const int ns = ctx->spec_n;
+    if (ns == 0)
+        return MBS_OK;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipMemcpyAsync(beams, ctx->spec_beam.p, ns * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    if (f)
+        HIP_TRY(ctx, hipMemcpyAsync(f, ctx->spec_f.p, (size_t)ns * ctx->cells * sizeof(float2),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return MBS_OK;
 }
 

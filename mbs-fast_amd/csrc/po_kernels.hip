@@ -1058,6 +1058,8 @@ struct KernelArgs {
     double *out_ns;          // same shape, no-shadow (NS only)
     double *per_orient;      // [n_orient][cells][16] (PER only, Mueller mode)
     int *queue;              // work-queue header (queue mode), else null
+    const int32_t *sidx;     // [n_beams] spectrum slot of each beam, -1: direct
+    const float2 *fbuf;      // [spectrum beams][cells] F from the spectra
 };
 
 // Mueller of one orientation's Jones sum, weighted, stored to (first) or
@@ -1316,7 +1318,9 @@ __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1
     }
 }
 
-template <int NS, int JONES, int PER, int QUEUE>
+// FSRC: beams with sidx[b] >= 0 take F from fbuf (the near-forward spectrum
+// source) instead of the aperture integral.
+template <int NS, int JONES, int PER, int QUEUE, int FSRC = 0>
 __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,
@@ -1325,7 +1329,8 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const int32_t *__restrict__ nv, const int64_t *__restrict__ voff,
     const int32_t *__restrict__ ext, const int64_t *__restrict__ ooff,
     const double *__restrict__ weights, double k, double eps1, double *__restrict__ out,
-    double *__restrict__ out_ns, double *__restrict__ per_orient, int *__restrict__ queue)
+    double *__restrict__ out_ns, double *__restrict__ per_orient, int *__restrict__ queue,
+    const int32_t *__restrict__ sidx = nullptr, const float2 *__restrict__ fbuf = nullptr)
 {
     __shared__ float4 s_tab[kTabN32];
     __shared__ float s_g[4 * 32 * G_LD];
@@ -1376,9 +1381,16 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
                 nv_next = nv[b + 1];
             }
             float fr, fi;
-            aperture_mixed(hdr + (size_t)b * HDR_STRIDE, vtx + (size_t)v0 * VTX_STRIDE,
-                           vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nvb, d, s_tab, k, eps1,
-                           fr, fi);
+            const int slot = FSRC ? __builtin_amdgcn_readfirstlane(sidx[b]) : -1;
+            if (FSRC && slot >= 0) {
+                const float2 f = fbuf[(size_t)slot * cells + c];
+                fr = f.x;
+                fi = f.y;
+            } else {
+                aperture_mixed(hdr + (size_t)b * HDR_STRIDE, vtx + (size_t)v0 * VTX_STRIDE,
+                               vdd + (size_t)v0 * VDD_STRIDE, vflag + v0, nvb, d, s_tab, k,
+                               eps1, fr, fi);
+            }
             // A operands: lanes 0-31 (Re F), lanes 32-63 (Im F) of the
             // wave's directions 0-31, then the same for directions 32-63
             const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(fr),

@@ -79,6 +79,11 @@ def lib():
         L.mbs_aperture_fft.argtypes = [vp, C.POINTER(C.c_int64), C.c_int32, C.c_int32,
                                        C.c_double, C.c_int32, dp, C.c_int32, dp]
         L.mbs_aperture_fft.restype = C.c_int
+        L.mbs_spectrum_source.argtypes = [vp, C.POINTER(abi.mbs_spectrum_params),
+                                          C.POINTER(abi.mbs_spectrum_info)]
+        L.mbs_spectrum_source.restype = C.c_int
+        L.mbs_spectrum_values.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+        L.mbs_spectrum_values.restype = C.c_int
         L.mbs_aperture_fft_spectrum.argtypes = [vp, C.c_int32, C.POINTER(C.c_float)]
         L.mbs_aperture_fft_spectrum.restype = C.c_int
         L.mbs_aperture_fft_timing.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -306,6 +311,31 @@ class Context:
             self._h, beams.ctypes.data_as(C.POINTER(C.c_int64)), len(beams), int(n),
             float(pad), len(pts), _d(pts), int(radius), _d(out)))
         return out[..., 0] + 1j * out[..., 1]
+
+    def spectrum_source(self, a_max=0.1, oversample=8.0, min_n=64, max_n=2048):
+        """mbs_spectrum_source on the uploaded batch and the current grid: the
+        near-forward beams' F from their spectra for mbs_diffract(... |
+        abi.SPECTRUM).  Returns the selection summary as a dict."""
+        p = abi.mbs_spectrum_params(float(a_max), float(oversample), int(min_n), int(max_n))
+        info = abi.mbs_spectrum_info()
+        self._check(self._L.mbs_spectrum_source(self._h, C.byref(p), C.byref(info)))
+        out = {f: getattr(info, f) for f, _ in abi.mbs_spectrum_info._fields_
+               if not f.startswith("reserved")}
+        out["n_by_size"] = {64 << i: int(v) for i, v in enumerate(info.n_by_size)}
+        return out
+
+    def spectrum_clear(self):
+        self._check(self._L.mbs_spectrum_source(self._h, None, None))
+
+    def spectrum_values(self, n_selected):
+        """(beams [n_selected], F [n_selected, cells] complex64) of the
+        current selection, F in the kernels' convention s k^2 I e^{i psi}."""
+        beams = np.zeros(int(n_selected), np.int32)
+        f = np.zeros((int(n_selected), self.grid.cells, 2), np.float32)
+        self._check(self._L.mbs_spectrum_values(
+            self._h, beams.ctypes.data_as(C.POINTER(C.c_int32)),
+            f.ctypes.data_as(C.POINTER(C.c_float))))
+        return beams, f[..., 0] + 1j * f[..., 1]
 
     def aperture_fft_spectrum(self, index, n):
         out = np.zeros((n, n, 2), dtype=np.float32)
