@@ -174,9 +174,45 @@ def cpu_baseline_reference(budget_s=12.0):
                 "kind": "reference",
                 "sample": "reference CLI, %d Sobol orientations of the D=50 um column "
                           "(%d beams, %.1f per orientation) x 541x96 directions = %.3g units; "
-                          "phase-2 (diffraction) %.2f s" % (n, beams, beams / n, units, t2)}
+                          "phase-2 (diffraction) %.2f s" % (n, beams, beams / n, units, t2)}, fx
     finally:
         shutil.rmtree(work, ignore_errors=True)
+
+
+def bench_traced(eng, abi, fx, args):
+    """The engine on the reference tracer's own beams of the CPU baseline's
+    command (the same orientations, beams and grid -- the handler's own, from
+    the dump -- that the reference CLI just diffracted on the host):
+    autotuned mixed precision, graph replay."""
+    batch = abi.BeamBatch.from_fixture(fx)
+    consts = abi.consts_from_fixture(fx)
+    grid = abi.Grid.from_fixture(fx)
+    units = float(batch.n_beams) * grid.cells
+    hip = Hip()
+    ctx = eng.Context(0)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    ctx.set_timing(True)
+    ctx.autotune(abi.PREC_MIXED)
+    ks = []
+    for _ in range(max(1, args.warmup)):
+        ctx.diffract(abi.PREC_MIXED)
+    for _ in range(min(args.steps, 5)):
+        ctx.diffract(abi.PREC_MIXED)
+        ks.append(ctx.timing()["kernel_ms"])
+    ctx.set_graph(True)
+    ctx.diffract(abi.PREC_MIXED)
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.diffract(abi.PREC_MIXED)
+    hip.sync()
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"value": units * args.steps / wall, "unit": "beam.dir evals/s",
+            "ms_per_step": 1e3 * wall / args.steps, "kernel_ms": float(np.mean(ks)),
+            "orientations": int(batch.n_orient), "beams": int(batch.n_beams),
+            "units_per_step": units}
 
 
 def cpu_baseline(pkg, batch, grid, consts, budget_s=12.0):
@@ -382,6 +418,8 @@ def run(args):
         if graph:
             ctx.set_graph(False)
         kms = float(np.mean(kernel_ms))
+        # the committed counter figures of this leg's kernel and launch shape
+        tb = traffic.get(name)
         results[name] = {
             "value": units * args.steps / wall,
             "ms_per_step": 1e3 * wall / args.steps,
@@ -389,11 +427,9 @@ def run(args):
             "roofline": {"bound": "valu", "achieved": flops_launch / (kms * 1e-3) / 1e12,
                          "peak": peak / 1e12, "unit": "TFLOP/s",
                          "frac": flops_launch / (kms * 1e-3) / peak,
-                         "traffic": traffic.get(name.split("_")[0]),
+                         "traffic": tb,
                          "algorithmic_bytes": alg_bytes,
-                         "traffic_over_algorithmic": (traffic[name.split("_")[0]] / alg_bytes
-                                                      if name.split("_")[0] in traffic
-                                                      else None)},
+                         "traffic_over_algorithmic": tb / alg_bytes if tb else None},
             "work_items": tm["n_items"], "workgroups": tm["n_blocks"],
             "graph": graph,
         }
@@ -475,10 +511,11 @@ def run(args):
     if not args.no_cpu:
         port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)
         ref = None
+        traced_fx = None
         if os.path.exists(os.path.join(REF_DIR, "mbs_po")) and \
                 os.path.exists(os.path.join(REF_DIR, "mbs_po_dump")):
             try:
-                ref = cpu_baseline_reference(args.cpu_budget)
+                ref, traced_fx = cpu_baseline_reference(args.cpu_budget)
             except Exception as e:  # the oracle port stays the baseline
                 port["reference_error"] = str(e)
         out["cpu_baseline"] = ref if ref is not None else port
@@ -486,6 +523,11 @@ def run(args):
         if ref is not None:
             out["cpu_oracle_port"] = port
         out["gpu_over_cpu_fp64"] = results["fp64"]["value"] / out["cpu_baseline"]["value"]
+        if traced_fx is not None:
+            # the same beams the reference CLI diffracted, through the engine
+            tr = bench_traced(eng, abi, traced_fx, args)
+            tr["over_reference_cpu_same_beams"] = tr["value"] / ref["value"]
+            out["traced"] = tr
     print(json.dumps(out))
 
 

@@ -70,6 +70,7 @@ struct mbs_ctx {
     int n_az = 0, n_zen1 = 0, cells = 0, tab_mode = -1;
     double az_step = 0;
     DevBuf<double> theta, vf, tab;
+    DevBuf<float> tab32;
     // beams
     bool beams_ok = false;
     int n_orient = 0, n_beams = 0;
@@ -313,6 +314,7 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->theta.release();
     ctx->vf.release();
     ctx->tab.release();
+    ctx->tab32.release();
     ctx->bs_in.release();
     ctx->sc_raw.release();
     ctx->sc_path.release();
@@ -403,6 +405,7 @@ int mbs_set_grid(mbs_ctx *ctx, const mbs_grid *g)
     HIP_TRY(ctx, ctx->theta.ensure(ctx->n_zen1));
     HIP_TRY(ctx, ctx->vf.ensure((size_t)cells * 3));
     HIP_TRY(ctx, ctx->tab.ensure((size_t)cells * DT_COUNT));
+    HIP_TRY(ctx, ctx->tab32.ensure((size_t)cells * DT32_COUNT));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->theta.p, g->theta, sizeof(double) * ctx->n_zen1,
                                 hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(ctx, hipMemcpyAsync(ctx->vf.p, g->vf, sizeof(double) * cells * 3,
@@ -422,7 +425,8 @@ static int ensure_tables(mbs_ctx *ctx)
     const int threads = 256;
     const int blocks = (ctx->cells + threads - 1) / threads;
     grid_tables_kernel<<<blocks, threads, 0, ctx->stream>>>(
-        ctx->theta.p, ctx->vf.p, ctx->n_az, ctx->n_zen1, ctx->az_step, ctx->tab.p);
+        ctx->theta.p, ctx->vf.p, ctx->n_az, ctx->n_zen1, ctx->az_step, ctx->tab.p,
+        ctx->tab32.p);
     HIP_TRY(ctx, hipGetLastError());
     ctx->tab_mode = 0;
     return MBS_OK;
@@ -804,7 +808,7 @@ static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
         diffract_mfma_kernel<NS, JONES, PER, Q, 1><<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
             a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
-            a.per_orient, a.queue, a.sidx, a.fbuf);
+            a.per_orient, a.queue, a.tab32, a.sidx, a.fbuf);
     else if constexpr (PREC == 3)
         diffract_incoherent_kernel<<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.tab, a.hdr, a.vtx, a.vdd, a.vflag, a.nv,
@@ -814,12 +818,12 @@ static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx, a.vdd,
             a.vflag,
             a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
-            a.per_orient, a.queue);
+            a.per_orient, a.queue, a.tab32);
     else
         diffract_kernel<PREC, NS, JONES, PER, Q><<<blocks, 256, 0, s>>>(
             a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.hdr32, a.vtx, a.vdd,
             a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out, a.out_ns,
-            a.per_orient, a.queue);
+            a.per_orient, a.queue, a.tab32);
 }
 
 // queue: the persistent work-queue kernels (beam segments, no per-orientation
@@ -875,13 +879,20 @@ static int issue_diffraction(mbs_ctx *ctx, int prec, int ns, bool jones, int per
         HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     const int rt = 256;
     if (jones) {
-        jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-            a.out, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res, per_res, a.queue);
-        HIP_TRY(ctx, hipGetLastError());
-        if (ns) {
-            jones_to_mueller_kernel<<<(cells + rt - 1) / rt, rt, 0, s>>>(
-                a.out_ns, a.items, n_items, cells, ctx->n_az, ctx->weights.p, res_ns, nullptr,
-                a.queue);
+        // the mixed kernels (prec 1, 2, 4) leave FP32 Jones partials
+        const bool f32 = prec == 1 || prec == 2 || prec == 4;
+        const dim3 jb((cells + rt - 1) / rt);
+        for (int q = 0; q <= ns; ++q) {
+            const double *src = q ? a.out_ns : a.out;
+            double *dst = q ? res_ns : res;
+            double *po = q ? nullptr : per_res;
+            if (f32)
+                jones_to_mueller_kernel<float><<<jb, rt, 0, s>>>(
+                    reinterpret_cast<const float *>(src), a.items, n_items, cells, ctx->n_az,
+                    ctx->weights.p, dst, po, a.queue);
+            else
+                jones_to_mueller_kernel<double><<<jb, rt, 0, s>>>(
+                    src, a.items, n_items, cells, ctx->n_az, ctx->weights.p, dst, po, a.queue);
             HIP_TRY(ctx, hipGetLastError());
         }
     } else {
@@ -905,6 +916,7 @@ static void batch_args(const mbs_ctx *ctx, int prec, int n_items, int n_tiles, K
     a.cells = ctx->cells;
     a.n_az = ctx->n_az;
     a.tab = ctx->tab.p;
+    a.tab32 = ctx->tab32.p;
     a.hdr = ctx->hdr.p;
     a.hdr32 = ctx->hdr32.p;
     a.wb = ctx->wb.p;

@@ -75,7 +75,7 @@ __device__ __forceinline__ size_t abi_cell(int cell, int n_az, int cells)
 __global__ void grid_tables_kernel(const double *__restrict__ theta,
                                    const double *__restrict__ vf, int n_az,
                                    int n_zen1, double az_step,
-                                   double *__restrict__ tab)
+                                   double *__restrict__ tab, float *__restrict__ tab32)
 {
     const int cells = n_az * n_zen1;
     const int cell = blockIdx.x * blockDim.x + threadIdx.x;
@@ -120,6 +120,10 @@ __global__ void grid_tables_kernel(const double *__restrict__ theta,
     u[DT_U1 + 5] = f[2] - d[2] * dvf;
     for (int q = 0; q < DT_COUNT; ++q)
         tab[(size_t)q * cells + cell] = u[q];
+    // FP32 u0, u1 for the mixed kernels (half the bytes per work unit), one
+    // 48-byte record per cell
+    for (int q = 0; q < DT32_COUNT; ++q)
+        tab32[(size_t)cell * DT32_COUNT + q] = (float)u[DT_U0 + q];
 }
 
 // ---------------------------------------------------------------------------
@@ -1058,6 +1062,7 @@ struct KernelArgs {
     double *out_ns;          // same shape, no-shadow (NS only)
     double *per_orient;      // [n_orient][cells][16] (PER only, Mueller mode)
     int *queue;              // work-queue header (queue mode), else null
+    const float *tab32;      // FP32 u0, u1 [cells][DT32_COUNT] (mixed kernels)
     const int32_t *sidx;     // [n_beams] spectrum slot of each beam, -1: direct
     const float2 *fbuf;      // [spectrum beams][cells] F from the spectra
 };
@@ -1099,7 +1104,7 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     const int64_t *__restrict__ voff, const int32_t *__restrict__ ext,
     const int64_t *__restrict__ ooff, const double *__restrict__ weights, double k,
     double eps1, double *__restrict__ out, double *__restrict__ out_ns,
-    double *__restrict__ per_orient, int *__restrict__ queue)
+    double *__restrict__ per_orient, int *__restrict__ queue, const float *__restrict__ tab32)
 {
     using U = typename std::conditional<PREC == 1, float, double>::type;
     using U2 = typename std::conditional<PREC == 1, float4, double2>::type;
@@ -1136,8 +1141,12 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
     double d[3];
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
-    for (int q = 0; q < 12; ++q)
-        s_u[q * 256 + threadIdx.x] = (U)tab[(size_t)(DT_U0 + q) * cells + c];
+    for (int q = 0; q < 12; ++q) {
+        if constexpr (PREC == 1)
+            s_u[q * 256 + threadIdx.x] = tab32[(size_t)c * DT32_COUNT + q];
+        else
+            s_u[q * 256 + threadIdx.x] = tab[(size_t)(DT_U0 + q) * cells + c];
+    }
     __syncthreads();
     const WorkItem it = items[item];
     for (int o = it.o_begin; o < it.o_end; ++o) {
@@ -1181,11 +1190,13 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
         if (!live)
             continue;
         if constexpr (JONES) {
-            double *dst = out + ((size_t)item * cells + cell) * 8;
+            // Jones partials in the kernel's own precision (FP32 for the
+            // mixed kernel: half the bytes, the same values)
+            U *dst = reinterpret_cast<U *>(out) + ((size_t)item * cells + cell) * 8;
             for (int q = 0; q < 8; ++q)
                 dst[q] = j[q];
             if constexpr (NS) {
-                double *dn = out_ns + ((size_t)item * cells + cell) * 8;
+                U *dn = reinterpret_cast<U *>(out_ns) + ((size_t)item * cells + cell) * 8;
                 for (int q = 0; q < 8; ++q)
                     dn[q] = jn[q];
             }
@@ -1286,7 +1297,7 @@ enum { G_LD = 25 };  // LDS row stride (floats) of the per-wave G transpose
 // five workgroups share a CU).  Called by every thread of the block.
 __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1,
                                               float *__restrict__ sg,
-                                              const double *__restrict__ u, int cells,
+                                              const float *__restrict__ u, int cells,
                                               float *j)
 {
     const int lane = threadIdx.x & 63;
@@ -1306,7 +1317,7 @@ __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1
                 const int i = t >> 1, c = t & 1;
                 float re = 0.0f, im = 0.0f;
                 for (int m = 0; m < 6; ++m) {
-                    const float um = (float)u[(size_t)(6 * i + m) * cells];
+                    const float um = u[6 * i + m];
                     re = __builtin_fmaf(um, gr[4 * m + 2 * c], re);
                     im = __builtin_fmaf(um, gr[4 * m + 2 * c + 1], im);
                 }
@@ -1330,7 +1341,8 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const int32_t *__restrict__ ext, const int64_t *__restrict__ ooff,
     const double *__restrict__ weights, double k, double eps1, double *__restrict__ out,
     double *__restrict__ out_ns, double *__restrict__ per_orient, int *__restrict__ queue,
-    const int32_t *__restrict__ sidx = nullptr, const float2 *__restrict__ fbuf = nullptr)
+    const float *__restrict__ tab32, const int32_t *__restrict__ sidx = nullptr,
+    const float2 *__restrict__ fbuf = nullptr)
 {
     __shared__ float4 s_tab[kTabN32];
     __shared__ float s_g[4 * 32 * G_LD];
@@ -1354,9 +1366,10 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     double d[3];
     for (int q = 0; q < 3; ++q)
         d[q] = tab[(size_t)(DT_DIR + q) * cells + c];
-    // this lane's u0, u1, read by the epilogue straight from the grid tables
-    // (once per work item; staging them in LDS cost 12 KB per workgroup)
-    const double *u = tab + (size_t)DT_U0 * cells + c;
+    // this lane's u0, u1, read by the epilogue straight from its 48-byte
+    // FP32 record (once per work item; staging them in LDS cost 12 KB per
+    // workgroup, and one base address keeps the loads from holding twelve)
+    const float *u = tab32 + (size_t)c * DT32_COUNT;
     __syncthreads();  // the phasor table
     float *sg = s_g + (threadIdx.x >> 6) * 32 * G_LD;
     const int lane = threadIdx.x & 63;
@@ -1410,11 +1423,12 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
         if (!live)
             continue;
         if constexpr (JONES) {
-            double *dst = out + ((size_t)item * cells + cell) * 8;
+            // FP32 Jones partials (the sums are FP32)
+            float *dst = reinterpret_cast<float *>(out) + ((size_t)item * cells + cell) * 8;
             for (int q = 0; q < 8; ++q)
                 dst[q] = j[q];
             if constexpr (NS) {
-                double *dn = out_ns + ((size_t)item * cells + cell) * 8;
+                float *dn = reinterpret_cast<float *>(out_ns) + ((size_t)item * cells + cell) * 8;
                 for (int q = 0; q < 8; ++q)
                     dn[q] = jn[q];
             }
@@ -1462,7 +1476,8 @@ __global__ void reduce_mueller_kernel(const double *__restrict__ part, int n_ite
 // Jones segments -> Mueller: items are ordered by orientation; for each
 // orientation the segment Jones sums are added in segment order, converted
 // and accumulated with the orientation weight (TracerPOTotal.cpp:5985-5994).
-__global__ void jones_to_mueller_kernel(const double *__restrict__ part,
+template <typename T>
+__global__ void jones_to_mueller_kernel(const T *__restrict__ part,
                                         const WorkItem *__restrict__ items,
                                         int n_items, int cells, int n_az,
                                         const double *__restrict__ weights,
@@ -1484,7 +1499,7 @@ __global__ void jones_to_mueller_kernel(const double *__restrict__ part,
         const int o = items[i].o_begin;
         double j[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         while (i < n_items && items[i].o_begin == o) {
-            const double *src = part + ((size_t)i * cells + cell) * 8;
+            const T *src = part + ((size_t)i * cells + cell) * 8;
             for (int q = 0; q < 8; ++q)
                 j[q] += src[q];
             ++i;

@@ -52,6 +52,8 @@ enum { EDGE_VX = 1, EDGE_VY = 2 };
 
 // Grid tables, SoA [DT_COUNT][cells], cells in the device's theta-major order.
 enum { DT_DIR = 0, DT_U0 = 3, DT_U1 = 9, DT_COUNT = 15 };
+// FP32 copies of u0, u1 for the mixed kernels, [cells][DT32_COUNT].
+enum { DT32_COUNT = 12 };
 
 // A unit of work: orientations [o_begin, o_end) whole (Mueller mode), or the
 // beam range [b_begin, b_end) of orientation o_begin (Jones mode).
