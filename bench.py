@@ -448,7 +448,8 @@ def run(args):
     best = min(tune, key=lambda e: e[3])
     leg("mixed", abi.PREC_MIXED, FP32_PEAK, graph=True)
     results["mixed"]["tuned"] = {"target_blocks": best[0], "work_mode": best[1],
-                                 "mfma": bool(best[2]), "candidates": len(tune),
+                                 "mfma": bool(best[2]), "waves_per_simd": best[4],
+                                 "candidates": len(tune),
                                  "tuning_s": tune_s}
     fft = bench_aperture_fft(ctx, batch, args)
     ctx.close()

@@ -491,9 +491,10 @@ int mbs_set_work_mode(mbs_ctx *ctx, int mode);
  * times mbs_diffract(flags) on the uploaded batch for every candidate
  * (target workgroups 4096..131072 x whole-orientation / beam-segment /
  * device-queue work items x, for mixed precision, MFMA / vector-ALU coherent
- * sum), keeps the fastest for later mbs_diffract calls of the same
- * precision, and reports
- * what it timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.
+ * sum, and for the MFMA kernel with beam segments and no MBS_NOSHADOW its
+ * register allocation held to 4, 5 or 6 waves per SIMD), keeps the fastest
+ * for later mbs_diffract calls of the same precision, and reports what it
+ * timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.
  * flags: MBS_PREC_MIXED and/or MBS_NOSHADOW only.  Leaves a result like
  * mbs_diffract(flags). */
 typedef struct mbs_tune_entry {
@@ -501,14 +502,25 @@ typedef struct mbs_tune_entry {
     int32_t work_mode;  /* 1 whole orientations, 2 beam segments, 3 device queue */
     int32_t mfma;       /* 1 MFMA contraction, 0 vector ALU (mixed only) */
     float ms;           /* main kernel + reduction, device time */
+    int32_t waves;      /* MFMA kernel: waves per SIMD its registers are held to
+                           (4, 5 or 6; 4 and 6 with beam segments only); 0 for
+                           the other kernels */
 } mbs_tune_entry;
 int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t max_entries,
                  int32_t *n_entries);
+/* Applies one launch shape (an entry mbs_autotune reported, e.g. from an
+ * earlier process) for later mbs_diffract calls of the precision in flags,
+ * as if the autotuner had chosen it; entry->ms is ignored.  MBS_ERR_INVALID
+ * for a shape the tuner would not try. */
+int mbs_apply_tune_entry(mbs_ctx *ctx, uint32_t flags, const mbs_tune_entry *entry);
 
 /* Graph replay: with enable != 0, mbs_diffract captures its kernel sequence
- * (diffraction + reduction, no copies or memsets) into a HIP graph and
- * replays it while launch shape, buffers and sizes are unchanged (the work
- * list is re-uploaded before each replay); re-captured otherwise. */
+ * (diffraction + reduction, no copies or memsets; with MBS_WORK_QUEUE also
+ * the device's segment planning) into a HIP graph and replays it while
+ * launch shape, buffers and sizes are unchanged; re-captured otherwise.  A
+ * host-planned work list is copied to the device before each replay; the
+ * device queue plans each replay on the device, so a new upload of the same
+ * sizes replays without host work. */
 int mbs_set_graph(mbs_ctx *ctx, int enable);
 
 /* Graph use of the context so far: captures made and graph launches (a

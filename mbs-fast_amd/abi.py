@@ -108,7 +108,7 @@ class mbs_spectrum_info(C.Structure):
 
 class mbs_tune_entry(C.Structure):
     _fields_ = [("target_blocks", C.c_int32), ("work_mode", C.c_int32), ("mfma", C.c_int32),
-                ("ms", C.c_float)]
+                ("ms", C.c_float), ("waves", C.c_int32)]
 
 
 def _ptr(a, ct):

@@ -137,7 +137,7 @@ struct mbs_ctx {
     // launch shape chosen by mbs_autotune, per precision (0 FP64, 1 mixed)
     struct Tuned {
         bool valid = false;
-        int target_blocks = 0, work_mode = 0, mfma = 1;
+        int target_blocks = 0, work_mode = 0, mfma = 1, waves = 0;  // waves 0: default
     } tuned[2];
     // graph replay of the diffraction kernels (mbs_set_graph)
     bool use_graph = false;
@@ -827,10 +827,24 @@ static void launch_one(const KernelArgs &a, int blocks, hipStream_t s)
 }
 
 // queue: the persistent work-queue kernels (beam segments, no per-orientation
-// grids), which take their units from a.queue.
+// grids), which take their units from a.queue.  a.waves selects an occupancy
+// variant of the beam-segment MFMA kernel (4 or 6 waves per SIMD).
 static void launch_diffract(int prec, int ns, int jones, int per, bool queue,
                             const KernelArgs &a, int blocks, hipStream_t s)
 {
+    if (prec == 2 && !ns && jones && !per && !queue && (a.waves == 4 || a.waves == 6)) {
+        if (a.waves == 4)
+            diffract_mfma_kernel<0, 1, 0, 0, 0, 4><<<blocks, 256, 0, s>>>(
+                a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx,
+                a.vdd, a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out,
+                a.out_ns, a.per_orient, a.queue, a.tab32);
+        else
+            diffract_mfma_kernel<0, 1, 0, 0, 0, 6><<<blocks, 256, 0, s>>>(
+                a.items, a.n_items, a.n_tiles, a.cells, a.n_az, a.tab, a.hdr, a.wb, a.vtx,
+                a.vdd, a.vflag, a.nv, a.voff, a.ext, a.ooff, a.weights, a.k, a.eps1, a.out,
+                a.out_ns, a.per_orient, a.queue, a.tab32);
+        return;
+    }
 #define MBS_L(P, N, J, R, Q)                                                 \
     if (prec == P && ns == N && jones == J && per == R && (int)queue == Q) { \
         launch_one<P, N, J, R, Q>(a, blocks, s);                             \
@@ -1032,6 +1046,7 @@ int mbs_diffract(mbs_ctx *ctx, uint32_t flags)
                                     sizeof(WorkItem) * n_items, hipMemcpyHostToDevice, s));
     KernelArgs a;
     batch_args(ctx, prec, n_items, n_tiles, a);
+    a.waves = tn.valid ? tn.waves : 0;
     a.items = ctx->items.p;
     a.out = ctx->part.p;
     a.out_ns = ns ? ctx->part_ns.p : nullptr;
@@ -1948,8 +1963,18 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
     std::vector<mbs_tune_entry> tried;
     mbs_ctx::Tuned best;
     float best_ms = 0.0f;
+    // occupancy variants of the MFMA kernel with beam segments
+    struct Cand {
+        int mfma, mode, waves;
+    };
+    std::vector<Cand> cands;
     for (int mfma = (base ? 1 : 0); mfma >= 0; --mfma)
         for (int mode = 3; mode >= 1; --mode)
+            for (int w : {0, 4, 6})
+                if (w == 0 || (mfma && mode == MBS_WORK_SEGMENTS && !(flags & MBS_NOSHADOW)))
+                    cands.push_back(Cand{mfma, mode, w});
+    for (const Cand &cd : cands) {
+        const int mfma = cd.mfma, mode = cd.mode;
             for (int tb : kBlocks) {
                 std::vector<WorkItem> items;
                 const bool jones = plan_items(ctx, n_tiles, tb,
@@ -1965,6 +1990,7 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
                 t.target_blocks = tb;
                 t.work_mode = mode;
                 t.mfma = mfma;
+                t.waves = cd.waves;
                 ctx->tuned[base] = t;
                 float ms = 0.0f;
                 for (int rep = 0; rep < 2 && !rc; ++rep) {  // the first run allocates
@@ -1982,12 +2008,14 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
                 e.work_mode = mode;
                 e.mfma = mfma;
                 e.ms = ms;
+                e.waves = cd.waves ? cd.waves : (mfma ? MBS_MFMA_WAVES : 0);
                 tried.push_back(e);
                 if (!best.valid || ms < best_ms) {
                     best = t;
                     best_ms = ms;
                 }
             }
+    }
     ctx->tuned[base] = best;
     if (!had_timing)
         mbs_set_timing(ctx, 0);
@@ -1998,6 +2026,31 @@ int mbs_autotune(mbs_ctx *ctx, uint32_t flags, mbs_tune_entry *entries, int32_t 
     if (entries)
         for (size_t i = 0; i < tried.size() && (int32_t)i < max_entries; ++i)
             entries[i] = tried[i];
+    return MBS_OK;
+}
+
+int mbs_apply_tune_entry(mbs_ctx *ctx, uint32_t flags, const mbs_tune_entry *e)
+{
+    if (!ctx || !e)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_apply_tune_entry: NULL argument");
+    if (flags & ~(uint32_t)(MBS_PREC_MIXED | MBS_NOSHADOW))
+        return fail(ctx, MBS_ERR_INVALID,
+                    "mbs_apply_tune_entry takes MBS_PREC_MIXED and MBS_NOSHADOW");
+    const int base = (flags & MBS_PREC_MIXED) ? 1 : 0;
+    const bool waves_ok = e->waves == 0 || e->waves == MBS_MFMA_WAVES ||
+                          ((e->waves == 4 || e->waves == 6) && base && e->mfma &&
+                           e->work_mode == MBS_WORK_SEGMENTS);
+    if (e->target_blocks < 1 || e->work_mode < 1 || e->work_mode > 3 || e->mfma < 0 ||
+        e->mfma > 1 || (e->mfma && !base) || !waves_ok)
+        return fail(ctx, MBS_ERR_INVALID, "mbs_apply_tune_entry: shape (%d, %d, %d, %d)",
+                    e->target_blocks, e->work_mode, e->mfma, e->waves);
+    mbs_ctx::Tuned t;
+    t.valid = true;
+    t.target_blocks = e->target_blocks;
+    t.work_mode = e->work_mode;
+    t.mfma = e->mfma;
+    t.waves = (e->waves == 4 || e->waves == 6) ? e->waves : 0;
+    ctx->tuned[base] = t;
     return MBS_OK;
 }
 

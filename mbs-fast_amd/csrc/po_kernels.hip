@@ -39,13 +39,14 @@
 #define MBS_MIN_WAVES 4
 #endif
 #define MBS_OCCUPANCY __attribute__((amdgpu_waves_per_eu(MBS_MIN_WAVES, 8)))
-// The MFMA kernel is held to 5 waves per SIMD (~100 VGPRs + AGPRs) when it
-// keeps one accumulator set -- its 28.8 KB of LDS per workgroup also lets
-// five workgroups share a CU; the no-shadow variant (two sets) gets 2.
+// The MFMA kernel is held to WAVES waves per SIMD (default 5: 96 VGPRs)
+// when it keeps one accumulator set -- its 20.5 KB of LDS per workgroup lets
+// seven workgroups share a CU; the no-shadow variant (two sets) gets 2.  The
+// autotuner also tries 4 and 6 for the default beam-segment kernel.
 #ifndef MBS_MFMA_WAVES
 #define MBS_MFMA_WAVES 5
 #endif
-#define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : MBS_MFMA_WAVES, 8)))
+#define MBS_MFMA_OCCUPANCY __attribute__((amdgpu_waves_per_eu(NS ? 2 : WAVES, 8)))
 
 namespace mbs {
 
@@ -1065,6 +1066,7 @@ struct KernelArgs {
     const float *tab32;      // FP32 u0, u1 [cells][DT32_COUNT] (mixed kernels)
     const int32_t *sidx;     // [n_beams] spectrum slot of each beam, -1: direct
     const float2 *fbuf;      // [spectrum beams][cells] F from the spectra
+    int waves;               // MFMA kernel occupancy variant (0: the default)
 };
 
 // Mueller of one orientation's Jones sum, weighted, stored to (first) or
@@ -1330,8 +1332,9 @@ __device__ __forceinline__ void mfma_epilogue(const f32x16 &g0, const f32x16 &g1
 }
 
 // FSRC: beams with sidx[b] >= 0 take F from fbuf (the near-forward spectrum
-// source) instead of the aperture integral.
-template <int NS, int JONES, int PER, int QUEUE, int FSRC = 0>
+// source) instead of the aperture integral.  WAVES: the occupancy the
+// register allocation is held to (see MBS_MFMA_OCCUPANCY).
+template <int NS, int JONES, int PER, int QUEUE, int FSRC = 0, int WAVES = MBS_MFMA_WAVES>
 __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
     const WorkItem *__restrict__ items, int n_items, int n_tiles, int cells, int n_az,
     const double *__restrict__ tab, const double *__restrict__ hdr,

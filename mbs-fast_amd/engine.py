@@ -94,6 +94,8 @@ def lib():
         L.mbs_autotune.restype = C.c_int
         L.mbs_graph_stats.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.mbs_graph_stats.restype = C.c_int
+        L.mbs_apply_tune_entry.argtypes = [vp, C.c_uint32, C.POINTER(abi.mbs_tune_entry)]
+        L.mbs_apply_tune_entry.restype = C.c_int
         L.mbs_set_graph.argtypes = [vp, C.c_int]
         L.mbs_set_graph.restype = C.c_int
         fp = C.POINTER(C.c_float)
@@ -384,11 +386,16 @@ class Context:
 
     def autotune(self, flags=0):
         """Times candidate launch shapes, keeps the fastest; returns the list
-        of (target_blocks, work_mode, mfma, ms)."""
-        buf = (abi.mbs_tune_entry * 64)()
+        of (target_blocks, work_mode, mfma, ms, waves)."""
+        buf = (abi.mbs_tune_entry * 128)()
         n = C.c_int32()
-        self._check(self._L.mbs_autotune(self._h, flags, buf, 64, C.byref(n)))
-        return [(e.target_blocks, e.work_mode, e.mfma, e.ms) for e in buf[:n.value]]
+        self._check(self._L.mbs_autotune(self._h, flags, buf, 128, C.byref(n)))
+        return [(e.target_blocks, e.work_mode, e.mfma, e.ms, e.waves) for e in buf[:n.value]]
+
+    def apply_tune_entry(self, flags, target_blocks, work_mode, mfma, waves=0):
+        """mbs_apply_tune_entry: use this launch shape for later calls."""
+        e = abi.mbs_tune_entry(int(target_blocks), int(work_mode), int(mfma), 0.0, int(waves))
+        self._check(self._L.mbs_apply_tune_entry(self._h, flags, C.byref(e)))
 
     def set_graph(self, enable=True):
         self._check(self._L.mbs_set_graph(self._h, 1 if enable else 0))

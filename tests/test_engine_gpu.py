@@ -624,6 +624,26 @@ def test_graph_replay_and_autotune(pkg, oracle, ctx):
     ctx.set_target_blocks(65536)  # back to the default shape for later tests
 
 
+def test_mfma_occupancy_variants_agree(pkg, ctx):
+    """The MFMA kernel with its registers held to 4, 5 and 6 waves per SIMD
+    (the autotuner's occupancy dimension, applied with mbs_apply_tune_entry)
+    computes the same arithmetic: bit-identical grids."""
+    abi = pkg.abi
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    out = {}
+    for w in (4, 5, 6):
+        ctx.apply_tune_entry(abi.PREC_MIXED, 65536, abi.WORK_SEGMENTS, 1, w)
+        out[w], _ = ctx.mueller(abi.PREC_MIXED)
+    np.testing.assert_array_equal(out[4], out[5])
+    np.testing.assert_array_equal(out[6], out[5])
+    with pytest.raises(pkg.engine.EngineError):
+        ctx.apply_tune_entry(abi.PREC_MIXED, 65536, abi.WORK_QUEUE, 1, 6)
+    ctx.set_target_blocks(65536)  # drops the applied shape
+
+
 def test_mlp_gradient_against_oracle(pkg, ctx):
     """Device forward and backward (BF16 MFMA GEMMs, FP32 accumulation)
     against the FP64 oracle evaluated on the same BF16-rounded operands; and
