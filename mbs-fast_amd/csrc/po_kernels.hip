@@ -183,7 +183,6 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     h[H_KC + 0] = ext ? 0.0 : -k * cen[0];
     h[H_KC + 1] = ext ? 0.0 : -k * cen[1];
     h[H_KC + 2] = ext ? 0.0 : -k * cen[2];
-    h[H_EXT] = ext ? 1.0 : 0.0;
     // complWave is folded into J_phased below; the small-phase area limit is
     // therefore stored divided by complWave.
     const double area = s[12] * (sc * sc);
@@ -291,6 +290,7 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     // a degenerate or very large polygon keeps the per-vertex test everywhere
     h[H_PHB] = (wmin > 0.0 && wmin < __builtin_inf()) ? 2.04e-3 / (k * wmin)
                                                       : __builtin_inf();
+    h[H_PHBM] = h[H_PHB] * (kSmallPhaseMixed / kSmallPhase);
     // rotation vectors, pre-scaled by 1/2 (exact)
     for (int q = 0; q < 3; ++q) {
         h[H_T0 + q] = 0.5 * s[13 + q];      // NT
@@ -408,12 +408,12 @@ __device__ __forceinline__ bool all_phases_small(const double *__restrict__ vtx,
 
 template <int NV>
 __device__ __forceinline__ bool all_phases_small_nv(const double *kx, const double *ky, double A,
-                                                    double B)
+                                                    double B, double limit = kSmallPhase)
 {
     bool small = true;
 #pragma unroll
     for (int e = 0; e < NV; ++e)
-        small = small && __builtin_fabs(__builtin_fma(A, kx[e], B * ky[e])) <= kSmallPhase;
+        small = small && __builtin_fabs(__builtin_fma(A, kx[e], B * ky[e])) <= limit;
     return small;
 }
 
@@ -699,15 +699,17 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
 }
 
 // The end of the mixed aperture integral: the 1/B or -1/A normalisation,
-// the small-phase moment form when every vertex phase is within 1e-3 rad,
+// the small-phase moment form when every vertex phase is within 0.25 rad,
 // and a NaN result (degenerate beam) returned as 0, i.e. the beam skipped.
+// The normaliser is the hardware reciprocal (1 ulp), as for the edge
+// coefficients: the FP32 edge sum it scales carries more rounding than that.
 __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__ h, double A,
-                                                      double B, double psi, bool useB,
+                                                      double B, double psi, float nin,
                                                       bool small, float sr, float si,
                                                       const float4 *__restrict__ tab,
                                                       float &out_r, float &out_i)
 {
-    const float nrm = rcp_f32((float)(useB ? B : -A));
+    const float nrm = __builtin_amdgcn_rcpf(nin);  // nin: B or -A
     float fr = sr * nrm, fi = si * nrm;
     if (__builtin_expect(small, 0)) {
         // <e^{i phi}> = sum_n i^n <phi^n> / n! to fifth order
@@ -736,10 +738,10 @@ __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__
 // slopes, then a per-lane select of the result (selecting the slope first
 // costs two moves and a select per half of the FP64 operand).  An edge that
 // is invalid for the lane's branch carries kInvalidSlope, so its reciprocal
-// is zero and the sum needs no validity test.  A separate path for waves
-// whose lanes all take one branch of the reference's sum -- one coefficient
-// per edge, invalid edges skipped by a scalar branch -- measured slower here
-// and no faster in the FP64 sum (the code triples).  Near-singular edges
+// is zero and the sum needs no validity test.  A separate pass for waves
+// whose lanes all take one branch of the reference's sum (one coefficient
+// per edge; about half the waves of an orientation average) gained 0.4 %
+// for three copies of the loop, and none in the FP64 sum.  Near-singular edges
 // (|C| <= eps1, their term below zero) are evaluated in one pass after the
 // sum, taken only when some lane has one; the validity test lives there.
 template <int NV>
@@ -809,6 +811,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
     const double psi = d[0] * h[H_KC] + d[1] * h[H_KC + 1] + d[2] * h[H_KC + 2];
     const bool useB = __builtin_fabs(B) > __builtin_fabs(A);
     const int vmask = useB ? EDGE_VX : EDGE_VY;
+    const float nin = useB ? (float)B : -(float)A;
     float sr = 0.0f, si = 0.0f;
     bool small;
     if constexpr (NV > 0) {
@@ -820,21 +823,25 @@ __device__ __forceinline__ void aperture_mixed_nv(
             kx[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KX]);
             ky[e] = uniform_f64(vtx[(size_t)e * VTX_STRIDE + V_KY]);
         }
+        // the moment form's bound (loaded with the header's first fields,
+        // so no scalar load waits among the table reads)
+        const double bnd = uniform_f64(h[H_PHBM]);
         float dv[NV];
         float4 ev[NV];
-        // every vertex tested here: the per-beam bound that spares the FP64
-        // kernel this test measured slower in this one
-        small = true;
 #pragma unroll
-        for (int e = 0; e < NV; ++e) {
-            const double p = __builtin_fma(A, kx[e], B * ky[e]);
-            small = small && __builtin_fabs(p) <= kSmallPhaseMixed;
-            sincos_reduce_f32(p + psi, tab, dv[e], ev[e]);
-        }
+        for (int e = 0; e < NV; ++e)
+            sincos_reduce_f32(__builtin_fma(A, kx[e], __builtin_fma(B, ky[e], psi)), tab, dv[e],
+                              ev[e]);
         float sv[NV], cv[NV];
 #pragma unroll
         for (int e = 0; e < NV; ++e)
             sincos_finish_f32(dv[e], ev[e], sv[e], cv[e]);
+        // the vertex phases one by one only in waves where the beam's bound
+        // leaves the moment form possible for some lane
+        small = false;
+        const bool maybe = __builtin_fabsf(nin) <= (float)bnd;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(maybe) != 0, 0))
+            small = maybe && all_phases_small_nv<NV>(kx, ky, A, B, kSmallPhaseMixed);
         double syx[NV], sxy[NV];
         int fl[NV];
 #pragma unroll
@@ -847,10 +854,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
                               si);
     } else {
         small = false;
-        // H_PHB is the bound for 1e-3 rad; it scales with the phase limit
-        if (__builtin_expect(__builtin_fabsf((float)(useB ? B : -A)) <=
-                                 (float)(h[H_PHB] * (kSmallPhaseMixed / kSmallPhase)),
-                             0))
+        if (__builtin_expect(__builtin_fabsf(nin) <= (float)h[H_PHBM], 0))
             small = all_phases_small(vtx, nv, A, B, kSmallPhaseMixed);
         float s0, c0;
         sincos_table_f32(__builtin_fma(A, vtx[V_KX], __builtin_fma(B, vtx[V_KY], psi)), tab, s0,
@@ -860,7 +864,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
             edge_mixed(e, nv, vtx, vdd, vflag, A, B, psi, useB, vmask, tab, k, eps1,
                        s0, c0, sp, cp, sr, si, small);
     }
-    aperture_mixed_finish(h, A, B, psi, useB, small, sr, si, tab, out_r, out_i);
+    aperture_mixed_finish(h, A, B, psi, nin, small, sr, si, tab, out_r, out_i);
 }
 
 __device__ __forceinline__ void aperture_mixed(
@@ -1163,11 +1167,13 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_kernel(
         // ahead, so a beam's header and vertex loads issue together
         int64_t v0_next = b0 < b1 ? voff[b0] : 0;
         int nv_next = b0 < b1 ? nv[b0] : 0;
-        for (int64_t b = b0; b < b1; ++b) {
+        const int nbeam = (int)(b1 - b0);  // 32-bit loop test (scalar unit)
+        for (int ib = 0; ib < nbeam; ++ib) {
+            const int64_t b = b0 + ib;
             const double *h = hdr + (size_t)b * HDR_STRIDE;
             const int64_t v0 = v0_next;
             const int nvb = nv_next;
-            if (b + 1 < b1) {
+            if (ib + 1 < nbeam) {
                 v0_next = voff[b + 1];
                 nv_next = nv[b + 1];
             }
@@ -1388,11 +1394,15 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
         // ahead, so a beam's header and vertex loads issue together
         int64_t v0_next = b0 < b1 ? voff[b0] : 0;
         int nv_next = b0 < b1 ? nv[b0] : 0;
-        for (int64_t b = b0; b < b1; ++b) {
+        // a 32-bit trip count: the scalar unit has no 64-bit ordered compare,
+        // so a 64-bit loop test would go to the vector unit
+        const int nbeam = (int)(b1 - b0);
+        for (int ib = 0; ib < nbeam; ++ib) {
+            const int64_t b = b0 + ib;
             const float bop = wb[(size_t)b * WB_STRIDE + lane];
             const int64_t v0 = v0_next;
             const int nvb = nv_next;
-            if (b + 1 < b1) {
+            if (ib + 1 < nbeam) {
                 v0_next = voff[b + 1];
                 nv_next = nv[b + 1];
             }

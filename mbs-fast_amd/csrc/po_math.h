@@ -108,18 +108,19 @@ __device__ __forceinline__ void sincos_table(double x, const double2 *__restrict
 // e^{2 pi i j / 512}, in two halves so that callers can issue the table
 // reads of several arguments before using any of them: the FP64 reduction
 // and table read, then the FP32 correction and the product with the table
-// entry.  The reduction is exact as above (kTab32Step1 = double(pi)/256 is
-// again an exact scaling, and |x - j kTab32Step1| < 2^-6).  Against the
-// 256-entry table the index costs one more instruction (a mask and a shift
-// instead of one shift of the low byte) and the correction two fewer.
+// entry.  One FMA reduces: kTab32Step = double(pi)/256 differs from 2 pi/512
+// by 4.8e-19, so x - j kTab32Step is off by j 4.8e-19 = |x| 3.9e-17 at most,
+// below half an ulp of x itself (the FP64 kernels add the second term of
+// the split; an FP32 result cannot see it).  Against the 256-entry table the
+// index costs one more instruction (a mask and a shift instead of one shift
+// of the low byte) and the correction two fewer.
 //
 // An entry holds (cos, sin, cos/2, sin/2) of its angle (16 bytes, one LDS
 // read): with the halves at hand the correction and the product are four
 // FMAs (see sincos_finish_f32).
 constexpr int kTabN32 = 512;
 constexpr double kTab32InvStep = 81.487330863050411912;  // 512 / (2 pi)
-constexpr double kTab32Step1 = 3.141592653589793116 / 256.0;
-constexpr double kTab32Step2 = 1.2246467991473532e-16 / 256.0;
+constexpr double kTab32Step = 3.141592653589793116 / 256.0;
 
 // Entry j of the FP32 table.
 __device__ __forceinline__ float4 tab32_entry(int j)
@@ -134,9 +135,7 @@ __device__ __forceinline__ void sincos_reduce_f32(double x, const float4 *__rest
 {
     const double t = __builtin_fma(x, kTab32InvStep, kRoundMagic);
     const double j = t - kRoundMagic;
-    double dd = __builtin_fma(-j, kTab32Step1, x);
-    dd = __builtin_fma(-j, kTab32Step2, dd);
-    d = (float)dd;
+    d = (float)__builtin_fma(-j, kTab32Step, x);
     e = tab[low_bits(t) & (kTabN32 - 1)];
 }
 

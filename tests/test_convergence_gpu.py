@@ -433,7 +433,8 @@ C2_PARTICLE = ["--particle", "1", "100", "50", "--refractive-index", "1.3116", "
 def test_time_to_one_percent_config2():
     """D = 50 um, L = 100 um, 541 x 96, N = 1024 ... 32768 through the CLI on
     both GPU precisions, the converged reference 4 x 32768 (seeds 7-10), the
-    CPU reference CLI run at the N the GPU needs."""
+    CPU reference CLI run at the N the GPU needs (at most 1024, scaled
+    linearly above it)."""
     wd = tempfile.mkdtemp(prefix="mbs_conv2_")
     with open(os.path.join(wd, "theta.grid"), "w") as f:
         f.write("\n".join("%.10g" % t for t in THETA_C2) + "\n")
@@ -460,7 +461,16 @@ def test_time_to_one_percent_config2():
                 break
             n *= 2
     if "gpu fp64" in hit:
+        # the CPU reference at the N the GPU needs, or, above 1024 (an hour
+        # of host time at 32768), at 1024 orientations scaled linearly in N:
+        # both phases are per-orientation loops
         n = hit["gpu fp64"][0]
-        w, ph = _cli(CPU_BIN, "cpu", n, 1, wd, "cpu", C2_PARTICLE + grid)
-        print("cpu N=%d wall %.1f s (diffraction %.1f s) -> GPU FP64 %.1fx sooner"
-              % (n, w, ph[1], w / hit["gpu fp64"][1]))
+        n_cpu = min(n, 1024)
+        w, ph = _cli(CPU_BIN, "cpu", n_cpu, 1, wd, "cpu", C2_PARTICLE + grid)
+        f = n / n_cpu
+        print("cpu N=%d wall %.1f s (trace %.1f s, diffraction %.1f s)%s -> GPU FP64 %.1fx, "
+              "GPU mixed %.1fx sooner"
+              % (n_cpu, w, ph[0], ph[1],
+                 (", x%d for N=%d: wall %.0f s, diffraction %.0f s" % (f, n, w * f, ph[1] * f))
+                 if f > 1 else "", w * f / hit["gpu fp64"][1],
+                 w * f / hit["gpu mixed"][1] if "gpu mixed" in hit else float("nan")))
