@@ -290,7 +290,11 @@ __global__ void pack_beams_kernel(int n_beams, const double *__restrict__ bs,
     // a degenerate or very large polygon keeps the per-vertex test everywhere
     h[H_PHB] = (wmin > 0.0 && wmin < __builtin_inf()) ? 2.04e-3 / (k * wmin)
                                                       : __builtin_inf();
-    h[H_PHBM] = h[H_PHB] * (kSmallPhaseMixed / kSmallPhase);
+    // the mixed kernels compare in FP32: the bound is kept as an FP32 value
+    // in the slot's low word, so no conversion is needed per beam
+    h[H_PHBM] = __builtin_bit_cast(
+        double, (uint64_t)__builtin_bit_cast(
+                    uint32_t, (float)(h[H_PHB] * (kSmallPhaseMixed / kSmallPhase))));
     // rotation vectors, pre-scaled by 1/2 (exact)
     for (int q = 0; q < 3; ++q) {
         h[H_T0 + q] = 0.5 * s[13 + q];      // NT
@@ -703,6 +707,16 @@ __device__ __forceinline__ void edge_mixed(int e, int n, const double *__restric
 // and a NaN result (degenerate beam) returned as 0, i.e. the beam skipped.
 // The normaliser is the hardware reciprocal (1 ulp), as for the edge
 // coefficients: the FP32 edge sum it scales carries more rounding than that.
+// Word 3 of a raw buffer resource on gfx9-family devices (32-bit data
+// format; the element and index fields unused).
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
+// The FP32 small-phase bound stored in the low word of header slot H_PHBM.
+__device__ __forceinline__ float phase_bound_f32(double slot)
+{
+    return __builtin_bit_cast(float, low_bits(slot));
+}
+
 __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__ h, double A,
                                                       double B, double psi, float nin,
                                                       bool small, float sr, float si,
@@ -729,9 +743,15 @@ __device__ __forceinline__ void aperture_mixed_finish(const double *__restrict__
         fr = (float)ar * cd - (float)ai * sd;
         fi = (float)ar * sd + (float)ai * cd;
     }
+    // a NaN F (degenerate polygon) counts as zero; the selects only in
+    // waves that have one
+    out_r = fr;
+    out_i = fi;
     const bool bad = __builtin_isnan(fr);
-    out_r = bad ? 0.0f : fr;
-    out_i = bad ? 0.0f : fi;
+    if (__builtin_expect(__builtin_amdgcn_ballot_w64(bad) != 0, 0)) {
+        out_r = bad ? 0.0f : fr;
+        out_i = bad ? 0.0f : fi;
+    }
 }
 
 // The edge sum of aperture_mixed_nv.  Both coefficients come from scalar
@@ -752,6 +772,9 @@ __device__ __forceinline__ void edge_sum_mixed_nv(
 {
     bool rare[NV];
     bool any_rare = false;
+    // every term taken as it comes: a lane with a near-singular edge (its
+    // reciprocal huge or infinite) sums again below, so the common sum
+    // carries no select
 #pragma unroll
     for (int e = 0; e < NV; ++e) {
         const int f = e + 1 < NV ? e + 1 : 0;
@@ -762,15 +785,29 @@ __device__ __forceinline__ void edge_sum_mixed_nv(
         // hardware reciprocal (1 ulp): the FP32 phasor differences it scales
         // carry more rounding than that
         const float r = __builtin_amdgcn_rcpf((float)cc);
-        const float inv = big ? r : 0.0f;
-        sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
-        si = __builtin_fmaf(sv[f] - sv[e], inv, si);
+        sr = __builtin_fmaf(cv[f] - cv[e], r, sr);
+        si = __builtin_fmaf(sv[f] - sv[e], r, si);
         // near-singular lanes as scalar mask arithmetic (a second vector
         // compare for !big otherwise)
         rare[e] = __builtin_amdgcn_inverse_ballot_w64(~__builtin_amdgcn_ballot_w64(big));
         any_rare = any_rare || rare[e];
     }
+    // (lanes on the moment form discard the sum)
     if (__builtin_expect(any_rare && !small, 0)) {
+        // the sum without the near-singular terms (the same operations, so
+        // the same bits as a masked first pass) ...
+        sr = 0.0f;
+        si = 0.0f;
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int f = e + 1 < NV ? e + 1 : 0;
+            const double cc =
+                useB ? __builtin_fma(syx[e], B, A) : __builtin_fma(sxy[e], A, B);
+            const float inv = rare[e] ? 0.0f : __builtin_amdgcn_rcpf((float)cc);
+            sr = __builtin_fmaf(cv[f] - cv[e], inv, sr);
+            si = __builtin_fmaf(sv[f] - sv[e], inv, si);
+        }
+        // ... and those terms in the stable form
 #pragma unroll
         for (int e = 0; e < NV; ++e) {
             if (rare[e] && (fl[e] & vmask) != 0) {
@@ -825,7 +862,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
         }
         // the moment form's bound (loaded with the header's first fields,
         // so no scalar load waits among the table reads)
-        const double bnd = uniform_f64(h[H_PHBM]);
+        const float bnd = phase_bound_f32(uniform_f64(h[H_PHBM]));
         float dv[NV];
         float4 ev[NV];
 #pragma unroll
@@ -839,7 +876,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
         // the vertex phases one by one only in waves where the beam's bound
         // leaves the moment form possible for some lane
         small = false;
-        const bool maybe = __builtin_fabsf(nin) <= (float)bnd;
+        const bool maybe = __builtin_fabsf(nin) <= bnd;
         if (__builtin_expect(__builtin_amdgcn_ballot_w64(maybe) != 0, 0))
             small = maybe && all_phases_small_nv<NV>(kx, ky, A, B, kSmallPhaseMixed);
         double syx[NV], sxy[NV];
@@ -854,7 +891,7 @@ __device__ __forceinline__ void aperture_mixed_nv(
                               si);
     } else {
         small = false;
-        if (__builtin_expect(__builtin_fabsf(nin) <= (float)h[H_PHBM], 0))
+        if (__builtin_expect(__builtin_fabsf(nin) <= phase_bound_f32(h[H_PHBM]), 0))
             small = all_phases_small(vtx, nv, A, B, kSmallPhaseMixed);
         float s0, c0;
         sincos_table_f32(__builtin_fma(A, vtx[V_KX], __builtin_fma(B, vtx[V_KY], psi)), tab, s0,
@@ -1397,9 +1434,20 @@ __global__ __launch_bounds__(256) MBS_MFMA_OCCUPANCY void diffract_mfma_kernel(
         // a 32-bit trip count: the scalar unit has no 64-bit ordered compare,
         // so a 64-bit loop test would go to the vector unit
         const int nbeam = (int)(b1 - b0);
+        // B operand rows through a buffer resource based at the range's first
+        // beam: the beam's offset is a scalar register and the lane's a loop-
+        // invariant vector register, so no per-lane 64-bit address is formed
+        // or kept (a range holds far fewer than 2^24 beams; reads past it
+        // return zero)
+        const __amdgpu_buffer_rsrc_t wres = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float *>(wb + (size_t)b0 * WB_STRIDE), 0,
+            nbeam * (int)(WB_STRIDE * sizeof(float)), kBufferRsrcWord3);
+        const int lane_off = lane * (int)sizeof(float);
         for (int ib = 0; ib < nbeam; ++ib) {
             const int64_t b = b0 + ib;
-            const float bop = wb[(size_t)b * WB_STRIDE + lane];
+            const float bop = __builtin_bit_cast(
+                float, __builtin_amdgcn_raw_buffer_load_b32(
+                           wres, lane_off, ib * (int)(WB_STRIDE * sizeof(float)), 0));
             const int64_t v0 = v0_next;
             const int nvb = nv_next;
             if (ib + 1 < nbeam) {

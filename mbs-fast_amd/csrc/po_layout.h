@@ -13,7 +13,7 @@ enum {
     H_VER = 4,   // aperture vertical axis (3)
     H_B0 = 7,    // beamDir . ver
     H_KC = 8,    // -k * centre (3), zero for external beams
-    H_PHBM = 11, // the mixed kernels' small-phase bound: H_PHB for 0.25 rad
+    H_PHBM = 11, // the mixed kernels' bound for 0.25 rad (FP32, in the low word)
     H_AL = 12,   // small-phase area limit / complWave (complex)
     H_MOM = 14,  // k mX, k mY, k^2 mXX, 2 k^2 mXY, k^2 mYY (5)
     H_PHB = 19,  // max(|A|,|B|) above which some vertex phase exceeds 1e-3 rad
