@@ -1331,7 +1331,11 @@ __global__ __launch_bounds__(256) MBS_OCCUPANCY void diffract_incoherent_kernel(
 // v_mfma_f32_32x32x2f32 (two 32-direction blocks per wave).  The vector ALU
 // keeps only the aperture integral; the rotation (4 six-term dot products),
 // R.Jp and the accumulation (56 operations per beam and direction) move to
-// the matrix cores, which run beside the vector ALU.
+// the matrix cores.  The 24 columns pad to 32; the unpadded form on
+// v_mfma_f32_4x4x1f32 (16 blocks of 4 directions, 12 instructions and 48
+// bytes of operands per lane and beam) measured 30.0 ms against 28.4: its
+// operand registers cost the occupancy more than the matrix-core cycles it
+// saves.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 enum { G_LD = 25 };  // LDS row stride (floats) of the per-wave G transpose
 
