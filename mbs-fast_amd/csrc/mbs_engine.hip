@@ -109,10 +109,10 @@ struct mbs_ctx {
     bool spec_ok = false;
     int spec_n = 0;
     double pack_scale = 1.0;  // size scale of the packed beams (mbs_rescale_beams)
-    DevBuf<int32_t> spec_sidx, spec_beam, spec_logn;
+    DevBuf<int32_t> spec_sidx, spec_beam, spec_logn, spec_band;
     DevBuf<int64_t> spec_off;
     DevBuf<double> spec_scan, spec_h, spec_rec, spec_deap;
-    DevBuf<float2> spec_grid, spec_f;
+    DevBuf<float2> spec_grid, spec_f, spec_cph;
     std::vector<WorkItem> h_items;
     bool have_result = false, have_ns = false, have_per = false;
     double *pinned = nullptr;
@@ -359,6 +359,18 @@ int mbs_ctx_destroy(mbs_ctx *ctx)
     ctx->fft_ab.release();
     ctx->fft_out.release();
     ctx->fft_beams.release();
+    ctx->spec_sidx.release();
+    ctx->spec_beam.release();
+    ctx->spec_logn.release();
+    ctx->spec_band.release();
+    ctx->spec_off.release();
+    ctx->spec_scan.release();
+    ctx->spec_h.release();
+    ctx->spec_rec.release();
+    ctx->spec_deap.release();
+    ctx->spec_grid.release();
+    ctx->spec_f.release();
+    ctx->spec_cph.release();
     for (hipEvent_t &e : ctx->fft_ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -1624,22 +1636,28 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
 // Raster + row pass and column pass of one group of spectrum beams (all of
 // grid side 2^LOGN).
 template <int LOGN>
-static int launch_spectrum_fft(mbs_ctx *ctx, int nb, const double *rec, float2 *grid)
+static int launch_spectrum_fft(mbs_ctx *ctx, int nb, const double *rec, const int64_t *off,
+                               int pmax)
 {
     constexpr int N = 1 << LOGN;
-    const size_t lds_rows = (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2);
+    // the row kernel's LDS: the rows, then a queue slot per pixel
+    const size_t lds_rows =
+        (size_t)FFT_ROWS * fft_ld(N) * sizeof(float2) + (size_t)FFT_ROWS * N * sizeof(int);
     const size_t lds_cols = (size_t)FFT_COLS * fft_ld(N) * sizeof(float2);
     HIP_TRY(ctx, hipFuncSetAttribute((const void *)spectrum_rows_kernel<LOGN>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
-    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<LOGN>,
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)spectrum_cols_kernel<LOGN>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
     hipStream_t s = ctx->stream;
+    // band columns per beam: the group's widest band, in units of FFT_COLS
+    const int units = (pmax + FFT_COLS - 1) / FFT_COLS;
     const int rb = (nb * (N / FFT_ROWS) + 7) / 8 * 8;
-    const int cb = (nb * (N / FFT_COLS) + 7) / 8 * 8;
-    spectrum_rows_kernel<LOGN><<<rb, FFT_ROW_THREADS, lds_rows, s>>>(nb, rec, ctx->spec_deap.p,
-                                                                    ctx->fft_tw.p, grid);
+    const int cb = (nb * units + 7) / 8 * 8;
+    spectrum_rows_kernel<LOGN><<<rb, FFT_ROW_THREADS, lds_rows, s>>>(
+        nb, rec, ctx->spec_deap.p, ctx->fft_tw.p, ctx->spec_cph.p, off, ctx->spec_grid.p);
     HIP_TRY(ctx, hipGetLastError());
-    fft_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(nb, ctx->fft_tw.p, grid);
+    spectrum_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(
+        nb, units, rec, ctx->fft_tw.p, ctx->spec_cph.p, off, ctx->spec_grid.p);
     HIP_TRY(ctx, hipGetLastError());
     return MBS_OK;
 }
@@ -1707,7 +1725,7 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
     }
     // selection, grouped by FFT side (slots in ascending side order)
     struct Pick {
-        int beam, logn;
+        int beam, logn, band;
         double h;
     };
     std::vector<Pick> picks;
@@ -1729,7 +1747,10 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
             while ((1 << logn) < need || (1 << logn) < p->min_n)
                 ++logn;
             if ((1 << logn) <= p->max_n) {
-                picks.push_back(Pick{b, logn, h});
+                // the band of bins the taps reach (see spectrum_kernels.hip)
+                const double umax = k * amax * (double)(1 << logn) * h / (2.0 * kPi);
+                const int band = (int)std::ceil(umax + 0.5 * kSpecW) + 1;
+                picks.push_back(Pick{b, logn, band, h});
                 inf.selected_area += std::fabs(area[b]);
                 break;
             }
@@ -1738,18 +1759,21 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
     std::stable_sort(picks.begin(), picks.end(),
                      [](const Pick &x, const Pick &y) { return x.logn < y.logn; });
     const int ns = (int)picks.size();
-    std::vector<int32_t> sb(ns), sl(ns);
+    std::vector<int32_t> sb(ns), sl(ns), sm(ns), sp(ns);
     std::vector<double> sh(ns);
     std::vector<int64_t> so(ns);
     int64_t total = 0;
     int max_n = 0;
     for (int i = 0; i < ns; ++i) {
+        const int n = 1 << picks[i].logn;
         sb[i] = picks[i].beam;
         sl[i] = picks[i].logn;
         sh[i] = picks[i].h;
+        sm[i] = picks[i].band;
+        sp[i] = 2 * sm[i] + 1 < n ? 2 * sm[i] + 1 : n;  // as spectrum_setup_kernel
         so[i] = total;
         sidx[picks[i].beam] = i;
-        total += (int64_t)1 << (2 * picks[i].logn);
+        total += (int64_t)n * sp[i];  // [N][P] for the row pass, [P][P] kept
         max_n = std::max(max_n, 1 << picks[i].logn);
         ++inf.n_by_size[picks[i].logn - 6];
     }
@@ -1760,6 +1784,8 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
     if (ns > 0) {
         HIP_TRY(ctx, ctx->spec_beam.ensure(ns));
         HIP_TRY(ctx, ctx->spec_logn.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_band.ensure(ns));
+        HIP_TRY(ctx, ctx->spec_cph.ensure(max_n + 1));
         HIP_TRY(ctx, ctx->spec_h.ensure(ns));
         HIP_TRY(ctx, ctx->spec_off.ensure(ns));
         HIP_TRY(ctx, ctx->spec_rec.ensure((size_t)ns * SP_STRIDE));
@@ -1773,10 +1799,13 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
                                     hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_h.p, sh.data(), ns * sizeof(double),
                                     hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_band.p, sm.data(), ns * sizeof(int32_t),
+                                    hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->spec_off.p, so.data(), ns * sizeof(int64_t),
                                     hipMemcpyHostToDevice, s));
         HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
         spectrum_setup_kernel<<<(ns + 63) / 64, 64, 0, s>>>(ns, ctx->spec_beam.p, ctx->spec_h.p,
+                                                            ctx->spec_logn.p, ctx->spec_band.p,
                                                             ctx->nv.p, ctx->voff.p,
                                                             ctx->vx_in.p, ctx->vy_in.p,
                                                             ctx->pack_scale, ctx->spec_rec.p);
@@ -1786,18 +1815,23 @@ int mbs_spectrum_source(mbs_ctx *ctx, const mbs_spectrum_params *p, mbs_spectrum
             while (i1 < ns && sl[i1] == sl[i0])
                 ++i1;
             const int n = 1 << sl[i0];
+            int pmax = 0;
+            for (int i = i0; i < i1; ++i)
+                pmax = std::max(pmax, sp[i]);
             fft_twiddle_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, ctx->fft_tw.p);
-            spectrum_deap_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, ctx->spec_deap.p);
+            spectrum_deap_kernel<<<(n + 3) / 4, 256, 0, s>>>(n, ctx->spec_deap.p);
+            spectrum_phase_kernel<<<(n + 256) / 256, 256, 0, s>>>(n, ctx->spec_cph.p);
             HIP_TRY(ctx, hipGetLastError());
             const double *rec = ctx->spec_rec.p + (size_t)i0 * SP_STRIDE;
-            float2 *grid = ctx->spec_grid.p + so[i0];
+            const int64_t *off = ctx->spec_off.p + i0;
+            const int g = i1 - i0;
             switch (sl[i0]) {
-            case 6: rc = launch_spectrum_fft<6>(ctx, i1 - i0, rec, grid); break;
-            case 7: rc = launch_spectrum_fft<7>(ctx, i1 - i0, rec, grid); break;
-            case 8: rc = launch_spectrum_fft<8>(ctx, i1 - i0, rec, grid); break;
-            case 9: rc = launch_spectrum_fft<9>(ctx, i1 - i0, rec, grid); break;
-            case 10: rc = launch_spectrum_fft<10>(ctx, i1 - i0, rec, grid); break;
-            case 11: rc = launch_spectrum_fft<11>(ctx, i1 - i0, rec, grid); break;
+            case 6: rc = launch_spectrum_fft<6>(ctx, g, rec, off, pmax); break;
+            case 7: rc = launch_spectrum_fft<7>(ctx, g, rec, off, pmax); break;
+            case 8: rc = launch_spectrum_fft<8>(ctx, g, rec, off, pmax); break;
+            case 9: rc = launch_spectrum_fft<9>(ctx, g, rec, off, pmax); break;
+            case 10: rc = launch_spectrum_fft<10>(ctx, g, rec, off, pmax); break;
+            case 11: rc = launch_spectrum_fft<11>(ctx, g, rec, off, pmax); break;
             default: rc = fail(ctx, MBS_ERR_INVALID, "FFT side %d", n);
             }
             if (rc)

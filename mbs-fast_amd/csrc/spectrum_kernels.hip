@@ -18,8 +18,17 @@
 //      integrated over the clipped polygon from its moments;
 //   2. deapodisation: each raster value divided by the transform of the
 //      interpolation kernel at its position (below);
-//   3. the 2-D FFT (the row and column passes of the aperture FFT,
-//      fft_kernels.hip), N = 2 x the padded extent (oversampling 2);
+//   3. the 2-D FFT (the Stockham stages of the aperture FFT,
+//      fft_kernels.hip), N = 2 x the padded extent (oversampling 2), pruned
+//      to the band of bins the grid samples: with every |A|, |B| <= a_max
+//      the taps stay within M = ceil(k a_max N h / 2 pi + W/2) + 1 bins of
+//      zero, so the row pass stores only those P = 2M + 1 columns of each
+//      row's transform and the column pass transforms only those columns
+//      and stores P rows (P / N of the column work and traffic, (P / N)^2 of
+//      the footprint).  Rows whose tent support misses the polygon are zero
+//      and skip their transform.  Each stored bin (mx, my) is multiplied by
+//      the phases e^{i pi (N - 1) m / N} that centre the pixel grid, once,
+//      instead of at each of the W^2 taps of every sample;
 //   4. sampling at the grid's cells (spectrum_sample_kernel): the spectrum
 //      interpolated with the "exponential of semicircle" kernel
 //      phi(z) = exp(beta (sqrt(1 - z^2) - 1)) over W bins per axis
@@ -72,8 +81,16 @@ enum {
     SP_VX = 5,
     SP_VY = SP_VX + kSpecMaxNv,
     SP_EDGE = SP_VY + kSpecMaxNv,  // (a, b, c, r) per edge
-    SP_STRIDE = SP_EDGE + 4 * kSpecMaxNv
+    SP_M = SP_EDGE + 4 * kSpecMaxNv,  // band half-width (bins)
+    SP_P = SP_M + 1,                   // band width 2M + 1, or N unpruned
+    SP_YLO = SP_P + 1,                 // the polygon's y range
+    SP_YHI = SP_YLO + 1,
+    SP_STRIDE = SP_YHI + 1
 };
+
+// Band column of signed bin m (|m| <= M, or any m in [-N/2, N/2) when
+// P = N): non-negative bins first, then the negative ones.
+__device__ __forceinline__ int spec_band_col(int m, int p) { return m >= 0 ? m : m + p; }
 
 // What the selection needs of every beam, one workgroup per beam:
 // max over the grid's cells of max(|A|, |B|), the polygon's larger bounding
@@ -150,6 +167,8 @@ __global__ void __launch_bounds__(256) spectrum_scan_kernel(
 // group's grid size) come from the host's selection.
 __global__ void spectrum_setup_kernel(int n_sb, const int32_t *__restrict__ beam,
                                       const double *__restrict__ hpx,
+                                      const int32_t *__restrict__ logn,
+                                      const int32_t *__restrict__ band,
                                       const int32_t *__restrict__ nv_in,
                                       const int64_t *__restrict__ voff,
                                       const double *__restrict__ vx,
@@ -182,6 +201,12 @@ __global__ void spectrum_setup_kernel(int n_sb, const int32_t *__restrict__ beam
     r[SP_H] = h;
     r[SP_SG] = sg;
     r[SP_NV] = nv;
+    const int n = 1 << logn[i];
+    const int m = band[i];
+    r[SP_M] = 2 * m + 1 < n ? m : n / 2;
+    r[SP_P] = 2 * m + 1 < n ? 2 * m + 1 : n;
+    r[SP_YLO] = y0;
+    r[SP_YHI] = y1;
     // vertices counter-clockwise (reversed when the list runs clockwise)
     for (int e = 0; e < nv; ++e) {
         const int s = sg > 0.0 ? e : nv - 1 - e;
@@ -203,100 +228,110 @@ __global__ void spectrum_setup_kernel(int n_sb, const int32_t *__restrict__ beam
 // Transform of the interpolation kernel, phi_hat(s) = integral over
 // |u| <= W/2 (bins) of phi(2u/W) cos(2 pi u s) du, at the position
 // s = (j - N/2 + 1/2) / N of raster column j; deap[j] = 1 / phi_hat.  Midpoint
-// rule on 1024 nodes over [0, W/2] (the integrand is even).
-__global__ void spectrum_deap_kernel(int n, double *__restrict__ deap)
+// rule on 1024 nodes over [0, W/2] (the integrand is even): one wave per
+// column, 16 nodes per lane, then a fixed-order butterfly sum.
+__global__ void __launch_bounds__(256) spectrum_deap_kernel(int n, double *__restrict__ deap)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (j >= n)
-        return;
+        return;  // (whole waves)
     const double s = ((double)j - 0.5 * n + 0.5) / n;
     const int q = 1024;
     const double du = 0.5 * kSpecW / q;
     double acc = 0.0;
-    for (int i = 0; i < q; ++i) {
+    for (int i = lane; i < q; i += 64) {
         const double u = (i + 0.5) * du;
         const double z = 2.0 * u / kSpecW;
         acc += exp(kSpecBeta * (sqrt(1.0 - z * z) - 1.0)) * cospi(2.0 * u * s);
     }
-    deap[j] = 1.0 / (2.0 * du * acc);
-}
-
-// Sutherland-Hodgman clip of a polygon (counter-clockwise, local
-// coordinates) to u >= lo (axis 0: u, axis 1: v; dir +1) or u <= hi
-// (dir -1).  Returns the new vertex count.
-__device__ __forceinline__ int spec_clip(const double *iu, const double *iv, int n, double *ou,
-                                         double *ov, int axis, double lim, double dir)
-{
-    int m = 0;
-    for (int i = 0; i < n; ++i) {
-        const int j = i + 1 < n ? i + 1 : 0;
-        const double pi_ = axis ? iv[i] : iu[i], pj = axis ? iv[j] : iu[j];
-        const bool ini = dir * (pi_ - lim) >= 0.0, inj = dir * (pj - lim) >= 0.0;
-        if (ini) {
-            ou[m] = iu[i];
-            ov[m] = iv[i];
-            ++m;
-        }
-        if (ini != inj) {
-            const double t = (lim - pi_) / (pj - pi_);
-            ou[m] = __builtin_fma(t, iu[j] - iu[i], iu[i]);
-            ov[m] = __builtin_fma(t, iv[j] - iv[i], iv[i]);
-            ++m;
-        }
-    }
-    return m;
+    for (int w = 32; w > 0; w >>= 1)
+        acc += __shfl_xor(acc, w, 64);
+    if (lane == 0)
+        deap[j] = 1.0 / (2.0 * du * acc);
 }
 
 // Tent-weighted integral of the polygon over the support of the pixel at
-// (px, py), over h^2 (1 for a pixel whose support is inside): the polygon
-// clipped to each quadrant of [-h, h]^2 around the pixel, the weight
-// (h - qx u)(h - qy v) / h^2 integrated from the clipped polygon's moments
-// (area, first moments and the uv moment, by Green's theorem).
+// (px, py), over h^2 (1 for a pixel whose support is inside).  By Green's
+// theorem, with the cumulative tent T(u) = integral of t up to u,
+//   integral over P of t(x - px) t(y - py) = sum over the (counter-
+//   clockwise) edges of integral T(x(y) - px) t(y - py) dy,
+// and along an edge the integrand is a cubic in y between the kinks of t
+// (y = py; the band |y - py| <= h bounds the integral) and of T (where
+// x(y) - px = -h, 0, h), so Simpson's rule on the pieces is exact.  Edges
+// that miss the band contribute nothing; registers only.
+__device__ __forceinline__ double spec_tent_T(double u, double h)
+{
+    if (u <= -h)
+        return 0.0;
+    if (u <= 0.0)
+        return (u + h) * (u + h) / (2.0 * h);
+    if (u <= h)
+        return h - (h - u) * (h - u) / (2.0 * h);
+    return h;
+}
+
 __device__ double spec_tent_exact(const double *__restrict__ r, double px, double py, double h)
 {
     const int nv = (int)r[SP_NV];
-    double bu[2][kSpecMaxNv + 8], bv[2][kSpecMaxNv + 8];
+    const double ylo = py - h, yhi = py + h;
     double tot = 0.0;
-    for (int q = 0; q < 4; ++q) {
-        const double qx = (q & 1) ? 1.0 : -1.0, qy = (q & 2) ? 1.0 : -1.0;
-        for (int e = 0; e < nv; ++e) {
-            bu[0][e] = r[SP_VX + e] - px;
-            bv[0][e] = r[SP_VY + e] - py;
-        }
-        int n = nv;
-        // quadrant box: u between 0 and qx h, v between 0 and qy h
-        n = spec_clip(bu[0], bv[0], n, bu[1], bv[1], 0, 0.0, qx);
-        if (n)
-            n = spec_clip(bu[1], bv[1], n, bu[0], bv[0], 0, qx * h, -qx);
-        if (n)
-            n = spec_clip(bu[0], bv[0], n, bu[1], bv[1], 1, 0.0, qy);
-        if (n)
-            n = spec_clip(bu[1], bv[1], n, bu[0], bv[0], 1, qy * h, -qy);
-        if (n < 3)
+    for (int e = 0; e < nv; ++e) {
+        const int f = e + 1 < nv ? e + 1 : 0;
+        const double x0 = r[SP_VX + e], y0 = r[SP_VY + e];
+        const double x1 = r[SP_VX + f], y1 = r[SP_VY + f];
+        if (y0 == y1)
             continue;
-        double a = 0.0, mu = 0.0, mv = 0.0, muv = 0.0;
-        for (int i = 0; i < n; ++i) {
-            const int j = i + 1 < n ? i + 1 : 0;
-            const double u0 = bu[0][i], v0 = bv[0][i], u1 = bu[0][j], v1 = bv[0][j];
-            const double c = u0 * v1 - u1 * v0;
-            a += c;
-            mu += (u0 + u1) * c;
-            mv += (v0 + v1) * c;
-            muv += (u0 * v1 + 2.0 * u0 * v0 + 2.0 * u1 * v1 + u1 * v0) * c;
+        const double ya = fmax(fmin(y0, y1), ylo), yb = fmin(fmax(y0, y1), yhi);
+        if (!(ya < yb))
+            continue;
+        const double m = (x1 - x0) / (y1 - y0);  // dx/dy
+        // the kinks, clamped into [ya, yb] (outside ones give empty pieces)
+        double c[4];
+        c[0] = py;
+        if (x1 != x0) {
+            const double im = (y1 - y0) / (x1 - x0);
+            c[1] = __builtin_fma(px - h - x0, im, y0);
+            c[2] = __builtin_fma(px - x0, im, y0);
+            c[3] = __builtin_fma(px + h - x0, im, y0);
+        } else {
+            c[1] = c[2] = c[3] = ya;
         }
-        a *= 0.5;
-        mu *= 1.0 / 6.0;
-        mv *= 1.0 / 6.0;
-        muv *= 1.0 / 24.0;
-        tot += h * h * a - h * qx * mu - h * qy * mv + qx * qy * muv;
+        for (int q = 0; q < 4; ++q)
+            c[q] = fmin(fmax(c[q], ya), yb);
+        // sorting network on four values
+        auto cx = [](double &a, double &b) {
+            const double lo = fmin(a, b), hi = fmax(a, b);
+            a = lo;
+            b = hi;
+        };
+        cx(c[0], c[1]);
+        cx(c[2], c[3]);
+        cx(c[0], c[2]);
+        cx(c[1], c[3]);
+        cx(c[1], c[2]);
+        auto fy = [&](double y) {
+            return spec_tent_T(__builtin_fma(y - y0, m, x0) - px, h) *
+                   fmax(0.0, 1.0 - fabs(y - py) / h);
+        };
+        const double pts[6] = {ya, c[0], c[1], c[2], c[3], yb};
+        double fa = fy(pts[0]), s = 0.0;
+        for (int q = 0; q < 5; ++q) {
+            const double a = pts[q], b = pts[q + 1];
+            const double fb = fy(b);
+            s += (b - a) * (fa + 4.0 * fy(0.5 * (a + b)) + fb);
+            fa = fb;
+        }
+        tot += (y1 > y0 ? s : -s) * (1.0 / 6.0);
     }
-    return tot / (h * h * h * h);
+    return tot / (h * h);
 }
 
-// Raster value of pixel (px, py): 1 where the tent's support lies inside
-// every edge line, 0 where it lies outside one, else the exact tent
-// integral.
-__device__ __forceinline__ double spec_pixel(const double *__restrict__ r, double px, double py)
+// Class of the pixel at (px, py): 0 where the tent's support lies outside
+// an edge line, 1 where it lies inside every edge line, 2 otherwise (the
+// value is then the exact tent integral).
+__device__ __forceinline__ int spec_pixel_class(const double *__restrict__ r, double px,
+                                                double py)
 {
     const int nv = (int)r[SP_NV];
     bool inside = true;
@@ -304,18 +339,41 @@ __device__ __forceinline__ double spec_pixel(const double *__restrict__ r, doubl
         const double *l = r + SP_EDGE + 4 * e;
         const double v = __builtin_fma(l[0], px, __builtin_fma(l[1], py, l[2]));
         if (v <= -l[3])
-            return 0.0;  // the whole support is outside this edge's line
+            return 0;  // the whole support is outside this edge's line
         inside = inside && v > l[3];
     }
-    return inside ? 1.0 : spec_tent_exact(r, px, py, r[SP_H]);
+    return inside ? 1 : 2;
+}
+
+// e^{i pi (n - 1) m / n} for signed bins m in [-n/2, n/2] (cph[m + n/2]),
+// the phase that centres bin m of an n-point transform of pixel-centred
+// samples, with the angle reduced exactly in integers: (n - 1) m mod 2n.
+__global__ void spectrum_phase_kernel(int n, float2 *__restrict__ cph)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n)
+        return;
+    const long long m = q - n / 2;
+    const long long a = (((long long)(n - 1) * m) % (2LL * n) + 2LL * n) % (2LL * n);
+    double sn, cs;
+    sincospi((double)a / n, &sn, &cs);
+    cph[q] = make_float2((float)cs, (float)sn);
 }
 
 // Raster + deapodisation + row FFTs of the spectrum beams of one group (all
-// of grid size 2^LOGN): as fft_rows_kernel, with the tent raster.
+// of grid size 2^LOGN), as fft_rows_kernel with the tent raster; each row
+// stores its band of P columns ([N rows][P], bin mx at spec_band_col),
+// times the centring phase of mx.  The raster goes to LDS first: pixels
+// inside or outside are written at once, the boundary pixels (a few per
+// row, many in a row along a near-horizontal edge) are queued and their
+// exact tent integrals taken one per lane (evaluated where they fall, most
+// lanes of a wave would idle through each one); the first FFT stage then
+// reads LDS.  LDS: FFT_ROWS x fft_ld(N) float2 and FFT_ROWS x N queue slots.
 template <int LOGN>
 __global__ void __launch_bounds__(FFT_ROW_THREADS) spectrum_rows_kernel(
     int n_sb, const double *__restrict__ rec, const double *__restrict__ deap,
-    const float2 *__restrict__ tw, float2 *__restrict__ grid)
+    const float2 *__restrict__ tw, const float2 *__restrict__ cph,
+    const int64_t *__restrict__ off, float2 *__restrict__ spec)
 {
     constexpr int N = 1 << LOGN;
     extern __shared__ float2 lds[];
@@ -327,32 +385,93 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) spectrum_rows_kernel(
     const int row0 = (unit - beam * units_per_beam) * FFT_ROWS;
     const double *r = rec + (size_t)beam * SP_STRIDE;
     const double xc = r[SP_XC], yc = r[SP_YC], h = r[SP_H];
-    float2 *g = grid + ((size_t)beam * N + row0) * N;
-    auto raster = [&](int seq, int i) {
-        const int row = row0 + seq;
-        const double px = fft_pixel(i, N, h, xc), py = fft_pixel(row, N, h, yc);
-        const double v = spec_pixel(r, px, py);
-        return make_float2((float)(v * deap[row] * deap[i]), 0.0f);
+    const int M = (int)r[SP_M], P = (int)r[SP_P];
+    float2 *g = spec + off[beam] + (size_t)row0 * P;
+    // rows whose tent support [py - h, py + h] misses the polygon's y range
+    // are zero: their band is written as such, with no transform
+    const double ylo = fft_pixel(row0, N, h, yc) - h;
+    const double yhi = fft_pixel(row0 + FFT_ROWS - 1, N, h, yc) + h;
+    if (yhi <= r[SP_YLO] || ylo >= r[SP_YHI]) {
+        for (int q = threadIdx.x; q < FFT_ROWS * P; q += blockDim.x)
+            g[q] = make_float2(0.0f, 0.0f);
+        return;
+    }
+    constexpr int LD = fft_ld(N);
+    int *queue = reinterpret_cast<int *>(lds + FFT_ROWS * LD);
+    __shared__ int s_count;
+    if (threadIdx.x == 0)
+        s_count = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < FFT_ROWS * N; q += blockDim.x) {
+        const int seq = q / N, i = q - seq * N;
+        const int cls =
+            spec_pixel_class(r, fft_pixel(i, N, h, xc), fft_pixel(row0 + seq, N, h, yc));
+        float v = 0.0f;
+        if (cls == 1)
+            v = (float)(1.0 * deap[row0 + seq] * deap[i]);
+        else if (cls == 2)
+            queue[atomicAdd(&s_count, 1)] = q;
+        lds[seq * LD + fft_pad(i)] = make_float2(v, 0.0f);
+    }
+    __syncthreads();
+    const int count = s_count;
+    for (int t = threadIdx.x; t < count; t += blockDim.x) {
+        const int q = queue[t];
+        const int seq = q / N, i = q - seq * N;
+        const double v =
+            spec_tent_exact(r, fft_pixel(i, N, h, xc), fft_pixel(row0 + seq, N, h, yc), h);
+        lds[seq * LD + fft_pad(i)] = make_float2((float)(v * deap[row0 + seq] * deap[i]), 0.0f);
+    }
+    __syncthreads();
+    auto raster = [&](int seq, int i) { return lds[seq * LD + fft_pad(i)]; };
+    auto out = [&](int seq, int i, float2 x) {
+        const int m = i <= M ? i : (i >= N - M ? i - N : N);
+        if (m != N)
+            g[(size_t)seq * P + spec_band_col(m, P)] = cmulf(x, cph[m + N / 2]);
     };
-    auto out = [&](int seq, int i, float2 x) { g[(size_t)seq * N + i] = x; };
-    fft_stages<LOGN, FFT_ROWS, FFT_ROW_THREADS, false, 0>(lds, fft_ld(N), tw, raster, out);
+    fft_stages<LOGN, FFT_ROWS, FFT_ROW_THREADS, false, 0>(lds, LD, tw, raster, out);
 }
 
-// e^{i pi (n - 1) j / n}, the phase that centres bin j of an n-point
-// transform of pixel-centred samples, with the angle reduced exactly in
-// integers: (n - 1) j mod 2n.
-__device__ __forceinline__ float2 spec_centre_phase(int j, int n)
+// Column FFTs of the band: work unit = (beam, FFT_COLS adjacent band
+// columns); reads all N rows of its columns, stores rows my in the band
+// (the first P rows, bin my at spec_band_col) times the centring phase of
+// my.  In place: a unit owns its columns, and the first stage reads them
+// whole before the last one writes.
+template <int LOGN>
+__global__ void __launch_bounds__(FFT_COL_THREADS) spectrum_cols_kernel(
+    int n_sb, int units_per_beam, const double *__restrict__ rec,
+    const float2 *__restrict__ tw, const float2 *__restrict__ cph,
+    const int64_t *__restrict__ off, float2 *__restrict__ spec)
 {
-    const long long m = (((long long)(n - 1) * j) % (2LL * n) + 2LL * n) % (2LL * n);
-    float s, c;
-    sincospif((float)((double)m / n), &s, &c);
-    return make_float2(c, s);
+    constexpr int N = 1 << LOGN;
+    extern __shared__ float2 lds[];
+    const int unit = fft_logical_block();
+    if (unit >= n_sb * units_per_beam)
+        return;
+    const int beam = unit / units_per_beam;
+    const int col0 = (unit - beam * units_per_beam) * FFT_COLS;
+    const double *r = rec + (size_t)beam * SP_STRIDE;
+    const int M = (int)r[SP_M], P = (int)r[SP_P];
+    if (col0 >= P)
+        return;
+    float2 *g = spec + off[beam] + col0;
+    const int ncol = P - col0;  // columns of this unit inside the band
+    auto in = [&](int seq, int i) {
+        return seq < ncol ? g[(size_t)i * P + seq] : make_float2(0.0f, 0.0f);
+    };
+    auto out = [&](int seq, int i, float2 x) {
+        const int m = i <= M ? i : (i >= N - M ? i - N : N);
+        if (seq < ncol && m != N)
+            g[(size_t)spec_band_col(m, P) * P + seq] = cmulf(x, cph[m + N / 2]);
+    };
+    fft_stages<LOGN, FFT_COLS, FFT_COL_THREADS, true, 0>(lds, fft_ld(N), tw, in, out);
 }
 
-// F of spectrum beam i at every cell: thread per (i, cell).  spec[i] is the
-// offset of beam i's spectrum, logn[i] its grid size.  fbuf: [n_sb][cells].
+// F of spectrum beam i at every cell: thread per (i, cell).  off[i] is the
+// offset of beam i's band [P][P] (centring phases applied), logn[i] its grid
+// size.  fbuf: [n_sb][cells].
 __global__ void __launch_bounds__(256) spectrum_sample_kernel(
-    int n_sb, int cells, const int32_t *__restrict__ beam, const int64_t *__restrict__ spec,
+    int n_sb, int cells, const int32_t *__restrict__ beam, const int64_t *__restrict__ off,
     const int32_t *__restrict__ logn, const double *__restrict__ rec,
     const float2 *__restrict__ spectra, const double *__restrict__ hdr,
     const double *__restrict__ tab, double k, float2 *__restrict__ fbuf)
@@ -365,6 +484,7 @@ __global__ void __launch_bounds__(256) spectrum_sample_kernel(
     const double *h = hdr + (size_t)beam[i] * HDR_STRIDE;
     const double *r = rec + (size_t)i * SP_STRIDE;
     const int n = 1 << logn[i];
+    const int P = (int)r[SP_P];
     const double d0 = tab[(size_t)(DT_DIR + 0) * cells + c];
     const double d1 = tab[(size_t)(DT_DIR + 1) * cells + c];
     const double d2 = tab[(size_t)(DT_DIR + 2) * cells + c];
@@ -374,34 +494,38 @@ __global__ void __launch_bounds__(256) spectrum_sample_kernel(
     const double px_ = r[SP_H];
     const double inv2pi = 0.15915494309189533577;
     const double fx = -k * A * inv2pi, fy = -k * B * inv2pi;
-    // positions in bins; the W taps nearest each
+    // positions in bins; the W taps nearest each (inside the band by the
+    // choice of M)
     const double ux = fx * n * px_, uy = fy * n * px_;
     const int jx0 = (int)ceil(ux - 0.5 * kSpecW), jy0 = (int)ceil(uy - 0.5 * kSpecW);
     // kernel weights in FP32 (relative error ~1e-7, below the spectrum's)
-    float2 wx[kSpecW], wy[kSpecW];
+    float wx[kSpecW], wy[kSpecW];
     for (int t = 0; t < kSpecW; ++t) {
         const float zx = (float)(2.0 * (ux - (jx0 + t)) / kSpecW);
         const float zy = (float)(2.0 * (uy - (jy0 + t)) / kSpecW);
         const float beta = (float)kSpecBeta;
-        const float ex = fabsf(zx) < 1.0f ? expf(beta * (sqrtf(1.0f - zx * zx) - 1.0f)) : 0.0f;
-        const float ey = fabsf(zy) < 1.0f ? expf(beta * (sqrtf(1.0f - zy * zy) - 1.0f)) : 0.0f;
-        const float2 px = spec_centre_phase(jx0 + t, n), py = spec_centre_phase(jy0 + t, n);
-        wx[t] = make_float2(ex * px.x, ex * px.y);
-        wy[t] = make_float2(ey * py.x, ey * py.y);
+        wx[t] = fabsf(zx) < 1.0f ? expf(beta * (sqrtf(1.0f - zx * zx) - 1.0f)) : 0.0f;
+        wy[t] = fabsf(zy) < 1.0f ? expf(beta * (sqrtf(1.0f - zy * zy) - 1.0f)) : 0.0f;
     }
-    const float2 *g = spectra + spec[i];
+    const float2 *g = spectra + off[i];
     float sr = 0.0f, si = 0.0f;
+    // (a tap outside the band cannot occur; it would read as zero)
+    const int M = (int)r[SP_M];
     for (int ty = 0; ty < kSpecW; ++ty) {
-        const int iy = (((jy0 + ty) % n) + n) % n;
+        const int my = jy0 + ty;
+        if (my < -M || my > M)
+            continue;
+        const float2 *row = g + (size_t)spec_band_col(my, P) * P;
         float rr = 0.0f, ri = 0.0f;
         for (int tx = 0; tx < kSpecW; ++tx) {
-            const int ix = (((jx0 + tx) % n) + n) % n;
-            const float2 z = g[(size_t)iy * n + ix];
-            rr = __builtin_fmaf(z.x, wx[tx].x, __builtin_fmaf(-z.y, wx[tx].y, rr));
-            ri = __builtin_fmaf(z.x, wx[tx].y, __builtin_fmaf(z.y, wx[tx].x, ri));
+            const int mx = jx0 + tx;
+            const float2 z = (mx < -M || mx > M) ? make_float2(0.0f, 0.0f)
+                                                 : row[spec_band_col(mx, P)];
+            rr = __builtin_fmaf(z.x, wx[tx], rr);
+            ri = __builtin_fmaf(z.y, wx[tx], ri);
         }
-        sr = __builtin_fmaf(rr, wy[ty].x, __builtin_fmaf(-ri, wy[ty].y, sr));
-        si = __builtin_fmaf(rr, wy[ty].y, __builtin_fmaf(ri, wy[ty].x, si));
+        sr = __builtin_fmaf(rr, wy[ty], sr);
+        si = __builtin_fmaf(ri, wy[ty], si);
     }
     // the tent's sinc^2 per axis, the pixel area, s k^2 and the phase of
     // the raster centre and of the beam centre
