@@ -16,9 +16,13 @@
 // Layout: one grid per beam, row-major [n rows (y)][n columns (x)] of float2,
 // the whole batch contiguous.  Two passes over HBM:
 //   fft_rows_kernel   rasterises ROWS rows into LDS (no global read), does
-//                     their n-point FFTs in LDS, writes them (n^2 x 8 B)
-//   fft_cols_kernel   reads COLS adjacent columns (64-byte row segments),
-//                     does their FFTs in LDS, writes them back (2 n^2 x 8 B)
+//                     their n-point FFTs in LDS, writes them (n^2 x 8 B at
+//                     most: rows of pixel centres outside the polygon's y
+//                     range are zero and skipped -- with padding p, all but
+//                     about 1/p of the rows)
+//   fft_cols_kernel   reads COLS adjacent columns (32-byte row segments) of
+//                     the rows written, the others as zero, does their FFTs
+//                     in LDS, writes them back (n^2 x 8 B)
 // The 1-D FFT is the Stockham autosort formulation (Govindaraju et al.,
 // "High performance discrete Fourier transforms on graphics processors",
 // SC08): radix-8 stages and one radix-4 or radix-2 stage, each thread taking
@@ -38,11 +42,12 @@ namespace mbs {
 // Workgroups: 128 threads per row or column.  The row kernel holds FFT_ROWS
 // rows: one row (17 KB of LDS at n = 2048, nine workgroups per CU) ran the
 // row pass 16 % faster than four (68 KB, two per CU), whose barriers stall
-// more of the CU at once.  The column kernel holds FFT_COLS columns (8: one
-// 135 KB workgroup of 1024 threads per CU; 4 columns, two 68 KB workgroups,
-// measured the same).
+// more of the CU at once.  The column kernel holds FFT_COLS columns: 4 (two
+// 68 KB workgroups of 512 threads per CU) ran the column pass of 64 padded
+// 2048^2 grids in 1.13 ms, 8 (one 135 KB workgroup) in 1.31 ms and 2 in
+// 1.54 ms.
 #ifndef MBS_FFT_COLS
-#define MBS_FFT_COLS 8
+#define MBS_FFT_COLS 4
 #endif
 #ifndef MBS_FFT_ROWS
 #define MBS_FFT_ROWS 1
@@ -53,7 +58,17 @@ enum {
     FFT_COLS = MBS_FFT_COLS,
     FFT_COL_THREADS = MBS_FFT_COLS * 128
 };
-enum { FB_XC = 0, FB_YC = 1, FB_DX = 2, FB_DY = 3, FB_NE = 4, FB_EDGE = 5, FB_STRIDE = 5 + 3 * 32 };
+enum {
+    FB_XC = 0,
+    FB_YC = 1,
+    FB_DX = 2,
+    FB_DY = 3,
+    FB_NE = 4,
+    FB_EDGE = 5,                // (a, b, c) per edge, up to 32 edges
+    FB_YLO = FB_EDGE + 3 * 32,  // the polygon's y range
+    FB_YHI = FB_YLO + 1,
+    FB_STRIDE = FB_YHI + 1
+};
 
 // Per-beam raster set-up: bounding box, pixel size and the edge lines
 // a x + b y + c (> 0 inside) of the convex aperture polygon.  One thread per
@@ -86,6 +101,8 @@ __global__ void fft_setup_kernel(int n_beams, const int64_t *__restrict__ beams,
     r[FB_DX] = pad * (x1 - x0) / n;
     r[FB_DY] = pad * (y1 - y0) / n;
     r[FB_NE] = nv;
+    r[FB_YLO] = y0;
+    r[FB_YHI] = y1;
     const double sg = area2 >= 0.0 ? 1.0 : -1.0;  // inside on the left of each edge
     for (int e = 0; e < nv; ++e) {
         const int f = e + 1 < nv ? e + 1 : 0;
@@ -244,6 +261,14 @@ __device__ __forceinline__ void fft_stages(float2 *buf, int ld, const float2 *__
 // sequences (the columns of the column pass) start 16 banks apart.
 __host__ __device__ constexpr int fft_ld(int n) { return n + (n >> 5) + 8; }
 
+// Row of pixel centres outside the polygon's y range: all zero.  Such rows
+// are neither transformed nor stored; the column pass reads them as zero.
+__device__ __forceinline__ bool fft_zero_row(const double *__restrict__ r, int row, int n)
+{
+    const double py = fft_pixel(row, n, r[FB_DY], r[FB_YC]);
+    return py < r[FB_YLO] || py > r[FB_YHI];
+}
+
 // Raster + row FFTs.  Work unit = (beam, FFT_ROWS rows); LDS FFT_ROWS x
 // fft_ld(n) float2 plus FFT_ROWS x 32 doubles of per-row edge offsets.  Row
 // pixels are rasterised as stage 0 reads them.
@@ -261,6 +286,11 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
     const int beam = unit / units_per_beam;
     const int row0 = (unit - beam * units_per_beam) * FFT_ROWS;
     const double *r = rec + (size_t)beam * FB_STRIDE;
+    bool all_zero = true;
+    for (int rr = 0; rr < FFT_ROWS; ++rr)
+        all_zero = all_zero && fft_zero_row(r, row0 + rr, N);
+    if (all_zero)
+        return;
     const int ne = (int)r[FB_NE];
     const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
     float2 *g = grid + ((size_t)beam * N + row0) * N;
@@ -287,10 +317,12 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
 
 // Column FFTs.  Work unit = (beam, FFT_COLS adjacent columns); LDS FFT_COLS x
 // fft_ld(n) float2.  Lanes take columns fastest, so a wave reads and writes
-// 64-byte row segments.
+// FFT_COLS x 8-byte row segments; rows outside the polygon's y range read as
+// zero without a load.
 template <int LOGN>
 __global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
-    int n_beams, const float2 *__restrict__ tw, float2 *__restrict__ grid)
+    int n_beams, const double *__restrict__ rec, const float2 *__restrict__ tw,
+    float2 *__restrict__ grid)
 {
     constexpr int N = 1 << LOGN;
     extern __shared__ float2 lds[];
@@ -301,7 +333,10 @@ __global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
     const int beam = unit / units_per_beam;
     const int col0 = (unit - beam * units_per_beam) * FFT_COLS;
     float2 *g = grid + (size_t)beam * N * N + col0;
-    auto in = [&](int seq, int i) { return g[(size_t)i * N + seq]; };
+    const double *r = rec + (size_t)beam * FB_STRIDE;
+    auto in = [&](int seq, int i) {
+        return fft_zero_row(r, i, N) ? make_float2(0.0f, 0.0f) : g[(size_t)i * N + seq];
+    };
     auto out = [&](int seq, int i, float2 x) { g[(size_t)i * N + seq] = x; };
     fft_stages<LOGN, FFT_COLS, FFT_COL_THREADS, true, 0>(lds, fft_ld(N), tw, in, out);
 }

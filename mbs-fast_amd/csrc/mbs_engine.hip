@@ -1300,7 +1300,7 @@ static int launch_fft(mbs_ctx *ctx, int nb)
     HIP_TRY(ctx, hipGetLastError());
     if (ctx->timing)
         HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
-    fft_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(nb, ctx->fft_tw.p,
+    fft_cols_kernel<LOGN><<<cb, FFT_COL_THREADS, lds_cols, s>>>(nb, ctx->fft_rec.p, ctx->fft_tw.p,
                                                                ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
     if (ctx->timing)

@@ -25,8 +25,8 @@
 //      zero, so the row pass stores only those P = 2M + 1 columns of each
 //      row's transform and the column pass transforms only those columns
 //      and stores P rows (P / N of the column work and traffic, (P / N)^2 of
-//      the footprint).  Rows whose tent support misses the polygon are zero
-//      and skip their transform.  Each stored bin (mx, my) is multiplied by
+//      the footprint).  Rows whose tent support misses the polygon are zero:
+//      neither transformed nor stored, and read as zero by the column pass.  Each stored bin (mx, my) is multiplied by
 //      the phases e^{i pi (N - 1) m / N} that centre the pixel grid, once,
 //      instead of at each of the W^2 taps of every sample;
 //   4. sampling at the grid's cells (spectrum_sample_kernel): the spectrum
@@ -360,6 +360,15 @@ __global__ void spectrum_phase_kernel(int n, float2 *__restrict__ cph)
     cph[q] = make_float2((float)cs, (float)sn);
 }
 
+// Row whose tent support [py - h, py + h] misses the polygon's y range: all
+// zero.  Such rows are neither transformed nor stored; the column pass reads
+// them as zero.
+__device__ __forceinline__ bool spec_zero_row(const double *__restrict__ r, int row, int n)
+{
+    const double py = fft_pixel(row, n, r[SP_H], r[SP_YC]);
+    return py + r[SP_H] <= r[SP_YLO] || py - r[SP_H] >= r[SP_YHI];
+}
+
 // Raster + deapodisation + row FFTs of the spectrum beams of one group (all
 // of grid size 2^LOGN), as fft_rows_kernel with the tent raster; each row
 // stores its band of P columns ([N rows][P], bin mx at spec_band_col),
@@ -387,15 +396,11 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) spectrum_rows_kernel(
     const double xc = r[SP_XC], yc = r[SP_YC], h = r[SP_H];
     const int M = (int)r[SP_M], P = (int)r[SP_P];
     float2 *g = spec + off[beam] + (size_t)row0 * P;
-    // rows whose tent support [py - h, py + h] misses the polygon's y range
-    // are zero: their band is written as such, with no transform
-    const double ylo = fft_pixel(row0, N, h, yc) - h;
-    const double yhi = fft_pixel(row0 + FFT_ROWS - 1, N, h, yc) + h;
-    if (yhi <= r[SP_YLO] || ylo >= r[SP_YHI]) {
-        for (int q = threadIdx.x; q < FFT_ROWS * P; q += blockDim.x)
-            g[q] = make_float2(0.0f, 0.0f);
+    bool all_zero = true;
+    for (int rr = 0; rr < FFT_ROWS; ++rr)
+        all_zero = all_zero && spec_zero_row(r, row0 + rr, N);
+    if (all_zero)
         return;
-    }
     constexpr int LD = fft_ld(N);
     int *queue = reinterpret_cast<int *>(lds + FFT_ROWS * LD);
     __shared__ int s_count;
@@ -457,7 +462,8 @@ __global__ void __launch_bounds__(FFT_COL_THREADS) spectrum_cols_kernel(
     float2 *g = spec + off[beam] + col0;
     const int ncol = P - col0;  // columns of this unit inside the band
     auto in = [&](int seq, int i) {
-        return seq < ncol ? g[(size_t)i * P + seq] : make_float2(0.0f, 0.0f);
+        return (seq < ncol && !spec_zero_row(r, i, N)) ? g[(size_t)i * P + seq]
+                                                       : make_float2(0.0f, 0.0f);
     };
     auto out = [&](int seq, int i, float2 x) {
         const int m = i <= M ? i : (i >= N - M ? i - N : N);
