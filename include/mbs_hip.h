@@ -69,7 +69,10 @@ enum {
 
 /* Prepared beams of a batch of orientations, as flat host arrays.  Replaces
  * std::vector<PreparedOrientation> (src/handler/HandlerPO.h:49-55) plus the
- * packing of pack_prepared_gpu_beam (src/cuda/GpuDiffraction.cu:273-358). */
+ * packing of pack_prepared_gpu_beam (src/cuda/GpuDiffraction.cu:273-358).
+ * An orientation holds at most MBS_MAX_BEAMS_PER_ORIENT beams (the upload
+ * returns MBS_ERR_CAPACITY beyond it). */
+#define MBS_MAX_BEAMS_PER_ORIENT 8388607
 typedef struct mbs_beam_batch {
     int32_t n_orient;
     int32_t n_beams;

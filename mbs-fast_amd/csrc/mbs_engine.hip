@@ -489,9 +489,17 @@ static int upload_impl(mbs_ctx *ctx, const mbs_beam_batch *bt, const mbs_consts 
     const int no = bt->n_orient, nb = bt->n_beams;
     if (no > 0 && (bt->orient_offsets[0] != 0 || bt->orient_offsets[no] != nb))
         return fail(ctx, MBS_ERR_INVALID, "orient_offsets must run from 0 to n_beams");
-    for (int o = 0; o < no; ++o)
+    for (int o = 0; o < no; ++o) {
         if (bt->orient_offsets[o + 1] < bt->orient_offsets[o])
             return fail(ctx, MBS_ERR_INVALID, "orient_offsets decrease at %d", o);
+        // the MFMA kernel addresses a beam range's operands with 32-bit byte
+        // offsets (WB_STRIDE floats per beam)
+        if (bt->orient_offsets[o + 1] - bt->orient_offsets[o] > MBS_MAX_BEAMS_PER_ORIENT)
+            return fail(ctx, MBS_ERR_CAPACITY,
+                        "orientation %d holds %lld beams; at most %d per orientation", o,
+                        (long long)(bt->orient_offsets[o + 1] - bt->orient_offsets[o]),
+                        MBS_MAX_BEAMS_PER_ORIENT);
+    }
     if (nb > 0 && bt->vert_offsets[0] != 0)
         return fail(ctx, MBS_ERR_INVALID, "vert_offsets[0] must be 0");
     std::memset(ctx->nv_hist, 0, sizeof ctx->nv_hist);
