@@ -215,6 +215,81 @@ def bench_traced(eng, abi, fx, args):
             "units_per_step": units}
 
 
+def bench_config3(pkg, args):
+    """The batched non-convex aggregate at its stated batch (opt-in,
+    --config3): 16,384 orientations with beam counts log-uniform over
+    50..40,000 (a synthetic resample of the 8-column aggregate's traced beams,
+    tests/golden/config3_aggregate.npz; ~1e8 beams, ~130 GB of device
+    records) on configuration 2's 541 x 96 grid.  One eager beam-segment
+    launch, then the persistent work-queue launch captured as a graph and
+    replayed twice; every replay must equal the eager result bit for bit."""
+    abi, syn, eng = pkg.abi, pkg.synthetic, pkg.engine
+    fx = dict(np.load(os.path.join(ROOT, "tests", "golden", "config3_aggregate.npz")))
+    src, consts = abi.BeamBatch.from_fixture(fx), abi.consts_from_fixture(fx)
+    n_orient = 16384
+    rng = np.random.default_rng(7)
+    counts = np.exp(rng.uniform(np.log(50), np.log(40000), n_orient)).astype(np.int64)
+    t0 = time.perf_counter()
+    parts = []
+    for i, chunk in enumerate(np.array_split(counts, 16)):  # with a progress line each
+        parts.append(syn.resample(src, chunk, seed=7 + i))
+        print("config 3: built %d of 16 parts" % (i + 1), file=sys.stderr, flush=True)
+    batch = syn.concat(parts, np.full(n_orient, 1.0 / n_orient))
+    del parts
+    build_s = time.perf_counter() - t0
+    n_beams = int(batch.n_beams)
+    print("config 3: %d orientations, %d beams built in %.1f s" % (n_orient, n_beams, build_s),
+          file=sys.stderr, flush=True)
+    grid = syn.config2_grid()
+    units = float(n_beams) * grid.cells
+    hip = Hip()
+    ctx = eng.Context(0)
+    ctx.set_grid(grid)
+    t0 = time.perf_counter()
+    ctx.upload(batch, consts)
+    hip.sync()
+    upload_s = time.perf_counter() - t0
+    del batch
+    print("config 3: uploaded in %.1f s" % upload_s, file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    ctx.set_target_blocks(131072)
+    ctx.set_work_mode(abi.WORK_SEGMENTS)
+    t0 = time.perf_counter()
+    want = ctx.mueller(abi.PREC_MIXED)[0]
+    seg_wall = time.perf_counter() - t0
+    seg_kernel = ctx.timing()["kernel_ms"]
+    print("config 3: beam segments %.1f s (kernel %.1f ms)" % (seg_wall, seg_kernel),
+          file=sys.stderr, flush=True)
+    ctx.set_work_mode(abi.WORK_QUEUE)
+    ctx.set_graph(True)
+    walls, same = [], True
+    for rep in range(3):  # the first call captures the graph
+        hip.sync()
+        t0 = time.perf_counter()
+        ctx.diffract(abi.PREC_MIXED)
+        hip.sync()
+        walls.append(time.perf_counter() - t0)
+        got = np.zeros_like(want)
+        ctx.accumulate(got)  # the last result
+        same = same and bool(np.array_equal(got, want))
+        print("config 3: work-queue graph call %d, %.1f s, equal to the segments: %s"
+              % (rep, walls[-1], same), file=sys.stderr, flush=True)
+    captures, launches = ctx.graph_stats()
+    ctx.set_graph(False)
+    ctx.close()
+    replay = float(np.mean(walls[1:]))
+    return {"config3": {
+        "workload": "8-column hexagonal aggregate (seed 7, D~U(5,30) um, L/D~U(1,4)), "
+                    "m=1.3116+1e-4i, 6 reflections; 16384 orientations, beams per "
+                    "orientation log-uniform 50..40000 (synthetic resample of the traced "
+                    "beams); 541 theta x 96 phi",
+        "orientations": n_orient, "beams": n_beams, "units": units,
+        "value": units / replay, "unit": "beam.dir evals/s",
+        "graph_replay_s": replay, "segments_wall_s": seg_wall, "segments_kernel_ms": seg_kernel,
+        "graph_captures": captures, "graph_launches": launches,
+        "replays_equal_segments": same, "batch_build_s": build_s, "upload_s": upload_s}}
+
+
 def cpu_baseline(pkg, batch, grid, consts, budget_s=12.0):
     """Times the FP64 CPU oracle (OpenMP, all host threads) on a bounded
     sample: the first beams of the batch's first orientation on the full grid,
@@ -563,6 +638,8 @@ def main():
                     help="seconds of CPU time for the baseline sample")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--config3", action="store_true",
+                    help="only run the 16,384-orientation aggregate batch (~1e8 beams)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (orientations "
@@ -571,6 +648,10 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.config3:
+        import mbs_loader
+        print(json.dumps(bench_config3(mbs_loader.load(), args)))
         return
     if args.steps < 1 or args.warmup < 0:
         sys.exit("--steps must be >= 1 and --warmup >= 0")

@@ -59,6 +59,23 @@ def resample(source, beams_per_orient, scale=1.0, seed=0, weights=None):
                          source.slope_xy[verts], source.edge_flags[verts])
 
 
+def concat(batches, weights=None):
+    """One batch holding the orientations of `batches` in order (weights
+    their own unless given)."""
+    oo, vo = [np.zeros(1, np.int64)], [np.zeros(1, np.int64)]
+    nb = nvert = 0
+    for b in batches:
+        oo.append(b.orient_offsets[1:] + nb)
+        vo.append(b.vert_offsets[1:] + nvert)
+        nb += b.n_beams
+        nvert += int(b.vert_offsets[-1])
+    cat = lambda k: np.concatenate([getattr(b, k) for b in batches])  # noqa: E731
+    w = cat("weights") if weights is None else weights
+    return abi.BeamBatch(np.concatenate(oo), w, cat("beam_scalars"), cat("beam_nv"),
+                         cat("beam_external"), np.concatenate(vo), cat("vx"), cat("vy"),
+                         cat("slope_yx"), cat("slope_xy"), cat("edge_flags"))
+
+
 def config2_grid():
     """theta 0..10 deg at 0.05 deg and 10.5..180 deg at 0.5 deg (541 rows) x
     96 phi -- the non-uniform grid of the orientation-average workload."""

@@ -264,7 +264,9 @@ def fixed_case(name, L, D, n_re, n_im, wave, refl, beta, gamma, th0, th1, nphi, 
 
 
 def sobol_case(name, L, D, n_re, n_im, wave, refl, n_orient, seed, nphi, nth,
-               th0=0, th1=180, tgrid=None, chunk=None):
+               th0=0, th1=180, tgrid=None, chunk=None, particle_file=None):
+    """particle_file: a callable writing a particle file into the run
+    directory (used instead of the built-in column L, D)."""
     wd = fresh_dir(name)
     if tgrid is not None:
         with open(os.path.join(wd, "theta.grid"), "w") as f:
@@ -272,7 +274,12 @@ def sobol_case(name, L, D, n_re, n_im, wave, refl, n_orient, seed, nphi, nth,
         grid = ["--theta-grid-file", "theta.grid", "--phi-points", str(nphi)]
     else:
         grid = ["--scattering-grid", str(th0), str(th1), str(nphi), str(nth)]
-    args = ["--method", "po", "--particle", "1", str(L), str(D),
+    if particle_file is not None:
+        particle_file(os.path.join(wd, "particle.txt"))
+        source = ["--pf", "particle.txt"]
+    else:
+        source = ["--particle", "1", str(L), str(D)]
+    args = ["--method", "po"] + source + [
             "--refractive-index", str(n_re), str(n_im), "--wavelength-um", str(wave),
             "--max-reflections", str(refl), "--sobol-seed", str(n_orient), str(seed)] + grid + [
             "--cutoff-profile", "off", "--threads", "8", "--no-shadow-output", "--close"]
@@ -331,7 +338,7 @@ def _main(only):
     if want("sobol_points"):
         save("sobol_points", sobol_points())
     if not any(want(n) for n in ("config1_fixed", "fixed_absorbing", "sobol_average",
-                                 "config5_near_forward")):
+                                 "config5_near_forward", "config3_aggregate")):
         return
     # Configuration 1: hexagonal column L=20 D=10 um, m=1.3116, lambda=0.532,
     # 8 reflections, beta=30 gamma=10, theta 0..180 in 720 steps x 48 phi.
@@ -355,6 +362,15 @@ def _main(only):
     if want("config5_near_forward"):
         save("config5_near_forward", sobol_case("config5_near_forward", 400, 200, 1.3116, 1e-3,
                                                 0.532, 12, 4, 1, 12, 100, th0=0, th1=5))
+    # Batched non-convex aggregate: the 8-column aggregate (seed 7, D ~ U(5, 30)
+    # um, L/D ~ U(1, 4)), m=1.3116+1e-4i, 6 reflections, 2 Sobol orientations
+    # (seed 1); expected grids on theta 0..180 in 90 steps x 12 phi.  The beams
+    # seed synthetic batches of the configuration's size.
+    if want("config3_aggregate"):
+        from particles import write_column_aggregate
+        save("config3_aggregate", sobol_case("config3_aggregate", 0, 0, 1.3116, 1e-4, 0.532, 6,
+                                             2, 1, 12, 90,
+                                             particle_file=write_column_aggregate))
     tg = [round(0.05 * i, 2) for i in range(201)] + [10.5 + 0.5 * i for i in range(340)]
     if want("sobol_average"):
         save("sobol_average", sobol_case("sobol_average", 20, 10, 1.3116, 1e-4, 0.532, 6,
