@@ -436,6 +436,52 @@ def bench_near_forward(eng, abi, args):
     }
 
 
+def bench_size_scan(eng, abi, syn, args):
+    """The configuration-5 size scan: 16 sizes, D = 10..200 um log-spaced,
+    1,024 orientations each (a synthetic resample of the traced D=200 um beams:
+    per-orientation counts drawn from the traced orientations', beams rotated
+    at random), theta 0-5 deg at 0.005 deg x 48 phi.  One upload; every size
+    re-derived on the device (mbs_rescale_beams: vertices and centres times
+    s, areas times s^2, phases at the scaled path).  Per size: mixed and FP64
+    evals/s from the kernel times (HIP events).  The FP64-oracle error per
+    size is tests/test_config5_gpu.py's."""
+    fx = dict(np.load(os.path.join(ROOT, "tests", "golden", "config5_near_forward.npz")))
+    src = abi.BeamBatch.from_fixture(fx)
+    consts = abi.consts_from_fixture(fx)
+    rng = np.random.default_rng(5)
+    counts = rng.choice(np.diff(src.orient_offsets), 1024)
+    batch = syn.resample(src, counts, seed=5)
+    theta = np.radians(np.round(np.arange(0, 1001) * 0.005, 10))
+    grid = abi.Grid.build(theta, 48)
+    nb = batch.n_beams
+    units = float(nb) * grid.cells
+    ctx = eng.Context(0)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    # scaling data that keeps J_phased as uploaded (raw J = J_phased, path 0,
+    # sign +1, no absorption paths)
+    ctx.upload_scaling(batch.beam_scalars[:, abi.B_JP:abi.B_JP + 8], np.zeros(nb), np.ones(nb),
+                       np.zeros(nb + 1, np.int64), np.zeros(0), 0.0, consts.wave_index)
+    ctx.set_timing(True)
+    rows = []
+    for d in np.geomspace(10.0, 200.0, 16):
+        ctx.rescale(d / 200.0)
+        per = {}
+        for name, flags, reps in (("mixed", abi.PREC_MIXED, 2), ("fp64", abi.PREC_FP64, 1)):
+            ctx.diffract(flags)
+            ks = []
+            for _ in range(reps):
+                ctx.diffract(flags)
+                ks.append(ctx.timing()["kernel_ms"])
+            per[name] = units / (float(np.mean(ks)) * 1e-3)
+        rows.append({"D_um": round(float(d), 2), "mixed": per["mixed"], "fp64": per["fp64"]})
+    ctx.close()
+    return {"workload": "hexagonal column D=10..200 um (16 sizes, log-spaced), m=1.3116+1e-3i, "
+                        "12 reflections; 1024 orientations, %d beams; theta 0-5 deg at 0.005 deg "
+                        "x 48 phi; device rescale per size" % nb,
+            "unit": "beam.dir evals/s", "units_per_size": units, "sizes": rows}
+
+
 def run(args):
     import mbs_loader
     pkg = mbs_loader.load()
@@ -549,6 +595,7 @@ def run(args):
     fft = bench_aperture_fft(ctx, batch, args)
     ctx.close()
     near = bench_near_forward(eng, abi, args)
+    scan = bench_size_scan(eng, abi, pkg.synthetic, args)
     hist = np.bincount(batch.beam_nv, minlength=32)
     out = {
         "metric": "beam.dir evals/s",
@@ -603,6 +650,7 @@ def run(args):
                                   "workgroups")},
         "aperture_fft": fft,
         "near_forward": near,
+        "size_scan": scan,
     }
     if not args.no_cpu:
         port = cpu_baseline(pkg, batch, grid, consts, args.cpu_budget / 2)
