@@ -25,7 +25,7 @@
 //                     in LDS, writes them back (n^2 x 8 B)
 // The 1-D FFT is the Stockham autosort formulation (Govindaraju et al.,
 // "High performance discrete Fourier transforms on graphics processors",
-// SC08): radix-8 stages and one radix-4 or radix-2 stage, each thread taking
+// SC08): radix-16 stages and one radix-8, -4 or -2 stage, each thread taking
 // whole butterflies into registers, with a barrier between the read and the
 // write phase so the stages run in place in one LDS buffer.  The first stage
 // reads the raster (row pass) or HBM (column pass) directly and the last
@@ -150,7 +150,7 @@ __device__ __forceinline__ int fft_logical_block()
 // (breaks the power-of-two strides of the early Stockham stages).
 __device__ __forceinline__ int fft_pad(int i) { return i + (i >> 5); }
 
-// In-place forward DFT of R points (exp(-2 pi i j k / R)), R = 2, 4, 8.
+// In-place forward DFT of R points (exp(-2 pi i j k / R)), R = 2, 4, 8, 16.
 template <int R>
 __device__ __forceinline__ void fft_dft(float2 *v)
 {
@@ -167,7 +167,7 @@ __device__ __forceinline__ void fft_dft(float2 *v)
         v[2] = make_float2(a0.x - a2.x, a0.y - a2.y);
         v[1] = make_float2(a1.x + a3.y, a1.y - a3.x);  // a1 - i a3
         v[3] = make_float2(a1.x - a3.y, a1.y + a3.x);  // a1 + i a3
-    } else {
+    } else if constexpr (R == 8) {
         float2 e[4] = {v[0], v[2], v[4], v[6]};
         float2 o[4] = {v[1], v[3], v[5], v[7]};
         fft_dft<4>(e);
@@ -184,6 +184,32 @@ __device__ __forceinline__ void fft_dft(float2 *v)
         v[6] = make_float2(e[2].x - o2.x, e[2].y - o2.y);
         v[3] = make_float2(e[3].x + o3.x, e[3].y + o3.y);
         v[7] = make_float2(e[3].x - o3.x, e[3].y - o3.y);
+    } else {
+        // radix 2 over two 8-point DFTs: X[k] = E[k] + w^k O[k],
+        // X[k+8] = E[k] - w^k O[k], w = exp(-2 pi i / 16)
+        float2 e[8], o[8];
+        for (int q = 0; q < 8; ++q) {
+            e[q] = v[2 * q];
+            o[q] = v[2 * q + 1];
+        }
+        fft_dft<8>(e);
+        fft_dft<8>(o);
+        const float c1 = 0.92387953251128676f;  // cos(pi/8)
+        const float s1 = 0.38268343236508977f;  // sin(pi/8)
+        const float h = 0.70710678118654752f;
+        float2 w[8];
+        w[0] = o[0];
+        w[1] = make_float2(o[1].x * c1 + o[1].y * s1, o[1].y * c1 - o[1].x * s1);
+        w[2] = make_float2(h * (o[2].x + o[2].y), h * (o[2].y - o[2].x));
+        w[3] = make_float2(o[3].x * s1 + o[3].y * c1, o[3].y * s1 - o[3].x * c1);
+        w[4] = make_float2(o[4].y, -o[4].x);
+        w[5] = make_float2(o[5].y * c1 - o[5].x * s1, -o[5].y * s1 - o[5].x * c1);
+        w[6] = make_float2(h * (o[6].y - o[6].x), -h * (o[6].x + o[6].y));
+        w[7] = make_float2(o[7].y * s1 - o[7].x * c1, -o[7].y * c1 - o[7].x * s1);
+        for (int q = 0; q < 8; ++q) {
+            v[q] = make_float2(e[q].x + w[q].x, e[q].y + w[q].y);
+            v[q + 8] = make_float2(e[q].x - w[q].x, e[q].y - w[q].y);
+        }
     }
 }
 
@@ -233,26 +259,32 @@ __device__ __forceinline__ void fft_stage(int n, int lg_ns, int cnt,
     __syncthreads();
 }
 
-// Stages S..end of a length 2^LOGN transform: radix 8 while at least 3 bits
-// remain, then one radix-4 or radix-2 stage.  Stage 0 reads through `first`,
-// the last stage writes through `last`, stages in between stay in LDS.
+// Stages S..end of a length 2^LOGN transform: radix 2^MBS_FFT_LOG_RADIX
+// while that many bits remain, then one stage of the rest.  Stage 0 reads
+// through `first`, the last stage writes through `last`, stages in between
+// stay in LDS.  Radix 16 (three stages at n = 2048: 16, 16, 8) makes one LDS
+// round trip and two barriers fewer than radix 8 (8, 8, 8, 4).
+#ifndef MBS_FFT_LOG_RADIX
+#define MBS_FFT_LOG_RADIX 4
+#endif
 template <int LOGN, int CNT, int T, bool COLFAST, int S, typename First, typename Last>
 __device__ __forceinline__ void fft_stages(float2 *buf, int ld, const float2 *__restrict__ tw,
                                            First first, Last last)
 {
     constexpr int N = 1 << LOGN;
-    constexpr int A = LOGN / 3, REM = LOGN % 3, NST = A + (REM ? 1 : 0);
+    constexpr int LR = MBS_FFT_LOG_RADIX;
+    constexpr int A = LOGN / LR, REM = LOGN % LR, NST = A + (REM ? 1 : 0);
     if constexpr (S < NST) {
-        constexpr int R = S < A ? 8 : (1 << REM);
+        constexpr int R = S < A ? (1 << LR) : (1 << REM);
         constexpr int PB = (CNT * (N / R) + T - 1) / T;
         auto lds_load = [&](int seq, int i) { return buf[seq * ld + fft_pad(i)]; };
         auto lds_store = [&](int seq, int i, float2 x) { buf[seq * ld + fft_pad(i)] = x; };
         if constexpr (S == 0)
-            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, first, lds_store);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, first, lds_store);
         else if constexpr (S == NST - 1)
-            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, lds_load, last);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, lds_load, last);
         else
-            fft_stage<R, PB, COLFAST>(N, 3 * S, CNT, tw, lds_load, lds_store);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, lds_load, lds_store);
         fft_stages<LOGN, CNT, T, COLFAST, S + 1>(buf, ld, tw, first, last);
     }
 }
