@@ -13,9 +13,9 @@
 //      aliased copies are suppressed by the same factor at their frequency
 //      (|f h - m| -> (f h)^2 / m^2), where a 0/1 raster leaves them whole.
 //      Pixels whose tent support lies inside every edge line are 1, outside
-//      one edge line 0; the others are integrated exactly: the polygon
-//      clipped to each quadrant of the support, and the bilinear tent
-//      integrated over the clipped polygon from its moments;
+//      one edge line 0; the others (boundary pixels) are integrated exactly
+//      by an edge form of Green's theorem (spec_tent_exact), one per lane
+//      from a per-row queue;
 //   2. deapodisation: each raster value divided by the transform of the
 //      interpolation kernel at its position (below);
 //   3. the 2-D FFT (the Stockham stages of the aperture FFT,
@@ -26,8 +26,9 @@
 //      row's transform and the column pass transforms only those columns
 //      and stores P rows (P / N of the column work and traffic, (P / N)^2 of
 //      the footprint).  Rows whose tent support misses the polygon are zero:
-//      neither transformed nor stored, and read as zero by the column pass.  Each stored bin (mx, my) is multiplied by
-//      the phases e^{i pi (N - 1) m / N} that centre the pixel grid, once,
+//      neither transformed nor stored, and read as zero by the column pass.
+//      Each stored bin (mx, my) is multiplied by the phases
+//      e^{i pi (N - 1) m / N} that centre the pixel grid, once,
 //      instead of at each of the W^2 taps of every sample;
 //   4. sampling at the grid's cells (spectrum_sample_kernel): the spectrum
 //      interpolated with the "exponential of semicircle" kernel
