@@ -495,16 +495,17 @@ int mbs_set_work_mode(mbs_ctx *ctx, int mode);
  * (target workgroups 4096..131072 x whole-orientation / beam-segment /
  * device-queue work items x, for mixed precision, MFMA / vector-ALU coherent
  * sum, and for the MFMA kernel with beam segments and no MBS_NOSHADOW its
- * register allocation held to 4, 5 or 6 waves per SIMD), keeps the fastest
- * for later mbs_diffract calls of the same precision, and reports what it
- * timed.  mbs_set_target_blocks / mbs_set_work_mode discard it.
+ * register allocation held to 4, 5 or 6 waves per SIMD; after a warm-up
+ * call, the least of three timed calls), keeps the fastest for later
+ * mbs_diffract calls of the same precision, and reports what it timed.
+ * mbs_set_target_blocks / mbs_set_work_mode discard it.
  * flags: MBS_PREC_MIXED and/or MBS_NOSHADOW only.  Leaves a result like
  * mbs_diffract(flags). */
 typedef struct mbs_tune_entry {
     int32_t target_blocks;
     int32_t work_mode;  /* 1 whole orientations, 2 beam segments, 3 device queue */
     int32_t mfma;       /* 1 MFMA contraction, 0 vector ALU (mixed only) */
-    float ms;           /* main kernel + reduction, device time */
+    float ms;           /* main kernel + reduction, device time (least of 3) */
     int32_t waves;      /* MFMA kernel: waves per SIMD its registers are held to
                            (4, 5 or 6; 4 and 6 with beam segments only); 0 for
                            the other kernels */
