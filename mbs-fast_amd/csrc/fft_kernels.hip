@@ -20,9 +20,10 @@
 //                     most: rows of pixel centres outside the polygon's y
 //                     range are zero and skipped -- with padding p, all but
 //                     about 1/p of the rows)
-//   fft_cols_kernel   reads COLS adjacent columns (32-byte row segments) of
-//                     the rows written, the others as zero, does their FFTs
-//                     in LDS, writes them back (n^2 x 8 B)
+//   fft_cols_kernel   reads WCOLS adjacent columns (64-byte row segments) of
+//                     the rows written, as n / L transforms of length L over
+//                     the window of L rows that holds them, and writes the
+//                     whole grid (n^2 x 8 B)
 // The 1-D FFT is the Stockham autosort formulation (Govindaraju et al.,
 // "High performance discrete Fourier transforms on graphics processors",
 // SC08): radix-16 stages and one radix-8, -4 or -2 stage, each thread taking
@@ -42,10 +43,10 @@ namespace mbs {
 // Workgroups: 128 threads per row or column.  The row kernel holds FFT_ROWS
 // rows: one row (17 KB of LDS at n = 2048, nine workgroups per CU) ran the
 // row pass 16 % faster than four (68 KB, two per CU), whose barriers stall
-// more of the CU at once.  The column kernel holds FFT_COLS columns: 4 (two
-// 68 KB workgroups of 512 threads per CU) ran the column pass of 64 padded
-// 2048^2 grids in 1.13 ms, 8 (one 135 KB workgroup) in 1.31 ms and 2 in
-// 1.54 ms.
+// more of the CU at once.  FFT_COLS columns of full length per workgroup
+// (the spectrum source's column pass): 4 (two 68 KB workgroups of 512
+// threads per CU) ran a full-length column pass of 64 padded 2048^2 grids
+// in 1.13 ms, 8 (one 135 KB workgroup) in 1.31 ms and 2 in 1.54 ms.
 #ifndef MBS_FFT_COLS
 #define MBS_FFT_COLS 4
 #endif
@@ -218,16 +219,17 @@ __device__ __forceinline__ void fft_dft(float2 *v)
 // position j (seq fastest when COLFAST, else j fastest); it reads elements
 // j + r n/R (twiddled by exp(-2 pi i k r / (ns R)), k = j mod ns) and writes
 // (j / ns) ns R + k + r ns.  load(seq, i) / store(seq, i, v) give the
-// element access; all reads of the stage precede all its writes.
+// element access; all reads of the stage precede all its writes.  tw holds
+// exp(-2 pi i m / tw_n), tw_n a multiple of n.
 template <int R, int PB, bool COLFAST, typename Load, typename Store>
 __device__ __forceinline__ void fft_stage(int n, int lg_ns, int cnt,
-                                          const float2 *__restrict__ tw, Load load,
+                                          const float2 *__restrict__ tw, int tw_n, Load load,
                                           Store store)
 {
     const int nb = n / R;
     const int total = cnt * nb;
     const int ns = 1 << lg_ns;
-    const int tstep = n / (ns * R);
+    const int tstep = tw_n / (ns * R);
     float2 v[PB][R];
     for (int b = 0; b < PB; ++b) {
         const int idx = threadIdx.x + b * blockDim.x;
@@ -259,33 +261,55 @@ __device__ __forceinline__ void fft_stage(int n, int lg_ns, int cnt,
     __syncthreads();
 }
 
-// Stages S..end of a length 2^LOGN transform: radix 2^MBS_FFT_LOG_RADIX
-// while that many bits remain, then one stage of the rest.  Stage 0 reads
-// through `first`, the last stage writes through `last`, stages in between
-// stay in LDS.  Radix 16 (three stages at n = 2048: 16, 16, 8) makes one LDS
-// round trip and two barriers fewer than radix 8 (8, 8, 8, 4).
+// Stages S..end of a length 2^LOGN transform: radix 2^LR while that many
+// bits remain, then one stage of the rest.  Stage 0 reads through `first`,
+// the last stage writes through `last`, stages in between stay in LDS.
+// Radix 16 (three stages at n = 2048: 16, 16, 8) makes one LDS round trip and
+// two barriers fewer than radix 8 (8, 8, 8, 4).  tw: exp(-2 pi i m / 2^LOGTW).
 #ifndef MBS_FFT_LOG_RADIX
 #define MBS_FFT_LOG_RADIX 4
 #endif
-template <int LOGN, int CNT, int T, bool COLFAST, int S, typename First, typename Last>
+// LDS index of element i of sequence seq, CNT sequences interleaved: the
+// sequences' elements i sit together in one block of CNT x 8 bytes, and the
+// low bits of the block index are XORed with the bits of i above the first
+// stage's radix 2^LR, so that every read and write pattern of a Stockham
+// stage (column-fastest lanes, 2^LR-radix butterflies) hits 32 lanes' worth
+// of distinct banks: consecutive elements, and the first stage's
+// outputs j 2^LR + r of consecutive j.
+template <int CNT, int LR>
+__device__ __forceinline__ int fft_swz(int seq, int i)
+{
+    constexpr int M = (32 / CNT > 1 ? 32 / CNT : 1) - 1;
+    static_assert(M < (1 << LR), "the XOR source bits lie above the bits they change");
+    return (i ^ ((i >> LR) & M)) * CNT + seq;
+}
+
+template <int LOGN, int CNT, int T, bool COLFAST, int S, int LR = MBS_FFT_LOG_RADIX,
+          int LOGTW = LOGN, bool SWZ = false, typename First, typename Last>
 __device__ __forceinline__ void fft_stages(float2 *buf, int ld, const float2 *__restrict__ tw,
                                            First first, Last last)
 {
     constexpr int N = 1 << LOGN;
-    constexpr int LR = MBS_FFT_LOG_RADIX;
     constexpr int A = LOGN / LR, REM = LOGN % LR, NST = A + (REM ? 1 : 0);
     if constexpr (S < NST) {
         constexpr int R = S < A ? (1 << LR) : (1 << REM);
         constexpr int PB = (CNT * (N / R) + T - 1) / T;
-        auto lds_load = [&](int seq, int i) { return buf[seq * ld + fft_pad(i)]; };
-        auto lds_store = [&](int seq, int i, float2 x) { buf[seq * ld + fft_pad(i)] = x; };
+        auto at = [&](int seq, int i) {
+            if constexpr (SWZ)
+                return fft_swz<CNT, LR>(seq, i);
+            else
+                return seq * ld + fft_pad(i);
+        };
+        auto lds_load = [&](int seq, int i) { return buf[at(seq, i)]; };
+        auto lds_store = [&](int seq, int i, float2 x) { buf[at(seq, i)] = x; };
+        constexpr int TWN = 1 << LOGTW;
         if constexpr (S == 0)
-            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, first, lds_store);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, TWN, first, lds_store);
         else if constexpr (S == NST - 1)
-            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, lds_load, last);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, TWN, lds_load, last);
         else
-            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, lds_load, lds_store);
-        fft_stages<LOGN, CNT, T, COLFAST, S + 1>(buf, ld, tw, first, last);
+            fft_stage<R, PB, COLFAST>(N, LR * S, CNT, tw, TWN, lds_load, lds_store);
+        fft_stages<LOGN, CNT, T, COLFAST, S + 1, LR, LOGTW, SWZ>(buf, ld, tw, first, last);
     }
 }
 
@@ -347,30 +371,128 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
     fft_stages<LOGN, FFT_ROWS, FFT_ROW_THREADS, false, 0>(lds, fft_ld(N), tw, raster, out);
 }
 
-// Column FFTs.  Work unit = (beam, FFT_COLS adjacent columns); LDS FFT_COLS x
-// fft_ld(n) float2.  Lanes take columns fastest, so a wave reads and writes
-// FFT_COLS x 8-byte row segments; rows outside the polygon's y range read as
-// zero without a load.
-template <int LOGN>
-__global__ void __launch_bounds__(FFT_COL_THREADS) fft_cols_kernel(
-    int n_beams, const double *__restrict__ rec, const float2 *__restrict__ tw,
+// Column FFTs over a row window.  Rows of pixel centres outside the
+// polygon's bounding box are zero, and with padding p those lie outside a
+// window of L = 2^LOGL <= N rows, [j0, j0 + L), L the power of two at or
+// above n / p (fft_window).  With D = N / L, w = exp(-2 pi i / N) and bins
+// m = D q + s (s < D), w^(row m) = w^(row s) w_L^((row mod L) q), w_L = w^D,
+// so
+//   X[D q + s] = sum_p z_s[p] w_L^(p q),  z_s[row mod L] = x[row] w^(row s):
+// each column's transform is D transforms of length L, one per residue s, of
+// the window placed circularly (row mod L) and twiddled by w^(row s).  The
+// LDS holds L elements per column instead of N; the window stays in
+// registers across the residues, twiddled from one residue to the next by
+// one complex product with w^row, and each residue's bins are stored while
+// the next residue is transformed.  Nothing between the stores and the end
+// reads global memory (the L-point twiddles w_L^i sit in LDS), so no wait on
+// the load counter, which also counts stores, holds the stores up; the
+// window's loads are issued unconditionally (zero rows are selected away
+// afterwards) so that they are all in flight at once.
+// Work unit = (beam, FFT_WCOLS adjacent columns); lanes take columns
+// fastest, so a wave reads and writes FFT_WCOLS x 8-byte row segments.  LDS:
+// FFT_WCOLS x L float2 in the fft_swz layout, then the L twiddles.
+// Column pass of 64 grids of 2048^2 at padding 4 (L = 512, four residues),
+// by columns x threads per workgroup: 8 x 512 0.67 ms, 16 x 1024 0.70 ms,
+// 8 x 256 and 16 x 512 0.78 ms, 32 x 1024 1.00 ms, 4 x 256 1.04 ms; radix-16
+// first stages (8 x 512) 1.02 ms.  The full-length column pass it replaced
+// (4 columns of 2048 rows per workgroup) took 1.05 ms.
+#ifndef MBS_FFT_WCOLS
+#define MBS_FFT_WCOLS 8
+#endif
+#ifndef MBS_FFT_WCOL_THREADS
+#define MBS_FFT_WCOL_THREADS 512
+#endif
+#ifndef MBS_FFT_WCOL_LOG_RADIX
+#define MBS_FFT_WCOL_LOG_RADIX 3
+#endif
+enum { FFT_WCOLS = MBS_FFT_WCOLS, FFT_WCOL_THREADS = MBS_FFT_WCOL_THREADS };
+
+// Shortest window: 64 rows (LOGL >= 6 keeps at least two Stockham stages).
+enum { FFT_MIN_LOGL = 6 };
+
+// The row window of an n-point grid at padding pad: pixel centres at
+// j - (n - 1)/2 pixels from the box centre, the box n / (2 pad) pixels
+// either side of it, so rows lo..hi below can carry the polygon (with 1e-3
+// pixel to spare for rounding); L the power of two at or above hi - lo + 1
+// (64 at least, n at most) and j0 the window start, clamped into the grid.
+__host__ inline void fft_window(int n, double pad, int *j0, int *logl)
+{
+    const double c = 0.5 * (n - 1), h = n / (2.0 * pad);
+    const int lo = (int)ceil(c - h - 1e-3), hi = (int)floor(c + h + 1e-3);
+    int lg = FFT_MIN_LOGL;
+    while ((1 << lg) < n && (1 << lg) < hi - lo + 1)
+        ++lg;
+    const int l = 1 << lg;
+    *logl = lg;
+    *j0 = lo < 0 ? 0 : (lo > n - l ? n - l : lo);
+}
+
+template <int LOGN, int LOGL>
+__global__ void __launch_bounds__(FFT_WCOL_THREADS) fft_cols_kernel(
+    int n_beams, int j0, const double *__restrict__ rec, const float2 *__restrict__ tw,
     float2 *__restrict__ grid)
 {
-    constexpr int N = 1 << LOGN;
+    constexpr int N = 1 << LOGN, L = 1 << LOGL, D = N / L;
+    constexpr int LR = LOGL < MBS_FFT_WCOL_LOG_RADIX ? LOGL : MBS_FFT_WCOL_LOG_RADIX;
+    constexpr int R = 1 << LR, NB = L / R;  // first-stage radix, butterflies per column
+    constexpr int C = FFT_WCOLS, T = FFT_WCOL_THREADS;
+    constexpr int PB = (C * NB + T - 1) / T;
     extern __shared__ float2 lds[];
-    const int units_per_beam = N / FFT_COLS;
+    float2 *twl = lds + C * L;  // w_L^i = w^(i D), i < L
+    const int units_per_beam = N / C;
     const int unit = fft_logical_block();
     if (unit >= n_beams * units_per_beam)
         return;
     const int beam = unit / units_per_beam;
-    const int col0 = (unit - beam * units_per_beam) * FFT_COLS;
+    const int col0 = (unit - beam * units_per_beam) * C;
     float2 *g = grid + (size_t)beam * N * N + col0;
     const double *r = rec + (size_t)beam * FB_STRIDE;
-    auto in = [&](int seq, int i) {
-        return fft_zero_row(r, i, N) ? make_float2(0.0f, 0.0f) : g[(size_t)i * N + seq];
-    };
-    auto out = [&](int seq, int i, float2 x) { g[(size_t)i * N + seq] = x; };
-    fft_stages<LOGN, FFT_COLS, FFT_COL_THREADS, true, 0>(lds, fft_ld(N), tw, in, out);
+    for (int i = threadIdx.x; i < L; i += T)
+        twl[i] = tw[i * D];
+    // the window in first-stage order: element p = j + e NB of butterfly b's
+    // column is row j0 + ((p - j0) mod L); y = x w^(row s), wr = w^row
+    float2 y[PB][R], wr[PB][R];
+    for (int b = 0; b < PB; ++b) {
+        const int idx = (threadIdx.x + b * T) % (C * NB);  // lanes past the end repeat
+        const int seq = idx % C, j = idx / C;
+        for (int e = 0; e < R; ++e) {
+            const int row = j0 + ((j + e * NB - j0) & (L - 1));
+            y[b][e] = g[(size_t)row * N + seq];
+            wr[b][e] = tw[row];
+        }
+        for (int e = 0; e < R; ++e) {
+            const int row = j0 + ((j + e * NB - j0) & (L - 1));
+            if (fft_zero_row(r, row, N))
+                y[b][e] = make_float2(0.0f, 0.0f);
+        }
+    }
+    // every load is back here, before the first store: the compiler's own
+    // wait for a loaded register used after a store would also wait for the
+    // store (s_waitcnt vmcnt(0); expcnt and lgkmcnt left alone)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    auto none = [&](int, int) { return make_float2(0.0f, 0.0f); };
+    for (int s = 0; s < D; ++s) {
+        // first stage (ns = 1): R-point DFT of the twiddled window, stored at
+        // j R + e
+        for (int b = 0; b < PB; ++b) {
+            const int idx = threadIdx.x + b * T;
+            if (idx >= C * NB)
+                break;
+            const int seq = idx % C, j = idx / C;
+            float2 v[R];
+            for (int e = 0; e < R; ++e) {
+                v[e] = y[b][e];
+                if (s + 1 < D)
+                    y[b][e] = cmulf(y[b][e], wr[b][e]);
+            }
+            fft_dft<R>(v);
+            for (int e = 0; e < R; ++e)
+                lds[fft_swz<C, LR>(seq, j * R + e)] = v[e];
+        }
+        __syncthreads();
+        auto out = [&](int seq, int q, float2 v) { g[(size_t)(D * q + s) * N + seq] = v; };
+        fft_stages<LOGL, C, T, true, 1, LR, LOGL, true>(lds, 0, twl, none, out);
+    }
 }
 
 __device__ __forceinline__ double lanczos(double x, int radius)

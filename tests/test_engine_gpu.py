@@ -592,6 +592,39 @@ def test_aperture_fft_on_device(pkg, oracle, ctx, n):
         ctx.aperture_fft([batch.n_beams], 256)
 
 
+@pytest.mark.parametrize("pad", [1.0, 1.5, 3.0, 6.0])
+def test_aperture_fft_row_window(pkg, oracle, ctx, pad):
+    """The column pass transforms only the window of rows that can carry the
+    polygon: n / pad rows rounded up to a power of two, as n / L transforms of
+    length L.  Padding 1, 1.5, 3 and 6 at n = 256 give L = 256, 256, 128 and
+    64 (one, one, two and four residues); every spectrum against the oracle's
+    FP64 FFT of the identical raster, and the sampled values to 1e-5 of the
+    aperture area."""
+    fx = load_fixture("sobol_average")
+    grid, batch, consts = _setup(pkg, fx)
+    ctx.set_grid(grid)
+    ctx.upload(batch, consts)
+    k = consts.wave_index
+    n = 256
+    rng = np.random.default_rng(int(pad * 10))
+    beams = rng.choice(batch.n_beams, 6, replace=False)
+    ab = rng.uniform(-0.02, 0.02, (16, 2))
+    got = ctx.aperture_fft(beams, n, pad=pad, ab=ab)
+    vo = batch.vert_offsets
+    worst_s = worst_v = 0.0
+    for i, b in enumerate(beams):
+        vx, vy = batch.vx[vo[b]:vo[b + 1]], batch.vy[vo[b]:vo[b + 1]]
+        want, X = oracle.aperture_fft(vx, vy, n, pad, ab, k)
+        S = ctx.aperture_fft_spectrum(i, n)
+        _, (_, _, dx, dy) = oracle.aperture_raster(vx, vy, n, pad)
+        worst_s = max(worst_s, np.linalg.norm(S - X) / np.linalg.norm(X))
+        worst_v = max(worst_v, np.max(np.abs(got[i] - want)) / (abs(X[0, 0]) * dx * dy))
+    print("pad %.1f: worst spectrum rel L2 %.2e, sampled max |d|/area %.2e"
+          % (pad, worst_s, worst_v))
+    assert worst_s < 1e-5
+    assert worst_v < 1e-5
+
+
 def test_graph_replay_and_autotune(pkg, oracle, ctx):
     """Graph replays equal the eager result on every replay, with eager work
     and synchronisations between replays; the autotuner's choice gives the
