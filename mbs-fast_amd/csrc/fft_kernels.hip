@@ -15,11 +15,11 @@
 //
 // Layout: one grid per beam, row-major [n rows (y)][n columns (x)] of float2,
 // the whole batch contiguous.  Two passes over HBM:
-//   fft_rows_kernel   rasterises ROWS rows into LDS (no global read), does
-//                     their n-point FFTs in LDS, writes them (n^2 x 8 B at
-//                     most: rows of pixel centres outside the polygon's y
-//                     range are zero and skipped -- with padding p, all but
-//                     about 1/p of the rows)
+//   fft_rows_kernel   rasterises each row's window of L pixels (no global
+//                     read), transforms it as n / L transforms of length L,
+//                     writes the row (n^2 x 8 B at most: rows of pixel
+//                     centres outside the polygon's y range are zero and
+//                     skipped -- with padding p, all but about 1/p of them)
 //   fft_cols_kernel   reads WCOLS adjacent columns (64-byte row segments) of
 //                     the rows written, as n / L transforms of length L over
 //                     the window of L rows that holds them, and writes the
@@ -325,52 +325,6 @@ __device__ __forceinline__ bool fft_zero_row(const double *__restrict__ r, int r
     return py < r[FB_YLO] || py > r[FB_YHI];
 }
 
-// Raster + row FFTs.  Work unit = (beam, FFT_ROWS rows); LDS FFT_ROWS x
-// fft_ld(n) float2 plus FFT_ROWS x 32 doubles of per-row edge offsets.  Row
-// pixels are rasterised as stage 0 reads them.
-template <int LOGN>
-__global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
-    int n_beams, const double *__restrict__ rec, const float2 *__restrict__ tw,
-    float2 *__restrict__ grid)
-{
-    constexpr int N = 1 << LOGN;
-    extern __shared__ float2 lds[];
-    const int units_per_beam = N / FFT_ROWS;
-    const int unit = fft_logical_block();
-    if (unit >= n_beams * units_per_beam)
-        return;
-    const int beam = unit / units_per_beam;
-    const int row0 = (unit - beam * units_per_beam) * FFT_ROWS;
-    const double *r = rec + (size_t)beam * FB_STRIDE;
-    bool all_zero = true;
-    for (int rr = 0; rr < FFT_ROWS; ++rr)
-        all_zero = all_zero && fft_zero_row(r, row0 + rr, N);
-    if (all_zero)
-        return;
-    const int ne = (int)r[FB_NE];
-    const double xc = r[FB_XC], yc = r[FB_YC], dx = r[FB_DX], dy = r[FB_DY];
-    float2 *g = grid + ((size_t)beam * N + row0) * N;
-    // per row and edge, the y part b*py + c of the edge line (the same FMA
-    // the pixel test would do per pixel), after the sequences in LDS
-    double *off = reinterpret_cast<double *>(lds + FFT_ROWS * fft_ld(N));
-    for (int q = threadIdx.x; q < FFT_ROWS * ne; q += blockDim.x) {
-        const int rr = q / ne, e = q - rr * ne;
-        const double *l = r + FB_EDGE + 3 * e;
-        off[q] = __builtin_fma(l[1], fft_pixel(row0 + rr, N, dy, yc), l[2]);
-    }
-    __syncthreads();
-    auto raster = [&](int seq, int i) {
-        const double px = fft_pixel(i, N, dx, xc);
-        const double *o = off + seq * ne;
-        bool in = true;
-        for (int e = 0; e < ne; ++e)
-            in = in && (__builtin_fma(r[FB_EDGE + 3 * e], px, o[e]) > 0.0);
-        return make_float2(in ? 1.0f : 0.0f, 0.0f);
-    };
-    auto out = [&](int seq, int i, float2 x) { g[(size_t)seq * N + i] = x; };
-    fft_stages<LOGN, FFT_ROWS, FFT_ROW_THREADS, false, 0>(lds, fft_ld(N), tw, raster, out);
-}
-
 // Column FFTs over a row window.  Rows of pixel centres outside the
 // polygon's bounding box are zero, and with padding p those lie outside a
 // window of L = 2^LOGL <= N rows, [j0, j0 + L), L the power of two at or
@@ -407,21 +361,28 @@ __global__ void __launch_bounds__(FFT_ROW_THREADS) fft_rows_kernel(
 #endif
 enum { FFT_WCOLS = MBS_FFT_WCOLS, FFT_WCOL_THREADS = MBS_FFT_WCOL_THREADS };
 
-// Shortest window: 64 rows (LOGL >= 6 keeps at least two Stockham stages).
+// Shortest window: 64 rows (LOGL >= 6 keeps at least two Stockham stages),
+// and n / 8 (at most eight residues).
 enum { FFT_MIN_LOGL = 6 };
+__host__ __device__ constexpr int fft_min_logl(int logn)
+{
+    return logn - 3 > FFT_MIN_LOGL ? logn - 3 : FFT_MIN_LOGL;
+}
 
 // The row window of an n-point grid at padding pad: pixel centres at
 // j - (n - 1)/2 pixels from the box centre, the box n / (2 pad) pixels
 // either side of it, so rows lo..hi below can carry the polygon (with 1e-3
 // pixel to spare for rounding); L the power of two at or above hi - lo + 1
-// (64 at least, n at most) and j0 the window start, clamped into the grid.
+// (64 and n / 8 at least -- at most eight residues -- and n at most) and j0
+// the window start, clamped into the grid.  The columns that can carry the
+// polygon are the same window: the box is n / pad pixels wide either way.
 __host__ inline void fft_window(int n, double pad, int *j0, int *logl)
 {
     const double c = 0.5 * (n - 1), h = n / (2.0 * pad);
     const int lo = (int)ceil(c - h - 1e-3), hi = (int)floor(c + h + 1e-3);
     int lg = FFT_MIN_LOGL;
-    while ((1 << lg) < n && (1 << lg) < hi - lo + 1)
-        ++lg;
+    while ((1 << lg) < n && ((1 << lg) < hi - lo + 1 || (8 << lg) < n))
+        ++lg;  // ends at fft_min_logl(log2 n) or above
     const int l = 1 << lg;
     *logl = lg;
     *j0 = lo < 0 ? 0 : (lo > n - l ? n - l : lo);
@@ -492,6 +453,123 @@ __global__ void __launch_bounds__(FFT_WCOL_THREADS) fft_cols_kernel(
         __syncthreads();
         auto out = [&](int seq, int q, float2 v) { g[(size_t)(D * q + s) * N + seq] = v; };
         fft_stages<LOGL, C, T, true, 1, LR, LOGL, true>(lds, 0, twl, none, out);
+    }
+}
+
+// Raster + row FFTs over the column window [j0, j0 + L) (fft_window): as in
+// the column pass, each row's transform is D = N / L transforms of length L
+// of the rasterised window placed circularly (col mod L) and twiddled by
+// w^(col s).  Thread j keeps bins q = j + e L / R of every residue in
+// registers and stores them at the end: bins m = D q .. D q + D - 1 are
+// consecutive, so each wave's stores are contiguous, and no store is
+// outstanding.  Every global read happens before the first stage: the
+// window's twiddles w^col into registers, the L-point stage twiddles into
+// LDS (a dependent global read per stage and residue would put the load
+// latency on the critical path twelve times per row).  Work unit = (beam,
+// window row); rows outside the polygon's y range neither transform nor
+// store (the column pass reads them as zero).  One or two waves per row;
+// LDS fft_ld(L) + L float2 plus the row's edge offsets (b y + c per edge).
+#ifndef MBS_FFT_WROW_LOG_RADIX
+#define MBS_FFT_WROW_LOG_RADIX 3
+#endif
+// Threads per row: one first-stage butterfly each, 64 to 128.
+__host__ __device__ constexpr int fft_wrow_threads(int logl)
+{
+    return logl - MBS_FFT_WROW_LOG_RADIX <= 6 ? 64 : 128;
+}
+template <int LOGN, int LOGL>
+__global__ void __launch_bounds__(fft_wrow_threads(LOGL)) fft_rows_kernel(
+    int n_beams, int j0, const double *__restrict__ rec, const float2 *__restrict__ tw,
+    float2 *__restrict__ grid)
+{
+    constexpr int N = 1 << LOGN, L = 1 << LOGL, D = N / L;
+    constexpr int LR = LOGL < MBS_FFT_WROW_LOG_RADIX ? LOGL : MBS_FFT_WROW_LOG_RADIX;
+    constexpr int R = 1 << LR, NB = L / R;  // first-stage radix, butterflies
+    constexpr int T = fft_wrow_threads(LOGL);
+    constexpr int PB = (NB + T - 1) / T;
+    constexpr int LD = fft_ld(L);
+    static_assert(D <= 8, "at most eight residues (fft_window)");
+    extern __shared__ float2 lds[];
+    const int unit = fft_logical_block();
+    if (unit >= n_beams * L)
+        return;
+    const int beam = unit / L;
+    const int row = j0 + (unit - beam * L);
+    const double *r = rec + (size_t)beam * FB_STRIDE;
+    if (fft_zero_row(r, row, N))
+        return;
+    const int ne = (int)r[FB_NE];
+    const double xc = r[FB_XC], dx = r[FB_DX];
+    float2 *twl = lds + LD;  // w_L^i = w^(i D), i < L
+    double *off = reinterpret_cast<double *>(twl + L);
+    const double py = fft_pixel(row, N, r[FB_DY], r[FB_YC]);
+    for (int e = threadIdx.x; e < ne; e += T)
+        off[e] = __builtin_fma(r[FB_EDGE + 3 * e + 1], py, r[FB_EDGE + 3 * e + 2]);
+    {
+        static_assert(L % T == 0, "whole table rows per thread");
+        float2 t[L / T];  // all loads in flight before the first LDS write
+        for (int k = 0; k < L / T; ++k)
+            t[k] = tw[(threadIdx.x + k * T) * D];
+        for (int k = 0; k < L / T; ++k)
+            twl[threadIdx.x + k * T] = t[k];
+    }
+    __syncthreads();
+    // the rasterised window in first-stage order (element p = j + e NB is
+    // column j0 + ((p - j0) mod L)); y = raster w^(col s), wr = w^col
+    float2 y[PB][R], wr[PB][R];
+    for (int b = 0; b < PB; ++b) {
+        const int j = (threadIdx.x + b * T) % NB;  // lanes past the end repeat
+        for (int e = 0; e < R; ++e) {
+            const int col = j0 + ((j + e * NB - j0) & (L - 1));
+            wr[b][e] = tw[col];
+            const double px = fft_pixel(col, N, dx, xc);
+            bool in = true;
+            for (int q = 0; q < ne; ++q)
+                in = in && (__builtin_fma(r[FB_EDGE + 3 * q], px, off[q]) > 0.0);
+            y[b][e] = make_float2(in ? 1.0f : 0.0f, 0.0f);
+        }
+    }
+    float2 o[D][PB][R];
+    auto none = [&](int, int) { return make_float2(0.0f, 0.0f); };
+    auto keep = [&](int, int i, float2 x) { lds[fft_pad(i)] = x; };
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+        for (int b = 0; b < PB; ++b) {
+            const int j = threadIdx.x + b * T;
+            if (j >= NB)
+                break;
+            float2 v[R];
+            for (int e = 0; e < R; ++e) {
+                v[e] = y[b][e];
+                if (s + 1 < D)
+                    y[b][e] = cmulf(y[b][e], wr[b][e]);
+            }
+            fft_dft<R>(v);
+            for (int e = 0; e < R; ++e)
+                lds[fft_pad(j * R + e)] = v[e];
+        }
+        __syncthreads();
+        fft_stages<LOGL, 1, T, false, 1, LR, LOGL>(lds, LD, twl, none, keep);
+        for (int b = 0; b < PB; ++b)
+            for (int e = 0; e < R; ++e)
+                o[s][b][e] = lds[fft_pad((threadIdx.x + b * T) % NB + e * NB)];
+        __syncthreads();
+    }
+    float2 *g = grid + ((size_t)beam * N + row) * N;
+    for (int b = 0; b < PB; ++b) {
+        const int j = threadIdx.x + b * T;
+        if (j >= NB)
+            break;
+        for (int e = 0; e < R; ++e) {
+            float2 *dst = g + D * (j + e * NB);
+            if constexpr (D == 1) {
+                dst[0] = o[0][b][e];
+            } else {
+                for (int s = 0; s < D; s += 2)
+                    *reinterpret_cast<float4 *>(dst + s) =
+                        make_float4(o[s][b][e].x, o[s][b][e].y, o[s + 1][b][e].x, o[s + 1][b][e].y);
+            }
+        }
     }
 }
 

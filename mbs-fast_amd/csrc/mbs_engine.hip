@@ -1287,52 +1287,42 @@ int mbs_get_timing(mbs_ctx *ctx, mbs_timing *out)
 
 }  // extern "C"
 
-// LDS of the aperture FFT's row and column workgroups.
-static size_t fft_rows_lds(int n)
+// LDS of the aperture FFT's row and column workgroups for a window of
+// 2^logl.
+static size_t fft_rows_lds(int logl)
 {
-    return (size_t)FFT_ROWS * fft_ld(n) * sizeof(float2) +
-           (size_t)FFT_ROWS * MBS_MAX_VERTICES * sizeof(double);
+    return (size_t)(fft_ld(1 << logl) + (1 << logl)) * sizeof(float2) +
+           (size_t)MBS_MAX_VERTICES * sizeof(double);
 }
 static size_t fft_cols_lds(int logl) { return (size_t)(FFT_WCOLS + 1) * ((size_t)1 << logl) * sizeof(float2); }
 
-// Column pass over the row window [j0, j0 + 2^logl): the instance of the
-// window length, FFT_MIN_LOGL <= logl <= LOGN.
+// Row and column passes over the window [j0, j0 + 2^logl): the instance of
+// the window length, fft_min_logl(LOGN) <= logl <= LOGN.
 template <int LOGN, int LOGL>
-static int launch_fft_cols(mbs_ctx *ctx, int nb, int j0, int logl)
+static int launch_fft(mbs_ctx *ctx, int nb, int j0, int logl)
 {
     if constexpr (LOGL < LOGN)
         if (logl != LOGL)
-            return launch_fft_cols<LOGN, LOGL + 1>(ctx, nb, j0, logl);
-    constexpr int N = 1 << LOGN;
-    const size_t lds = fft_cols_lds(LOGL);
-    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<LOGN, LOGL>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    const int cb = (nb * (N / FFT_WCOLS) + 7) / 8 * 8;
-    fft_cols_kernel<LOGN, LOGL><<<cb, FFT_WCOL_THREADS, lds, ctx->stream>>>(
-        nb, j0, ctx->fft_rec.p, ctx->fft_tw.p, ctx->fft_grid.p);
-    HIP_TRY(ctx, hipGetLastError());
-    return MBS_OK;
-}
-
-template <int LOGN>
-static int launch_fft(mbs_ctx *ctx, int nb, int j0, int logl)
-{
-    constexpr int N = 1 << LOGN;
-    const size_t lds_rows = fft_rows_lds(N);
-    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<LOGN>,
+            return launch_fft<LOGN, LOGL + 1>(ctx, nb, j0, logl);
+    constexpr int N = 1 << LOGN, L = 1 << LOGL;
+    const size_t lds_rows = fft_rows_lds(LOGL), lds_cols = fft_cols_lds(LOGL);
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_rows_kernel<LOGN, LOGL>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_rows));
+    HIP_TRY(ctx, hipFuncSetAttribute((const void *)fft_cols_kernel<LOGN, LOGL>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_cols));
     hipStream_t s = ctx->stream;
-    const int rb = (nb * (N / FFT_ROWS) + 7) / 8 * 8;
+    const int rb = (nb * L + 7) / 8 * 8;
+    const int cb = (nb * (N / FFT_WCOLS) + 7) / 8 * 8;
     if (ctx->timing)
         HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[0], s));
-    fft_rows_kernel<LOGN><<<rb, FFT_ROW_THREADS, lds_rows, s>>>(nb, ctx->fft_rec.p, ctx->fft_tw.p,
-                                                               ctx->fft_grid.p);
+    fft_rows_kernel<LOGN, LOGL><<<rb, fft_wrow_threads(LOGL), lds_rows, s>>>(
+        nb, j0, ctx->fft_rec.p, ctx->fft_tw.p, ctx->fft_grid.p);
     HIP_TRY(ctx, hipGetLastError());
     if (ctx->timing)
         HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[1], s));
-    const int rc = launch_fft_cols<LOGN, FFT_MIN_LOGL>(ctx, nb, j0, logl);
-    if (rc)
-        return rc;
+    fft_cols_kernel<LOGN, LOGL><<<cb, FFT_WCOL_THREADS, lds_cols, s>>>(
+        nb, j0, ctx->fft_rec.p, ctx->fft_tw.p, ctx->fft_grid.p);
+    HIP_TRY(ctx, hipGetLastError());
     if (ctx->timing)
         HIP_TRY(ctx, hipEventRecord(ctx->fft_ev[2], s));
     return MBS_OK;
@@ -1603,7 +1593,7 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
                                        ctx->device));
     int j0 = 0, logl = 0;
     fft_window(n, pad, &j0, &logl);
-    const size_t lds_need = std::max(fft_rows_lds(n), fft_cols_lds(logl));
+    const size_t lds_need = std::max(fft_rows_lds(logl), fft_cols_lds(logl));
     if ((size_t)limit < lds_need)
         return fail(ctx, MBS_ERR_CAPACITY, "FFT size %d needs %zu bytes of LDS per workgroup "
                     "(device limit %d)", n, lds_need, limit);
@@ -1628,12 +1618,12 @@ int mbs_aperture_fft(mbs_ctx *ctx, const int64_t *beams, int32_t n_beams, int32_
                 HIP_TRY(ctx, hipEventCreate(&e));
     int rc = MBS_OK;
     switch (logn) {
-    case 6: rc = launch_fft<6>(ctx, n_beams, j0, logl); break;
-    case 7: rc = launch_fft<7>(ctx, n_beams, j0, logl); break;
-    case 8: rc = launch_fft<8>(ctx, n_beams, j0, logl); break;
-    case 9: rc = launch_fft<9>(ctx, n_beams, j0, logl); break;
-    case 10: rc = launch_fft<10>(ctx, n_beams, j0, logl); break;
-    case 11: rc = launch_fft<11>(ctx, n_beams, j0, logl); break;
+    case 6: rc = launch_fft<6, fft_min_logl(6)>(ctx, n_beams, j0, logl); break;
+    case 7: rc = launch_fft<7, fft_min_logl(7)>(ctx, n_beams, j0, logl); break;
+    case 8: rc = launch_fft<8, fft_min_logl(8)>(ctx, n_beams, j0, logl); break;
+    case 9: rc = launch_fft<9, fft_min_logl(9)>(ctx, n_beams, j0, logl); break;
+    case 10: rc = launch_fft<10, fft_min_logl(10)>(ctx, n_beams, j0, logl); break;
+    case 11: rc = launch_fft<11, fft_min_logl(11)>(ctx, n_beams, j0, logl); break;
     default: return fail(ctx, MBS_ERR_INVALID, "FFT size %d", n);
     }
     if (rc)
