@@ -477,8 +477,16 @@ __host__ __device__ constexpr int fft_wrow_threads(int logl)
 {
     return logl - MBS_FFT_WROW_LOG_RADIX <= 6 ? 64 : 128;
 }
+// Registers held to four waves per SIMD up to four residues: the row pass
+// of 64 padded 2048^2 grids took 0.20 ms, against 0.22 ms at the 132 VGPRs
+// the compiler takes unbounded (three waves) and 0.24 ms at five (spills).
+// Eight residues would spill at four waves and are left unbounded.
+#ifndef MBS_FFT_WROW_WAVES
+#define MBS_FFT_WROW_WAVES 4
+#endif
 template <int LOGN, int LOGL>
-__global__ void __launch_bounds__(fft_wrow_threads(LOGL)) fft_rows_kernel(
+__global__ void __launch_bounds__(fft_wrow_threads(LOGL),
+                                  LOGN - LOGL <= 2 ? MBS_FFT_WROW_WAVES : 1) fft_rows_kernel(
     int n_beams, int j0, const double *__restrict__ rec, const float2 *__restrict__ tw,
     float2 *__restrict__ grid)
 {
