@@ -592,21 +592,21 @@ def test_aperture_fft_on_device(pkg, oracle, ctx, n):
         ctx.aperture_fft([batch.n_beams], 256)
 
 
-@pytest.mark.parametrize("pad", [1.0, 1.5, 3.0, 6.0])
-def test_aperture_fft_row_window(pkg, oracle, ctx, pad):
-    """The column pass transforms only the window of rows that can carry the
-    polygon: n / pad rows rounded up to a power of two, as n / L transforms of
-    length L.  Padding 1, 1.5, 3 and 6 at n = 256 give L = 256, 256, 128 and
-    64 (one, one, two and four residues); every spectrum against the oracle's
-    FP64 FFT of the identical raster, and the sampled values to 1e-5 of the
+@pytest.mark.parametrize("n,pad", [(256, 1.0), (256, 1.5), (256, 3.0), (256, 6.0), (512, 8.0)])
+def test_aperture_fft_row_window(pkg, oracle, ctx, n, pad):
+    """Both passes transform only the window of rows (columns) that can carry
+    the polygon: n / pad rounded up to a power of two, at least n / 8, as
+    n / L transforms of length L.  Padding 1, 1.5, 3 and 6 at n = 256 give
+    L = 256, 256, 128 and 64 (one, one, two and four residues), padding 8 at
+    n = 512 gives L = 64 (eight); every spectrum against the oracle's FP64
+    FFT of the identical raster, and the sampled values to 1e-5 of the
     aperture area."""
     fx = load_fixture("sobol_average")
     grid, batch, consts = _setup(pkg, fx)
     ctx.set_grid(grid)
     ctx.upload(batch, consts)
     k = consts.wave_index
-    n = 256
-    rng = np.random.default_rng(int(pad * 10))
+    rng = np.random.default_rng(int(pad * 10) + n)
     beams = rng.choice(batch.n_beams, 6, replace=False)
     ab = rng.uniform(-0.02, 0.02, (16, 2))
     got = ctx.aperture_fft(beams, n, pad=pad, ab=ab)
@@ -619,8 +619,8 @@ def test_aperture_fft_row_window(pkg, oracle, ctx, pad):
         _, (_, _, dx, dy) = oracle.aperture_raster(vx, vy, n, pad)
         worst_s = max(worst_s, np.linalg.norm(S - X) / np.linalg.norm(X))
         worst_v = max(worst_v, np.max(np.abs(got[i] - want)) / (abs(X[0, 0]) * dx * dy))
-    print("pad %.1f: worst spectrum rel L2 %.2e, sampled max |d|/area %.2e"
-          % (pad, worst_s, worst_v))
+    print("n %d pad %.1f: worst spectrum rel L2 %.2e, sampled max |d|/area %.2e"
+          % (n, pad, worst_s, worst_v))
     assert worst_s < 1e-5
     assert worst_v < 1e-5
 
