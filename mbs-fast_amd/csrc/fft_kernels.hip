@@ -461,8 +461,9 @@ __global__ void __launch_bounds__(FFT_WCOL_THREADS) fft_cols_kernel(
 // of the rasterised window placed circularly (col mod L) and twiddled by
 // w^(col s).  Thread j keeps bins q = j + e L / R of every residue in
 // registers and stores them at the end: bins m = D q .. D q + D - 1 are
-// consecutive, so each wave's stores are contiguous, and no store is
-// outstanding.  Every global read happens before the first stage: the
+// consecutive, so each wave's stores are contiguous (storing each residue's
+// bins as it finishes, a stride-D scatter, took the row pass of 64 padded
+// 2048^2 grids from 0.21 to 0.44 ms), and no store is outstanding.  Every global read happens before the first stage: the
 // window's twiddles w^col into registers, the L-point stage twiddles into
 // LDS (a dependent global read per stage and residue would put the load
 // latency on the critical path twelve times per row).  Work unit = (beam,
